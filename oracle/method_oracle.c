@@ -1,0 +1,898 @@
+/*
+ * CPU oracle for the METHOD time-step hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Serial C restatement of the reference algorithm; every function names the
+ * reference file:line it follows (paths relative to Project/CPU/ of
+ * AlexJamesWright/METHOD). Only tests/, the smoke check and bench.py's CPU
+ * baseline may load this library, and only as the checker.
+ *
+ * What pins each part (besides the reference runs kept as fixtures under
+ * tests/golden/, which compare every field after whole steps):
+ *   weno3/5        exact on linear data, reference KATs (test_weno.cc);
+ *                  brute force against the Jiang-Shu closed form.
+ *   flux vectors   stationary state -> zero net flux; x/y/z symmetry.
+ *   primsToAll     closed form (test_srmhd.cc:228-282).
+ *   C2P            round trip cons(prims) -> prims to 1e-12, residual at the
+ *                  root, hydro (B = 0) and rest (v = 0) special cases.
+ *   BCs            index patterns (test_boundaryConds.cc).
+ *   dt             closed form (test_simulation.cc:34-72).
+ */
+#include "method_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ sizes */
+
+typedef struct {
+  int Nx, Ny, Nz, dims;
+  int is, ie, js, je, ks, ke;
+  long ntot;
+  double dx, dy, dz;
+} geom;
+
+/* Data::initData (simData.cc:61-97) and Simulation::Simulation (simulation.cc:34-36). */
+static geom mkgeom(const oc_config *c) {
+  geom g;
+  g.Nx = c->nx + 2 * c->Ng;
+  g.Ny = c->ny + 2 * c->Ng;
+  g.Nz = c->nz + 2 * c->Ng;
+  g.dims = 3;
+  double zmin = c->zmin, zmax = c->zmax, ymin = c->ymin, ymax = c->ymax;
+  if (c->nz == 0) {
+    g.Nz = 1;
+    zmin = -1e20;
+    zmax = 1e20;
+    g.dims = 2;
+  }
+  if (c->ny == 0) {
+    g.Nz = g.Ny = 1;
+    zmin = ymin = -1e20;
+    zmax = ymax = 1e20;
+    g.dims = 1;
+  }
+  g.is = c->Ng; g.ie = g.Nx - c->Ng;
+  g.js = c->Ng; g.je = g.Ny - c->Ng;
+  g.ks = c->Ng; g.ke = g.Nz - c->Ng;
+  if (g.dims < 3) { g.ks = 0; g.ke = 1; }
+  if (g.dims < 2) { g.js = 0; g.je = 1; }
+  g.ntot = (long)g.Nx * g.Ny * g.Nz;
+  g.dx = (c->xmax - c->xmin) / c->nx;
+  g.dy = (ymax - ymin) / c->ny;
+  g.dz = (zmax - zmin) / c->nz;
+  return g;
+}
+
+#define IX(v, i, j, k) ((long)(v) * g.ntot + ((long)(i) * g.Ny + (j)) * g.Nz + (k))
+
+int oc_ncons(int model) { return model == OC_SRMHD ? 9 : 14; }
+int oc_nprims(int model) { return model == OC_SRMHD ? 8 : 11; }
+int oc_naux(int model) { return model == OC_SRMHD ? 13 : 17; }
+
+/* -------------------------------------------------------------------- BCs */
+
+/* Periodic / Outflow / Flow::apply (boundaryConds.cc:5-145,449-730): per-axis
+ * copies, x first, then y when Ny > 1, then z when Nz > 1. */
+void oc_apply_bcs(const oc_config *c, double *a, int nvar) {
+  geom g = mkgeom(c);
+  int Ng = c->Ng, nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int v = 0; v < nvar; v++)
+    for (int i = 0; i < Ng; i++)
+      for (int j = 0; j < g.Ny; j++)
+        for (int k = 0; k < g.Nz; k++) {
+          if (c->periodic[0]) {
+            a[IX(v, i, j, k)] = a[IX(v, nx + i, j, k)];
+            a[IX(v, nx + Ng + i, j, k)] = a[IX(v, Ng + i, j, k)];
+          } else {
+            a[IX(v, i, j, k)] = a[IX(v, Ng, j, k)];
+            a[IX(v, nx + Ng + i, j, k)] = a[IX(v, nx + Ng - 1, j, k)];
+          }
+        }
+  if (g.Ny > 1)
+    for (int v = 0; v < nvar; v++)
+      for (int i = 0; i < g.Nx; i++)
+        for (int j = 0; j < Ng; j++)
+          for (int k = 0; k < g.Nz; k++) {
+            if (c->periodic[1]) {
+              a[IX(v, i, j, k)] = a[IX(v, i, ny + j, k)];
+              a[IX(v, i, ny + Ng + j, k)] = a[IX(v, i, Ng + j, k)];
+            } else {
+              a[IX(v, i, j, k)] = a[IX(v, i, Ng, k)];
+              a[IX(v, i, ny + Ng + j, k)] = a[IX(v, i, ny + Ng - 1, k)];
+            }
+          }
+  if (g.Nz > 1)
+    for (int v = 0; v < nvar; v++)
+      for (int i = 0; i < g.Nx; i++)
+        for (int j = 0; j < g.Ny; j++)
+          for (int k = 0; k < Ng; k++) {
+            if (c->periodic[2]) {
+              a[IX(v, i, j, k)] = a[IX(v, i, j, nz + k)];
+              a[IX(v, i, j, nz + Ng + k)] = a[IX(v, i, j, Ng + k)];
+            } else {
+              a[IX(v, i, j, k)] = a[IX(v, i, j, Ng)];
+              a[IX(v, i, j, nz + Ng + k)] = a[IX(v, i, j, nz + Ng - 1)];
+            }
+          }
+}
+
+static void apply_bcs_all(const oc_config *c, double *cons, double *prims, double *aux) {
+  oc_apply_bcs(c, cons, oc_ncons(c->model));
+  oc_apply_bcs(c, prims, oc_nprims(c->model));
+  oc_apply_bcs(c, aux, oc_naux(c->model));
+}
+
+/* ----------------------------------------------------------------- SRMHD */
+
+/* SRMHD::primsToAll (srmhd.cc:573-673). */
+static void srmhd_prims_to_all(const oc_config *c, double *cons, double *prims, double *aux) {
+  geom g = mkgeom(c);
+  double gam = c->gamma;
+  for (int i = 0; i < g.Nx; i++)
+    for (int j = 0; j < g.Ny; j++)
+      for (int k = 0; k < g.Nz; k++) {
+#define P(v) prims[IX(v, i, j, k)]
+#define U(v) cons[IX(v, i, j, k)]
+#define A(v) aux[IX(v, i, j, k)]
+        U(5) = P(5);
+        U(6) = P(6);
+        U(7) = P(7);
+        A(11) = P(5) * P(5) + P(6) * P(6) + P(7) * P(7);
+        U(8) = 0;
+        A(9) = P(1) * P(1) + P(2) * P(2) + P(3) * P(3);
+        A(1) = 1.0 / sqrt(1 - A(9));
+        A(4) = A(1) * (P(1) * P(5) + P(2) * P(6) + P(3) * P(7));
+        A(5) = P(5) / A(1) + A(4) * P(1);
+        A(6) = P(6) / A(1) + A(4) * P(2);
+        A(7) = P(7) / A(1) + A(4) * P(3);
+        A(8) = (P(5) * P(5) + P(6) * P(6) + P(7) * P(7) + A(4) * A(4)) / (A(1) * A(1));
+        A(0) = 1 + P(4) / P(0) * (gam / (gam - 1));
+        A(2) = P(4) / (P(0) * (gam - 1));
+        A(3) = sqrt(A(2) * gam * (gam - 1) / A(0));
+        U(0) = P(0) * A(1);
+        U(1) = (P(0) * A(0) + A(8)) * A(1) * A(1) * P(1) - A(4) * A(5);
+        U(2) = (P(0) * A(0) + A(8)) * A(1) * A(1) * P(2) - A(4) * A(6);
+        U(3) = (P(0) * A(0) + A(8)) * A(1) * A(1) * P(3) - A(4) * A(7);
+        A(12) = U(1) * U(1) + U(2) * U(2) + U(3) * U(3);
+        A(10) = P(5) * U(1) + P(6) * U(2) + P(7) * U(3);
+        U(4) = (P(0) * A(0) + A(8)) * A(1) * A(1) - (P(4) + A(8) / 2.0) - A(4) * A(4) - U(0);
+#undef P
+#undef U
+#undef A
+      }
+}
+
+/* SRMHD::fluxVector (srmhd.cc:72-206). With quirks on, the z-direction flux
+ * of phi is By as at srmhd.cc:200; off, it is Bz. */
+static void srmhd_flux_vector(const oc_config *c, const double *cons, const double *prims, const double *aux,
+                              double *f, int dir) {
+  geom g = mkgeom(c);
+  int vn = 1 + dir;      /* normal velocity prim index */
+  int bn = 5 + dir;      /* normal B prim index */
+  for (int i = 0; i < g.Nx; i++)
+    for (int j = 0; j < g.Ny; j++)
+      for (int k = 0; k < g.Nz; k++) {
+#define P(v) prims[IX(v, i, j, k)]
+#define U(v) cons[IX(v, i, j, k)]
+#define A(v) aux[IX(v, i, j, k)]
+#define F(v) f[IX(v, i, j, k)]
+        double vdir = P(vn), Bdir = P(bn), W = A(1);
+        F(0) = U(0) * vdir;
+        for (int m = 0; m < 3; m++) {
+          double val = U(1 + m) * vdir;
+          /* the x form groups (p + bsq/2), the y and z forms add left to right */
+          if (m == dir) val = (dir == 0) ? val + (P(4) + A(8) * 0.5) : val + P(4) + A(8) * 0.5;
+          F(1 + m) = val - A(5 + m) * Bdir / W;
+        }
+        F(4) = (U(4) + P(4) + A(8) * 0.5) * vdir - A(4) * Bdir / W;
+        for (int m = 0; m < 3; m++) {
+          if (m == dir)
+            F(5 + m) = U(8);
+          else
+            F(5 + m) = P(5 + m) * vdir - Bdir * P(1 + m);
+        }
+        F(8) = (dir == 2 && c->quirks) ? P(6) : Bdir;
+#undef P
+#undef U
+#undef A
+#undef F
+      }
+}
+
+/* SRMHD::sourceTerm (srmhd.cc:233-250). */
+static void srmhd_source(const oc_config *c, const double *cons, double *source) {
+  geom g = mkgeom(c);
+  for (int v = 0; v < 9; v++)
+    for (long n = 0; n < g.ntot; n++)
+      source[v * g.ntot + n] = (v == 8) ? -cons[8 * g.ntot + n] / (c->cp * c->cp) : 0.0;
+}
+
+/* Residual of srmhd.cc:269-295 and its analytic Jacobian; x[0] = v^2, x[1] = Z. */
+static void srmhd_residual(double D, double tau, double Bsq, double Ssq, double BS, double gam,
+                           const double *x, double *f, double *J) {
+  double v2 = x[0], Z = x[1];
+  double s = 1.0 - v2, sq = sqrt(s);
+  double gf = (gam - 1.0) / gam;
+  double p = gf * (Z * s - D * sq);
+  double ZB = Z + Bsq, BS2 = BS * BS, iZ2 = 1.0 / (Z * Z);
+  f[0] = ZB * ZB * v2 - (2.0 * Z + Bsq) * BS2 * iZ2 - Ssq;
+  f[1] = ZB - p - 0.5 * Bsq * s - 0.5 * BS2 * iZ2 - D - tau;
+  if (J) {
+    double BS2iZ3 = BS2 * iZ2 / Z;
+    J[0] = ZB * ZB;
+    J[1] = 2.0 * ZB * (v2 + BS2iZ3);
+    J[2] = gf * (Z - 0.5 * D / sq) + 0.5 * Bsq;
+    J[3] = 1.0 - gf * s + BS2iZ3;
+  }
+}
+
+#define SRMHD_NEWTON_TOL 1e-12
+#define SRMHD_NEWTON_MAXIT 60
+
+/* Newton on the 2x2 C2P system of srmhd.cc:269-295 (the reference solves the
+ * same system with cminpack hybrd1, srmhd.cc:446-447). Converged when the
+ * scaled residual max(|f0|/(Z+B^2)^2, |f1|/(Z+B^2)) <= 1e-12. Steps that leave
+ * the physical region (v^2 in [0,1), Z > 0) are pulled back half way. */
+int oc_srmhd_newton(double D, double Sx, double Sy, double Sz, double tau, double Bx, double By, double Bz,
+                    double gamma, double *x) {
+  double BS = Bx * Sx + By * Sy + Bz * Sz;
+  double Bsq = Bx * Bx + By * By + Bz * Bz;
+  double Ssq = Sx * Sx + Sy * Sy + Sz * Sz;
+  if (!(x[0] >= 0.0 && x[0] < 1.0)) x[0] = 0.0;
+  if (!(x[1] > 0.0)) x[1] = D + tau + Bsq;
+  for (int it = 0; it <= SRMHD_NEWTON_MAXIT; it++) {
+    double f[2], J[4];
+    srmhd_residual(D, tau, Bsq, Ssq, BS, gamma, x, f, J);
+    double ZB = x[1] + Bsq;
+    double r = fmax(fabs(f[0]) / (ZB * ZB), fabs(f[1]) / ZB);
+    if (r <= SRMHD_NEWTON_TOL) return it;
+    if (!(r == r) || it == SRMHD_NEWTON_MAXIT) break;
+    double det = J[0] * J[3] - J[1] * J[2];
+    double d0 = (J[3] * f[0] - J[1] * f[1]) / det;
+    double d1 = (J[0] * f[1] - J[2] * f[0]) / det;
+    double v2 = x[0] - d0, Z = x[1] - d1;
+    if (v2 < 0.0) v2 = 0.5 * x[0];
+    if (v2 >= 1.0) v2 = 0.5 * (x[0] + 1.0);
+    if (Z <= 0.0) Z = 0.5 * x[1];
+    x[0] = v2;
+    x[1] = Z;
+  }
+  return -1;
+}
+
+/* SRMHD::getPrimitiveVars (srmhd.cc:390-562): guess (v^2, rho h / (1 - v^2))
+ * from the old prims/aux (srmhd.cc:438-442), root, then the recovery formulas
+ * of srmhd.cc:509-557. A cell whose solve fails is retried from the mean
+ * root of its converged interior face neighbours (srmhd.cc:465-507; the
+ * reference uses the last cell's arguments there, which is reproduced with
+ * quirks on). */
+static long srmhd_c2p(const oc_config *c, const double *cons, double *prims, double *aux, oc_stats *st) {
+  geom g = mkgeom(c);
+  double gam = c->gamma;
+  double *sol = (double *)malloc(sizeof(double) * 2 * g.ntot);
+  char *ok = (char *)calloc(g.ntot, 1);
+  long fails = 0;
+  int li = g.ie - 1, lj = g.je - 1, lk = g.ke - 1; /* last interior cell */
+  for (int i = g.is; i < g.ie; i++)
+    for (int j = g.js; j < g.je; j++)
+      for (int k = g.ks; k < g.ke; k++) {
+        long n = ((long)i * g.Ny + j) * g.Nz + k;
+#define P(v) prims[IX(v, i, j, k)]
+#define U(v) cons[IX(v, i, j, k)]
+#define A(v) aux[IX(v, i, j, k)]
+        double x[2];
+        x[0] = P(1) * P(1) + P(2) * P(2) + P(3) * P(3);
+        x[1] = P(0) * A(0) / (1 - x[0]);
+        int its = oc_srmhd_newton(U(0), U(1), U(2), U(3), U(4), U(5), U(6), U(7), gam, x);
+        if (st) {
+          st->cells++;
+          if (its >= 0) {
+            st->iterations += its;
+            if (its > st->max_iterations) st->max_iterations = its;
+          }
+        }
+        if (its < 0) {
+          fails++;
+        } else {
+          ok[n] = 1;
+          sol[2 * n] = x[0];
+          sol[2 * n + 1] = x[1];
+        }
+#undef P
+#undef U
+#undef A
+      }
+  if (fails) {
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) {
+          long n = ((long)i * g.Ny + j) * g.Nz + k;
+          if (ok[n]) continue;
+          int nb[6][3] = {{i - 1, j, k}, {i, j - 1, k}, {i, j, k - 1}, {i + 1, j, k}, {i, j + 1, k}, {i, j, k + 1}};
+          double x[2] = {0.0, 0.0};
+          int cnt = 0;
+          for (int q = 0; q < 6; q++) {
+            int a = nb[q][0], b = nb[q][1], d = nb[q][2];
+            if (a < g.is || a >= g.ie || b < g.js || b >= g.je || d < g.ks || d >= g.ke) continue;
+            long m = ((long)a * g.Ny + b) * g.Nz + d;
+            if (!ok[m]) continue;
+            x[0] += sol[2 * m];
+            x[1] += sol[2 * m + 1];
+            cnt++;
+          }
+          if (cnt) {
+            x[0] /= cnt;
+            x[1] /= cnt;
+          }
+          int si = c->quirks ? li : i, sj = c->quirks ? lj : j, sk = c->quirks ? lk : k;
+          int its = oc_srmhd_newton(cons[IX(0, si, sj, sk)], cons[IX(1, si, sj, sk)], cons[IX(2, si, sj, sk)],
+                                    cons[IX(3, si, sj, sk)], cons[IX(4, si, sj, sk)], cons[IX(5, si, sj, sk)],
+                                    cons[IX(6, si, sj, sk)], cons[IX(7, si, sj, sk)], gam, x);
+          if (st && its < 0) st->failures++;
+          sol[2 * n] = x[0];
+          sol[2 * n + 1] = x[1];
+        }
+  }
+  for (int i = g.is; i < g.ie; i++)
+    for (int j = g.js; j < g.je; j++)
+      for (int k = g.ks; k < g.ke; k++) {
+        long n = ((long)i * g.Ny + j) * g.Nz + k;
+#define P(v) prims[IX(v, i, j, k)]
+#define U(v) cons[IX(v, i, j, k)]
+#define A(v) aux[IX(v, i, j, k)]
+        double s0 = sol[2 * n], s1 = sol[2 * n + 1];
+        P(5) = U(5);
+        P(6) = U(6);
+        P(7) = U(7);
+        A(10) = U(5) * U(1) + U(6) * U(2) + U(7) * U(3);
+        A(11) = U(5) * U(5) + U(6) * U(6) + U(7) * U(7);
+        A(12) = U(1) * U(1) + U(2) * U(2) + U(3) * U(3);
+        A(1) = 1 / sqrt(1 - s0);
+        P(0) = U(0) / A(1);
+        A(0) = s1 / (P(0) * A(1) * A(1));
+        P(4) = (A(0) - 1) * P(0) * (gam - 1) / gam;
+        A(2) = P(4) / (P(0) * (gam - 1));
+        P(1) = (U(5) * A(10) + U(1) * s1) / (s1 * (A(11) + s1));
+        P(2) = (U(6) * A(10) + U(2) * s1) / (s1 * (A(11) + s1));
+        P(3) = (U(7) * A(10) + U(3) * s1) / (s1 * (A(11) + s1));
+        A(9) = P(1) * P(1) + P(2) * P(2) + P(3) * P(3);
+        A(3) = sqrt(A(2) * gam * (gam - 1) / A(0));
+        A(4) = A(1) * (U(5) * P(1) + U(6) * P(2) + U(7) * P(3));
+        A(5) = U(5) / A(1) + A(4) * P(1);
+        A(6) = U(6) / A(1) + A(4) * P(2);
+        A(7) = U(7) / A(1) + A(4) * P(3);
+        A(8) = (P(5) * P(5) + P(6) * P(6) + P(7) * P(7) + A(4) * A(4)) / (A(1) * A(1));
+#undef P
+#undef U
+#undef A
+      }
+  if (st) st->failures += fails;
+  free(sol);
+  free(ok);
+  return fails;
+}
+
+/* ----------------------------------------------------------------- SRRMHD */
+
+#define SRRMHD_TOL 1.0e-12
+#define SRRMHD_EPS 1.0e-4
+#define SRRMHD_MAXITER 50
+
+/* Data::sigmaFunc for constant conductivity (simData.cc:114-119). */
+static double sigma_of(const oc_config *c) { return c->sigma; }
+
+/* SRRMHD::primsToAll (srrmhd.cc:358-510): qch from the 4th-order divergence
+ * of E where the stencil fits, zero (with J) elsewhere. */
+static void srrmhd_prims_to_all(const oc_config *c, double *cons, double *prims, double *aux) {
+  geom g = mkgeom(c);
+  double gam = c->gamma;
+  for (int i = 0; i < g.Nx; i++)
+    for (int j = 0; j < g.Ny; j++)
+      for (int k = 0; k < g.Nz; k++) {
+#define P(v) prims[IX(v, i, j, k)]
+#define U(v) cons[IX(v, i, j, k)]
+#define A(v) aux[IX(v, i, j, k)]
+#define PE(v, a, b, d) prims[IX(v, a, b, d)]
+        U(11) = U(12) = 0.0;
+        U(5) = P(5);
+        U(6) = P(6);
+        U(7) = P(7);
+        U(8) = P(8);
+        U(9) = P(9);
+        U(10) = P(10);
+        A(7) = P(5) * P(5) + P(6) * P(6) + P(7) * P(7);
+        A(8) = P(8) * P(8) + P(9) * P(9) + P(10) * P(10);
+        A(9) = P(1) * P(1) + P(2) * P(2) + P(3) * P(3);
+        A(11) = P(1) * U(8) + P(2) * U(9) + P(3) * U(10);
+        A(1) = 1.0 / sqrt(1 - A(9));
+        U(0) = P(0) * A(1);
+        A(2) = P(4) / (P(0) * (gam - 1));
+        A(0) = 1 + A(2) * gam;
+        A(3) = sqrt((A(2) * gam * (gam - 1)) / A(0));
+        A(10) = P(0) * A(0) * A(1) * A(1);
+        double sigma = sigma_of(c);
+        int have = 0;
+        if (i > 1 && j > 1 && k > 1 && i < g.Nx - 2 && j < g.Ny - 2 && k < g.Nz - 2) {
+          U(13) = (-PE(8, i + 2, j, k) + 8 * PE(8, i + 1, j, k) - 8 * PE(8, i - 1, j, k) + PE(8, i - 2, j, k)) /
+                      (12 * g.dx) +
+                  (-PE(9, i, j + 2, k) + 8 * PE(9, i, j + 1, k) - 8 * PE(9, i, j - 1, k) + PE(9, i, j - 2, k)) /
+                      (12 * g.dy) +
+                  (-PE(10, i, j, k + 2) + 8 * PE(10, i, j, k + 1) - 8 * PE(10, i, j, k - 1) + PE(10, i, j, k - 2)) /
+                      (12 * g.dz);
+          have = 1;
+        } else if (i > 1 && j > 1 && i < g.Nx - 2 && j < g.Ny - 2 && c->nz == 0) {
+          U(13) = (-PE(8, i + 2, j, k) + 8 * PE(8, i + 1, j, k) - 8 * PE(8, i - 1, j, k) + PE(8, i - 2, j, k)) /
+                      (12 * g.dx) +
+                  (-PE(9, i, j + 2, k) + 8 * PE(9, i, j + 1, k) - 8 * PE(9, i, j - 1, k) + PE(9, i, j - 2, k)) /
+                      (12 * g.dy);
+          have = 1;
+        } else if (i > 1 && i < g.Nx - 2 && c->ny == 0) {
+          U(13) = (-PE(8, i + 2, j, k) + 8 * PE(8, i + 1, j, k) - 8 * PE(8, i - 1, j, k) + PE(8, i - 2, j, k)) /
+                  (12 * g.dx);
+          have = 1;
+        }
+        if (have) {
+          A(4) = U(13) * P(1) + A(1) * sigma * (U(8) + P(2) * U(7) - P(3) * U(6) - A(11) * P(1));
+          A(5) = U(13) * P(2) + A(1) * sigma * (U(9) + P(3) * U(5) - P(1) * U(7) - A(11) * P(2));
+          A(6) = U(13) * P(3) + A(1) * sigma * (U(10) + P(1) * U(6) - P(2) * U(5) - A(11) * P(3));
+        } else {
+          U(13) = A(4) = A(5) = A(6) = 0.0;
+        }
+        U(1) = A(10) * P(1) + U(9) * U(7) - U(10) * U(6);
+        U(2) = A(10) * P(2) + U(10) * U(5) - U(8) * U(7);
+        U(3) = A(10) * P(3) + U(8) * U(6) - U(9) * U(5);
+        U(4) = A(10) - P(4) + 0.5 * (A(7) + A(8)) - U(0);
+        A(12) = A(10) * P(1);
+        A(13) = A(10) * P(2);
+        A(14) = A(10) * P(3);
+        A(15) = A(12) * A(12) + A(13) * A(13) + A(14) * A(14);
+        A(16) = A(10) - P(4) - U(0);
+#undef P
+#undef U
+#undef A
+#undef PE
+      }
+}
+
+/* SRRMHD::fluxVector (srrmhd.cc:65-192). With quirks on, the z-direction Sy
+ * flux is overwritten by the Sz formula and the Sz flux is left as whatever
+ * the caller's buffer held (srrmhd.cc:163-171): `f` then must hold the
+ * y-direction flux on entry, as the reference's shared buffer does. */
+static void srrmhd_flux_vector(const oc_config *c, const double *cons, const double *prims, const double *aux,
+                               double *f, int dir) {
+  geom g = mkgeom(c);
+  for (int i = 0; i < g.Nx; i++)
+    for (int j = 0; j < g.Ny; j++)
+      for (int k = 0; k < g.Nz; k++) {
+#define P(v) prims[IX(v, i, j, k)]
+#define U(v) cons[IX(v, i, j, k)]
+#define A(v) aux[IX(v, i, j, k)]
+#define F(v) f[IX(v, i, j, k)]
+        if (dir == 0) {
+          F(0) = U(0) * P(1);
+          F(1) = A(10) * P(1) * P(1) + P(4) - U(8) * U(8) - U(5) * U(5) + 0.5 * (A(7) + A(8));
+          F(2) = A(10) * P(1) * P(2) - U(8) * U(9) - U(5) * U(6);
+          F(3) = A(10) * P(1) * P(3) - U(8) * U(10) - U(5) * U(7);
+          F(4) = U(1) - U(0) * P(1);
+          F(5) = U(12);
+          F(6) = -U(10);
+          F(7) = U(9);
+          F(8) = U(11);
+          F(9) = U(7);
+          F(10) = -U(6);
+          F(11) = U(8);
+          F(12) = U(5);
+          F(13) = A(4);
+        } else if (dir == 1) {
+          F(0) = U(0) * P(2);
+          F(1) = A(10) * P(1) * P(2) - U(9) * U(8) - U(6) * U(5);
+          F(2) = A(10) * P(2) * P(2) + P(4) - U(9) * U(9) - U(6) * U(6) + 0.5 * (A(7) + A(8));
+          F(3) = A(10) * P(2) * P(3) - U(9) * U(10) - U(6) * U(7);
+          F(4) = U(2) - U(0) * P(2);
+          F(5) = U(10);
+          F(6) = U(12);
+          F(7) = -U(8);
+          F(8) = -U(7);
+          F(9) = U(11);
+          F(10) = U(5);
+          F(11) = U(9);
+          F(12) = U(6);
+          F(13) = A(5);
+        } else {
+          F(0) = U(0) * P(3);
+          F(1) = A(10) * P(1) * P(3) - U(10) * U(8) - U(7) * U(5);
+          if (c->quirks) {
+            F(2) = A(10) * P(3) * P(3) + P(4) - U(10) * U(10) - U(7) * U(7) + 0.5 * (A(7) + A(8));
+          } else {
+            F(2) = A(10) * P(2) * P(3) - U(10) * U(9) - U(7) * U(6);
+            F(3) = A(10) * P(3) * P(3) + P(4) - U(10) * U(10) - U(7) * U(7) + 0.5 * (A(7) + A(8));
+          }
+          F(4) = U(3) - U(0) * P(3);
+          F(5) = -U(9);
+          F(6) = U(8);
+          F(7) = U(12);
+          F(8) = U(6);
+          F(9) = -U(5);
+          F(10) = U(11);
+          F(11) = U(10);
+          F(12) = U(7);
+          F(13) = A(6);
+        }
+#undef P
+#undef U
+#undef A
+#undef F
+      }
+}
+
+/* SRRMHD::sourceTermSingleCell (srrmhd.cc:194-207). */
+static void srrmhd_source_cell(double cp, const double *u, const double *a, double *s) {
+  s[0] = s[1] = s[2] = s[3] = s[4] = s[5] = s[6] = s[7] = s[13] = 0.0;
+  s[8] = -a[4];
+  s[9] = -a[5];
+  s[10] = -a[6];
+  s[11] = u[13] - u[11] / (cp * cp);
+  s[12] = -u[12] / (cp * cp);
+}
+
+/* Residual of the secant C2P (srrmhd.cc:512-537). */
+static double srrmhd_residual(double Z, double StildeSq, double D, double tauTilde, double gamma) {
+  double vsq, W, rho, h, p, resid;
+  vsq = StildeSq / (Z * Z);
+  if (vsq >= 1.0 || Z < 0) return 1.0e6;
+  W = 1 / sqrt(1 - vsq);
+  rho = D / W;
+  h = Z / (rho * W * W);
+  p = (gamma - 1) * (h - rho) / gamma;
+  if (rho < 0 || p < 0 || W < 1 || h < 0) return 1.0e6;
+  resid = (1 - (gamma - 1) / (W * W * gamma)) * Z + ((gamma - 1) / (W * gamma) - 1) * D - tauTilde;
+  return resid;
+}
+
+/* Secant iteration of srrmhd.cc:539-581. Returns iterations, or -1 when it
+ * fails (the reference throws there); Z holds the best iterate either way. */
+static int srrmhd_secant(double *Z, double StildeSq, double D, double tauTilde, double gamma) {
+  double x0 = *Z, x1 = x0 + SRRMHD_EPS, x2;
+  double f0 = srrmhd_residual(x0, StildeSq, D, tauTilde, gamma);
+  double f1 = srrmhd_residual(x1, StildeSq, D, tauTilde, gamma);
+  double bestX = x0, bestF = f0;
+  for (int iter = 0; iter < SRRMHD_MAXITER; iter++) {
+    if (fabs(f0) < SRRMHD_TOL) {
+      *Z = x0;
+      return iter;
+    }
+    x2 = x1 - f1 * (x1 - x0) / (f1 - f0);
+    x1 = x0;
+    x0 = x2;
+    f1 = f0;
+    f0 = srrmhd_residual(x0, StildeSq, D, tauTilde, gamma);
+    if (fabs(f0) < fabs(bestF)) {
+      bestX = x0;
+      bestF = f0;
+    }
+  }
+  *Z = bestX;
+  return -1;
+}
+
+/* SRRMHD::getPrimitiveVarsSingleCell (srrmhd.cc:256-312): a[10] holds the
+ * guess Z on entry. Returns secant iterations or -1. */
+static int srrmhd_c2p_cell(double gam, double sigma, const double *u, double *p, double *a) {
+  p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+  p[8] = u[8]; p[9] = u[9]; p[10] = u[10];
+  a[7] = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+  a[8] = u[8] * u[8] + u[9] * u[9] + u[10] * u[10];
+  a[12] = u[1] - (u[9] * u[7] - u[10] * u[6]);
+  a[13] = u[2] - (u[10] * u[5] - u[8] * u[7]);
+  a[14] = u[3] - (u[8] * u[6] - u[9] * u[5]);
+  a[15] = a[12] * a[12] + a[13] * a[13] + a[14] * a[14];
+  a[16] = u[4] - 0.5 * (a[7] + a[8]);
+  int its = srrmhd_secant(&a[10], a[15], u[0], a[16], gam);
+  a[9] = a[15] / (a[10] * a[10]);
+  a[1] = 1.0 / sqrt(1 - a[9]);
+  p[0] = u[0] / a[1];
+  a[0] = a[10] / (p[0] * a[1] * a[1]);
+  a[2] = (a[0] - 1) / gam;
+  a[3] = sqrt((a[2] * gam * (gam - 1)) / a[0]);
+  p[4] = p[0] * a[2] * (gam - 1);
+  p[1] = a[12] / a[10];
+  p[2] = a[13] / a[10];
+  p[3] = a[14] / a[10];
+  a[11] = p[1] * u[8] + p[2] * u[9] + p[3] * u[10];
+  a[4] = u[13] * p[1] + a[1] * sigma * (u[8] + (p[2] * u[7] - p[3] * u[6]) - a[11] * p[1]);
+  a[5] = u[13] * p[2] + a[1] * sigma * (u[9] + (p[3] * u[5] - p[1] * u[7]) - a[11] * p[2]);
+  a[6] = u[13] * p[3] + a[1] * sigma * (u[10] + (p[1] * u[6] - p[2] * u[5]) - a[11] * p[3]);
+  return its;
+}
+
+/* SRRMHD::getPrimitiveVars (srrmhd.cc:314-356): guess Z = aux[10] of the cell. */
+static long srrmhd_c2p(const oc_config *c, const double *cons, double *prims, double *aux, oc_stats *st) {
+  geom g = mkgeom(c);
+  long fails = 0;
+  double u[14], p[11], a[17];
+  for (int i = g.is; i < g.ie; i++)
+    for (int j = g.js; j < g.je; j++)
+      for (int k = g.ks; k < g.ke; k++) {
+        for (int v = 0; v < 14; v++) u[v] = cons[IX(v, i, j, k)];
+        a[10] = aux[IX(10, i, j, k)];
+        int its = srrmhd_c2p_cell(c->gamma, sigma_of(c), u, p, a);
+        if (st) {
+          st->cells++;
+          if (its >= 0) {
+            st->iterations += its;
+            if (its > st->max_iterations) st->max_iterations = its;
+          }
+        }
+        if (its < 0) fails++;
+        for (int v = 0; v < 11; v++) prims[IX(v, i, j, k)] = p[v];
+        for (int v = 0; v < 17; v++) aux[IX(v, i, j, k)] = a[v];
+      }
+  if (st) st->failures += fails;
+  return fails;
+}
+
+/* ------------------------------------------------------------ model API */
+
+void oc_prims_to_all(const oc_config *c, double *cons, double *prims, double *aux) {
+  if (c->model == OC_SRMHD)
+    srmhd_prims_to_all(c, cons, prims, aux);
+  else
+    srrmhd_prims_to_all(c, cons, prims, aux);
+}
+
+void oc_set(const oc_config *c, double *cons, double *prims, double *aux) {
+  oc_prims_to_all(c, cons, prims, aux);
+  apply_bcs_all(c, cons, prims, aux);
+}
+
+void oc_flux_vector(const oc_config *c, const double *cons, const double *prims, const double *aux, double *f,
+                    int dir) {
+  if (c->model == OC_SRMHD)
+    srmhd_flux_vector(c, cons, prims, aux, f, dir);
+  else
+    srrmhd_flux_vector(c, cons, prims, aux, f, dir);
+}
+
+void oc_source(const oc_config *c, const double *cons, const double *prims, const double *aux, double *source) {
+  (void)prims;
+  if (c->model == OC_SRMHD) {
+    srmhd_source(c, cons, source);
+    return;
+  }
+  geom g = mkgeom(c);
+  double u[14], a[17], s[14];
+  for (long n = 0; n < g.ntot; n++) {
+    for (int v = 0; v < 14; v++) u[v] = cons[v * g.ntot + n];
+    for (int v = 0; v < 17; v++) a[v] = aux[v * g.ntot + n];
+    srrmhd_source_cell(c->cp, u, a, s);
+    for (int v = 0; v < 14; v++) source[v * g.ntot + n] = s[v];
+  }
+}
+
+long oc_c2p(const oc_config *c, const double *cons, double *prims, double *aux, oc_stats *st) {
+  if (c->model == OC_SRMHD) return srmhd_c2p(c, cons, prims, aux, st);
+  return srrmhd_c2p(c, cons, prims, aux, st);
+}
+
+/* ------------------------------------------------------------------ WENO */
+
+/* weno3_upwind (wenoUpwinds.cc:5-33): Jiang-Shu weights alpha = d/(eps + beta^2). */
+double oc_weno3_upwind(double vec0, double vec1, double vec2) {
+  double alpha0 = 0, alpha1 = 0, beta0 = 0, beta1 = 0, stencil0 = 0, stencil1 = 0, w0, w1;
+  double epsilon = 1e-16, alphaSum;
+  beta0 += 1.0 * vec1 * vec1;
+  beta0 += -2.0 * vec0 * vec1;
+  beta0 += 1.0 * vec0 * vec0;
+  alpha0 = (1.0 / 3.0) / (epsilon + beta0 * beta0);
+  beta1 += 1.0 * vec2 * vec2;
+  beta1 += -2.0 * vec1 * vec2;
+  beta1 += 1.0 * vec1 * vec1;
+  alpha1 = (2.0 / 3.0) / (epsilon + beta1 * beta1);
+  alphaSum = alpha0 + alpha1;
+  w0 = alpha0 / alphaSum;
+  w1 = alpha1 / alphaSum;
+  stencil0 += 1.5 * vec1;
+  stencil0 += -0.5 * vec0;
+  stencil1 += 0.5 * vec2;
+  stencil1 += 0.5 * vec1;
+  return (w0 * stencil0 + w1 * stencil1);
+}
+
+/* weno5_upwind (wenoUpwinds.cc:36-84). The smoothness indicators are the
+ * Jiang-Shu forms 13/12 (.)^2 + 1/4 (.)^2 expanded into monomials, as the
+ * reference writes them; coefficients are the exact fractions. */
+double oc_weno5_upwind(double vec0, double vec1, double vec2, double vec3, double vec4) {
+  double alpha0, alpha1, alpha2, beta0 = 0, beta1 = 0, beta2 = 0, stencil0 = 0, stencil1 = 0, stencil2 = 0;
+  double w0, w1, w2, epsilon = 1e-16, alphaSum;
+  beta0 += (10.0 / 3.0) * vec2 * vec2;
+  beta0 += (-31.0 / 3.0) * vec1 * vec2;
+  beta0 += (25.0 / 3.0) * vec1 * vec1;
+  beta0 += (11.0 / 3.0) * vec0 * vec2;
+  beta0 += (-19.0 / 3.0) * vec0 * vec1;
+  beta0 += (4.0 / 3.0) * vec0 * vec0;
+  alpha0 = 0.1 / (epsilon + beta0 * beta0);
+  beta1 += (4.0 / 3.0) * vec3 * vec3;
+  beta1 += (-13.0 / 3.0) * vec2 * vec3;
+  beta1 += (13.0 / 3.0) * vec2 * vec2;
+  beta1 += (5.0 / 3.0) * vec1 * vec3;
+  beta1 += (-13.0 / 3.0) * vec1 * vec2;
+  beta1 += (4.0 / 3.0) * vec1 * vec1;
+  alpha1 = 0.6 / (epsilon + beta1 * beta1);
+  beta2 += (4.0 / 3.0) * vec4 * vec4;
+  beta2 += (-19.0 / 3.0) * vec3 * vec4;
+  beta2 += (25.0 / 3.0) * vec3 * vec3;
+  beta2 += (11.0 / 3.0) * vec2 * vec4;
+  beta2 += (-31.0 / 3.0) * vec2 * vec3;
+  beta2 += (10.0 / 3.0) * vec2 * vec2;
+  alpha2 = 0.3 / (epsilon + beta2 * beta2);
+  alphaSum = alpha0 + alpha1 + alpha2;
+  w0 = alpha0 / alphaSum;
+  w1 = alpha1 / alphaSum;
+  w2 = alpha2 / alphaSum;
+  stencil0 += (11.0 / 6.0) * vec2;
+  stencil0 += (-7.0 / 6.0) * vec1;
+  stencil0 += (1.0 / 3.0) * vec0;
+  stencil1 += (1.0 / 3.0) * vec3;
+  stencil1 += (5.0 / 6.0) * vec2;
+  stencil1 += (-1.0 / 6.0) * vec1;
+  stencil2 += (-1.0 / 6.0) * vec4;
+  stencil2 += (5.0 / 6.0) * vec3;
+  stencil2 += (1.0 / 3.0) * vec2;
+  return (w0 * stencil0 + w1 * stencil1 + w2 * stencil2);
+}
+
+/* Upwind/downwind stencil gathers of Weno3/Weno5 (weno.cc:123-229): the
+ * upwind value at index i uses arr[i-shift .. i-shift+order-1], the
+ * downwind one arr[i+shift-1 .. i+shift-order], over i in [shift, N-shift). */
+static double recon_up(int order, const double *a, long i0, long stride) {
+  int shift = (order + 1) / 2;
+  const double *p = a + i0 - shift * stride;
+  if (order == 3) return oc_weno3_upwind(p[0], p[stride], p[2 * stride]);
+  return oc_weno5_upwind(p[0], p[stride], p[2 * stride], p[3 * stride], p[4 * stride]);
+}
+
+static double recon_down(int order, const double *a, long i0, long stride) {
+  int shift = (order + 1) / 2;
+  const double *p = a + i0 + (shift - 1) * stride;
+  if (order == 3) return oc_weno3_upwind(p[0], p[-stride], p[-2 * stride]);
+  return oc_weno5_upwind(p[0], p[-stride], p[-2 * stride], p[-3 * stride], p[-4 * stride]);
+}
+
+/* FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56): Lax-Friedrichs
+ * split with alpha = 1, upwind WENO on f+, downwind on f-, summed; entries
+ * outside [shift, N-shift) along dir are zero. */
+void oc_flux_reconstruction(const oc_config *c, const double *cons, const double *f, double *frecon, int dir) {
+  geom g = mkgeom(c);
+  int nv = oc_ncons(c->model);
+  long n = g.ntot * nv;
+  double *fp = (double *)calloc(n, sizeof(double));
+  double *fm = (double *)calloc(n, sizeof(double));
+  const double alpha = 1.0;
+  for (long q = 0; q < n; q++) {
+    fp[q] = 0.5 * (f[q] + alpha * cons[q]);
+    fm[q] = 0.5 * (f[q] - alpha * cons[q]);
+  }
+  int order = c->weno_order, shift = (order + 1) / 2;
+  long stride = dir == 0 ? (long)g.Ny * g.Nz : (dir == 1 ? g.Nz : 1);
+  int N = dir == 0 ? g.Nx : (dir == 1 ? g.Ny : g.Nz);
+  memset(frecon, 0, sizeof(double) * n);
+  for (int v = 0; v < nv; v++)
+    for (int i = 0; i < g.Nx; i++)
+      for (int j = 0; j < g.Ny; j++)
+        for (int k = 0; k < g.Nz; k++) {
+          int pos = dir == 0 ? i : (dir == 1 ? j : k);
+          if (pos < shift || pos >= N - shift) continue;
+          long q = IX(v, i, j, k);
+          double up = recon_up(order, fp, q, stride);
+          double dn = recon_down(order, fm, q, stride);
+          frecon[q] = up + dn;
+        }
+  free(fp);
+  free(fm);
+}
+
+/* FVS::F (fluxVectorSplitting.cc:58-131): per direction fluxVector then
+ * reconstruction, fnet over interior = sum_dir (f[i+1]/d - f[i]/d). The flux
+ * vector buffer is shared between directions, as the reference's d->f is. */
+void oc_fvs_F(const oc_config *c, const double *cons, const double *prims, const double *aux, double *fnet) {
+  geom g = mkgeom(c);
+  int nv = oc_ncons(c->model);
+  long n = g.ntot * nv;
+  double *f = (double *)calloc(n, sizeof(double));
+  double *fr[3] = {NULL, NULL, NULL};
+  for (int dir = 0; dir < g.dims; dir++) {
+    fr[dir] = (double *)malloc(sizeof(double) * n);
+    oc_flux_vector(c, cons, prims, aux, f, dir);
+    oc_flux_reconstruction(c, cons, f, fr[dir], dir);
+  }
+  double dd[3] = {g.dx, g.dy, g.dz};
+  long st[3] = {(long)g.Ny * g.Nz, g.Nz, 1};
+  int Ng = c->Ng;
+  int i1 = g.Nx - Ng, j0 = g.dims >= 2 ? Ng : 0, j1 = g.dims >= 2 ? g.Ny - Ng : 1;
+  int k0 = g.dims == 3 ? Ng : 0, k1 = g.dims == 3 ? g.Nz - Ng : 1;
+  for (int v = 0; v < nv; v++)
+    for (int i = Ng; i < i1; i++)
+      for (int j = j0; j < j1; j++)
+        for (int k = k0; k < k1; k++) {
+          long q = IX(v, i, j, k);
+          double acc = 0.0;
+          for (int dir = 0; dir < g.dims; dir++) {
+            double term = fr[dir][q + st[dir]] / dd[dir] - fr[dir][q] / dd[dir];
+            acc = dir == 0 ? term : acc + term;
+          }
+          fnet[q] = acc;
+        }
+  for (int dir = 0; dir < 3; dir++) free(fr[dir]);
+  free(f);
+}
+
+/* ------------------------------------------------------------ time step */
+
+/* Simulation::updateTime's dt policy (simulation.cc:113-124), alpha = 1. */
+double oc_timestep(const oc_config *c, double t, int iters, double end_time) {
+  geom g = mkgeom(c);
+  double sd = sqrt((double)g.dims);
+  double dtX = c->cfl * g.dx / (1.0 * sd);
+  double dtY = c->cfl * g.dy / (1.0 * sd);
+  double dtZ = c->cfl * g.dz / (1.0 * sd);
+  double dt = (dtX <= dtY && dtX <= dtZ) ? dtX : ((dtY < dtZ) ? dtY : dtZ);
+  if (iters < 15) dt *= 0.1;
+  if (iters >= 15 && iters < 30) dt *= 0.25;
+  if (t + dt > end_time) dt = end_time - t;
+  return dt;
+}
+
+/* TimeIntegrator::finalise (timeInt.h:117-132): C2P, then BCs. */
+static long finalise(const oc_config *c, double *cons, double *prims, double *aux, oc_stats *st) {
+  long fails = oc_c2p(c, cons, prims, aux, st);
+  apply_bcs_all(c, cons, prims, aux);
+  return fails;
+}
+
+/* RK2::predictorStep, correctorStep, step (RK2.cc:31-100). */
+long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
+  geom g = mkgeom(c);
+  int nc = oc_ncons(c->model), np = oc_nprims(c->model), na = oc_naux(c->model);
+  double *p1cons = (double *)calloc(g.ntot * nc, sizeof(double));
+  double *p1prims = (double *)malloc(sizeof(double) * g.ntot * np);
+  double *p1aux = (double *)malloc(sizeof(double) * g.ntot * na);
+  double *args = (double *)calloc(g.ntot * nc, sizeof(double));
+  memcpy(p1prims, prims, sizeof(double) * g.ntot * np);
+  memcpy(p1aux, aux, sizeof(double) * g.ntot * na);
+  memcpy(p1cons, cons, sizeof(double) * g.ntot * nc);
+
+  oc_fvs_F(c, cons, prims, aux, args);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) p1cons[IX(v, i, j, k)] = cons[IX(v, i, j, k)] - dt * args[IX(v, i, j, k)];
+  long fails = finalise(c, p1cons, p1prims, p1aux, st);
+
+  oc_fvs_F(c, p1cons, p1prims, p1aux, args);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) {
+          long q = IX(v, i, j, k);
+          cons[q] = 0.5 * (cons[q] + p1cons[q] - dt * args[q]);
+        }
+  fails += finalise(c, cons, prims, aux, st);
+  free(p1cons);
+  free(p1prims);
+  free(p1aux);
+  free(args);
+  return fails;
+}
+
+/* SSP2 IMEX step: implemented with the SRRMHD model below. */
+long oc_ssp2_step_impl(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
+
+long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
+  return oc_ssp2_step_impl(c, cons, prims, aux, dt, st);
+}
+
+long oc_ssp2_step_impl(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
+  (void)c; (void)cons; (void)prims; (void)aux; (void)dt; (void)st;
+  return -1; /* filled in with the implicit-stage restatement */
+}

@@ -1,0 +1,161 @@
+"""Write the golden fixtures under tests/golden/ from the reference build.
+
+Runs ``oracle/_ref/method_ref`` -- the reference METHOD serial CPU sources,
+compiled unmodified from the reference tree by ``oracle/ref/Makefile`` -- on
+initial data from ``method_amd.initdata`` and stores inputs and outputs as
+compressed .npz files. Only inputs and outputs are stored (no reference
+source). Re-run with ``python tests/golden/make_fixtures.py`` in a container
+that has the reference tree; the GPU box only reads the .npz files.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from method_amd import initdata as idt  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "method_ref")
+NV = {"srmhd": (9, 8, 13), "srrmhd": (14, 11, 17)}
+
+
+def _run(args, cwd):
+    r = subprocess.run([REF_BIN] + args, capture_output=True, text=True, cwd=cwd)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference run failed: {args}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return r.stdout
+
+
+def _load_state(path, model, shape):
+    nc, np_, na = NV[model]
+    n = shape[0] * shape[1] * shape[2]
+    a = np.fromfile(path)
+    assert a.size == (nc + np_ + na) * n, path
+    return (a[: nc * n].reshape((nc,) + shape), a[nc * n:(nc + np_) * n].reshape((np_,) + shape),
+            a[(nc + np_) * n:].reshape((na,) + shape))
+
+
+def _grid_args(g: idt.Grid, gamma, cfl, weno, integrator, bc, model, sigma=0.0, cp=0.1):
+    return [
+        f"model={model}", f"nx={g.nx}", f"ny={g.ny}", f"nz={g.nz}",
+        f"xmin={g.xmin!r}", f"xmax={g.xmax!r}", f"ymin={g.ymin!r}", f"ymax={g.ymax!r}",
+        f"zmin={g.zmin!r}", f"zmax={g.zmax!r}", f"Ng={g.Ng}", f"gamma={gamma!r}", f"cfl={cfl!r}",
+        f"sigma={sigma!r}", f"cp={cp!r}", f"weno={weno}", f"integrator={integrator}",
+        f"bcx={bc[0]}", f"bcy={bc[1]}", f"bcz={bc[2]}",
+    ]
+
+
+def run_case(name, g, prims0, model, weno, integrator, bc, steps, dumps, gamma=4.0 / 3.0, cfl=0.4,
+             sigma=0.0, cp=0.1, interior_only=False, with_fnet=False):
+    shape = g.shape
+    with tempfile.TemporaryDirectory() as td:
+        pin = os.path.join(td, "prims.bin")
+        np.ascontiguousarray(prims0).tofile(pin)
+        base = _grid_args(g, gamma, cfl, weno, integrator, bc, model, sigma, cp)
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", f"steps={steps}", f"prims_in={pin}", f"out={out}",
+                     "dump=" + ",".join(str(s) for s in dumps)], td)
+        log = open(out + "_log.txt").read().split("\n")
+        dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+        fail_line = [l for l in log if l.startswith("c2p_calls")]
+        rec = {
+            "prims0": prims0, "dts": np.array(dts),
+            "meta": np.array([g.nx, g.ny, g.nz, g.Ng, weno, steps]),
+            "bounds": np.array([g.xmin, g.xmax, g.ymin, g.ymax, g.zmin, g.zmax]),
+            "params": np.array([gamma, cfl, sigma, cp]),
+            "periodic": np.array([1 if b == "periodic" else 0 for b in bc]),
+            "dumps": np.array(dumps),
+        }
+        if fail_line:
+            rec["c2p_fail"] = np.array([int(fail_line[0].split()[-1])])
+        sl = (slice(None), slice(g.Ng, shape[0] - g.Ng),
+              slice(g.Ng, shape[1] - g.Ng) if g.ny > 0 else slice(None),
+              slice(g.Ng, shape[2] - g.Ng) if g.nz > 0 else slice(None))
+        for s in dumps:
+            c, p, a = _load_state(f"{out}_step{s}.bin", model, shape)
+            if interior_only:
+                c, p, a = c[sl], p[sl], a[sl]
+            rec[f"cons{s}"], rec[f"prims{s}"], rec[f"aux{s}"] = c.copy(), p.copy(), a.copy()
+        if with_fnet:
+            _run(base + ["mode=fnet", f"prims_in={pin}", f"out={out}"], td)
+            nc = NV[model][0]
+            fnet = np.fromfile(out + "_fnet.bin").reshape((nc,) + shape)
+            rec["fnet0"] = fnet[sl].copy()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **rec)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
+def c2p_vectors(name, n=2000, seed=7):
+    """Reference SRMHD::getPrimitiveVars on a 1-D row of synthetic states.
+
+    Primitive states are drawn over rho in [1e-2, 10], p/rho in [1e-2, 10],
+    |v| < 0.9 and |B| < 2 in random directions; cons/aux come from the
+    reference primsToAll (step 0 of a run with steps=0), and the C2P guess is
+    the true state perturbed by up to 1e-3 relative, as after one small step.
+    """
+    rng = np.random.default_rng(seed)
+    g = idt.Grid(n, 0, 0, 0.0, 1.0, 0.0, 1.0)
+    shape = g.shape
+    prims = np.zeros((8,) + shape)
+    rho = 10 ** rng.uniform(-2, 1, size=shape)
+    prims[0] = rho
+    prims[4] = rho * 10 ** rng.uniform(-2, 1, size=shape)
+    vdir = rng.normal(size=(3,) + shape)
+    vdir /= np.linalg.norm(vdir, axis=0)
+    prims[1:4] = vdir * rng.uniform(0, 0.9, size=shape)
+    bdir = rng.normal(size=(3,) + shape)
+    bdir /= np.linalg.norm(bdir, axis=0)
+    prims[5:8] = bdir * rng.uniform(0, 2.0, size=shape)
+    with tempfile.TemporaryDirectory() as td:
+        pin = os.path.join(td, "prims.bin")
+        prims.tofile(pin)
+        base = _grid_args(g, 4.0 / 3.0, 0.4, 3, "rk2", ("outflow", "outflow", "outflow"), "srmhd")
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", "steps=0", f"prims_in={pin}", f"out={out}", "dump=0"], td)
+        cons, ptrue, atrue = _load_state(out + "_step0.bin", "srmhd", shape)
+        guess_p = ptrue * (1.0 + 1e-3 * rng.uniform(-1, 1, size=ptrue.shape))
+        guess_a = atrue * (1.0 + 1e-3 * rng.uniform(-1, 1, size=atrue.shape))
+        for arr, nm in ((cons, "cons"), (guess_p, "gp"), (guess_a, "ga")):
+            np.ascontiguousarray(arr).tofile(os.path.join(td, nm + ".bin"))
+        txt = _run(base + ["mode=c2p", f"cons_in={td}/cons.bin", f"prims_in={td}/gp.bin", f"aux_in={td}/ga.bin",
+                           f"out={out}"], td)
+        res = np.fromfile(out + "_c2p.bin")
+        m = 8 * shape[0]
+        p_ref = res[:m].reshape((8,) + shape)
+        a_ref = res[m:].reshape((13,) + shape)
+    fail = int(txt.split("fail=")[1].split()[0])
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, prims_true=ptrue, aux_true=atrue, cons=cons, guess_prims=guess_p, guess_aux=guess_a,
+                        prims_ref=p_ref, aux_ref=a_ref, n=np.array([n]), c2p_fail=np.array([fail]))
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), reference hybrd failures: {fail}")
+
+
+def main():
+    if not os.path.exists(REF_BIN):
+        raise SystemExit(f"{REF_BIN} missing: build it with `make -C oracle/ref`")
+    # Ideal SRMHD 2-D Kelvin-Helmholtz, 64x64, WENO3, RK2, cfl 0.4, 10 steps,
+    # periodic x / outflow y (the reference's Flow), seed 1234.
+    g = idt.kh2d_grid(64)
+    run_case("kh2d_64_weno3_rk2", g, idt.kh_prims(g, seed=1234), "srmhd", 3, "rk2",
+             ("periodic", "outflow", "periodic"), 10, [1, 10], with_fnet=True)
+    # Ideal SRMHD 3-D Kelvin-Helmholtz, 16^3, WENO5, RK2, periodic x,z / outflow y.
+    g3 = idt.kh3d_grid(16)
+    run_case("kh3d_16_weno5_rk2", g3, idt.kh_prims(g3, seed=99), "srmhd", 5, "rk2",
+             ("periodic", "outflow", "periodic"), 3, [1, 3], interior_only=True, with_fnet=True)
+    # Resistive SRRMHD 2-D Orszag-Tang, 32x32, WENO3, SSP2, sigma=1e3, cp 0.1, cfl 0.5.
+    go = idt.ot2d_grid(32)
+    run_case("ot2d_32_weno3_ssp2", go, idt.ot_prims_resistive(go), "srrmhd", 3, "ssp2",
+             ("periodic", "periodic", "periodic"), 3, [1, 3], cfl=0.5, sigma=1e3, cp=0.1, with_fnet=True)
+    c2p_vectors("srmhd_c2p_vectors")
+
+
+if __name__ == "__main__":
+    main()
