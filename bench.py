@@ -1,0 +1,169 @@
+"""Benchmark: FP64 ideal SRMHD 3-D Kelvin-Helmholtz, 256^3, WENO5 FVS, RK2.
+
+One step = one RK2 step (two stages, each with the flux sweeps, the update and
+the per-cell Newton cons2prim) of the whole grid, state resident in HBM.
+Prints one JSON line. See DESIGN.md for the roofline accounting.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s FP64 SRMHD 256^3; cons2prim cells/s learned vs plain guess"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes of one SRMHD RK2 zone update: 98 FP64 values
+# (stage 1 reads U^n, V^n and writes U1, V1; stage 2 reads U^n, U1, V1 and the
+# V^n guess, writes U^{n+1}, V^{n+1}, A^{n+1}).
+BYTES_PER_ZONE_UPDATE = 784
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kh3d_256_weno5_rk2_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="cells per side")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=96, help="cells per side of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def cpu_baseline(n: int, steps: int):
+    """The C oracle (serial restatement of the reference step) on the host."""
+    from method_amd import initdata as idt
+    from oracle import oracle as orc
+
+    g = idt.kh3d_grid(n)
+    prims0 = idt.kh_prims(g, seed=1234)
+    o = orc.Oracle(orc.Case(nx=n, ny=n, nz=n, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                            weno_order=5, periodic=(1, 0, 1), cfl=0.4))
+    c, p, a = o.set(prims0)
+    t0 = time.perf_counter()
+    t = 0.0
+    for it in range(steps):
+        dt = o.timestep(t, it)
+        o.step(c, p, a, dt)
+        t += dt
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": f"C oracle (serial restatement of the reference RK2 step), KH 3-D {n}^3 WENO5 RK2, "
+                      f"{steps} steps, {el:.1f} s on 1 host core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU halo exchange "
+                                   "is a design note only)"}))
+        sys.exit(2)
+    from method_amd import engine as eng
+    from method_amd import initdata as idt
+
+    n = args.n
+    g = idt.kh3d_grid(n)
+    prims0 = idt.kh_prims(g, seed=1234)
+    cfg = eng.Config(nx=n, ny=n, nz=n, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                     weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), cfl=0.4)
+    cells = cfg.interior_cells
+    e = eng.Engine(cfg)
+    e.set_prims(prims0)
+    del prims0
+    print(f"bench: {n}^3 set up; warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        e.update_time()
+    e.sync()
+
+    # dt sequence of the next K updateTime calls (slow start changes dt only)
+    t, _, iters = e.time()
+    dts = []
+    for k in range(args.steps):
+        it = iters + k
+        base = 0.4 * (1.0 / n) / np.sqrt(3.0)
+        dts.append(base * (0.1 if it < 15 else (0.25 if it < 30 else 1.0)))
+
+    print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
+    e.sync()
+    w0 = time.perf_counter()
+    event_ms, slots = e.timed_steps(dts)
+    e.sync()
+    wall = time.perf_counter() - w0
+    e.set_time(t + float(np.sum(dts)), iters + args.steps)
+
+    # cons2prim throughput with the plain guess: previous-step prims against
+    # the next step's cons, as inside the time step
+    _, p_prev, _ = e.download()
+    _, st_step = e.step(dts[-1], stats=True)
+    e.upload(prims=p_prev)
+    del p_prev
+    e.sync()
+    c0 = time.perf_counter()
+    _, st_c2p = e.c2p()
+    c2p_s = time.perf_counter() - c0
+
+    ms_per_step = 1e3 * wall / args.steps
+    value = cells * args.steps / wall
+    step_kernel_ms = sum(slots.values())
+    dom = max(slots, key=slots.get)
+    achieved_step = BYTES_PER_ZONE_UPDATE * cells / (event_ms / args.steps * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("bytes_per_step")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Kelvin-Helmholtz shear layers, splitmix64 perturbation seed 1234)",
+        "config": {"workload": f"ideal SRMHD 3-D Kelvin-Helmholtz {n}^3, WENO5 FVS, RK2, cfl 0.4, "
+                               "periodic x,z / outflow y, FP64", "cells": cells, "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "one RK2 step (the fused sweep launches of both stages), 784 B per zone update",
+            "achieved": achieved_step,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved_step / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "event_ms_per_step": event_ms / args.steps,
+            "dominant_launch": dom,
+            "dominant_launch_ms": slots[dom],
+            "launch_ms": slots,
+            "launch_ms_sum": step_kernel_ms,
+        },
+        "c2p": {
+            "guess": "plain (previous-step primitives)",
+            "cells_per_s": st_c2p["cells"] / c2p_s,
+            "newton_steps_per_cell": st_c2p["iterations"] / max(st_c2p["cells"], 1),
+            "max_newton_steps": st_c2p["max_iterations"],
+            "failures": st_c2p["failures"],
+            "in_step_newton_steps_per_cell": st_step["iterations"] / max(st_step["cells"], 1),
+        },
+    }
+    e.close()
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+/*
+ * method_gpu.h -- C ABI of the MI355X time-step engine for METHOD's hot path.
+ *
+ * The reference drives its time step through abstract C++ classes that take
+ * raw host double* arrays in the layout [var][i][j][k] over the ghost-padded
+ * grid (AlexJamesWright/METHOD, Project/CPU/Include/simData.h:10). This header
+ * is the thin, C-only layer those classes call instead: plain pointers, sizes
+ * and int status codes, no C++ types or exceptions across it. Host arrays
+ * passed in or out always use the reference layout (padded, var-major, k
+ * fastest); on the device the engine keeps its own interior-only layout and
+ * fills ghost cells by index remapping inside the kernels.
+ *
+ * Status codes: 0 = ok, < 0 = error (text from mgpu_last_error()), > 0 = a
+ * count of cells whose cons2prim solve did not converge (warning).
+ *
+ * Threading: one HIP stream per engine; an engine handle must not be used
+ * from two host threads at once. Calls that return data to the host
+ * synchronise the engine's stream; mgpu_step may still be running on return
+ * until the next synchronising call.
+ */
+#ifndef METHOD_GPU_H
+#define METHOD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPU_ABI_VERSION 1
+
+enum { MGPU_SRMHD = 0, MGPU_SRRMHD = 1 };
+enum { MGPU_RK2 = 0, MGPU_SSP2 = 1 };
+enum { MGPU_BC_OUTFLOW = 0, MGPU_BC_PERIODIC = 1 };
+
+/* Simulation parameters: the constructor arguments of the reference Data
+ * object (Project/CPU/Include/simData.h:205-218) plus the choice of model,
+ * WENO order, integrator and per-axis boundary, which the reference makes by
+ * constructing SRMHD/SRRMHD, Weno3/Weno5, RK2/SSP2 and Periodic/Outflow/Flow
+ * objects (Project/CPU/Src/main.cc:48-60). nz == 0 selects a 2-D grid
+ * (simData.cc:74-79). */
+typedef struct {
+  int nx, ny, nz, Ng;
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  double gamma, sigma, cp, cfl;
+  int model;       /* MGPU_SRMHD | MGPU_SRRMHD */
+  int weno_order;  /* 3 | 5 */
+  int integrator;  /* MGPU_RK2 | MGPU_SSP2 */
+  int bc[3];       /* per axis: MGPU_BC_PERIODIC | MGPU_BC_OUTFLOW */
+  int quirks;      /* 1: reproduce the reference's flux-index quirks */
+  int device;      /* HIP device ordinal */
+} mgpu_config;
+
+/* Cons2prim work counters, summed over the call. */
+typedef struct {
+  long long cells;          /* cells solved */
+  long long iterations;     /* Newton (SRMHD) or secant (SRRMHD) steps */
+  long long failures;       /* cells that did not converge */
+  long long max_iterations; /* largest per-cell count */
+} mgpu_stats;
+
+typedef struct mgpu_engine mgpu_engine;
+
+/* Text of the last error on this thread (never NULL). */
+const char *mgpu_last_error(void);
+
+/* Allocates device state for one simulation. Replaces the allocations of
+ * Simulation::Simulation (Project/CPU/Src/simulation.cc:22-30) and the
+ * integrator's work arrays (RK2.cc:5-18, SSP2.cc:13-34). */
+int mgpu_create(const mgpu_config *cfg, mgpu_engine **out);
+int mgpu_destroy(mgpu_engine *e);
+
+/* Number of cons, prims and aux variables of the engine's model
+ * (srmhd.cc:25-29, srrmhd.h via SRRMHD::SRRMHD). */
+int mgpu_sizes(const mgpu_engine *e, int *ncons, int *nprims, int *naux);
+
+/* Host -> device state in the reference layout (padded [var][i][j][k]);
+ * only interior cells are read. Any pointer may be NULL to leave that array. */
+int mgpu_upload_state(mgpu_engine *e, const double *cons, const double *prims, const double *aux);
+
+/* Device -> host state in the reference layout with ghost cells filled by the
+ * engine's boundary conditions, as Bcs::apply leaves them
+ * (boundaryConds.cc:5-145,449-730). Any pointer may be NULL. */
+int mgpu_download_state(mgpu_engine *e, double *cons, double *prims, double *aux);
+
+/* Model::primsToAll over the resident primitives, then the boundary
+ * conditions: the body of Simulation::set (simulation.cc:94-96). */
+int mgpu_prims_to_all(mgpu_engine *e);
+
+/* One full time step of the configured integrator on the resident state with
+ * step size dt: TimeIntegrator::step (RK2.cc:93-100, SSP2.cc:53-173), with
+ * FVS::F, Model::getPrimitiveVars and Bcs::apply inside. stats may be NULL. */
+int mgpu_step(mgpu_engine *e, double dt, mgpu_stats *stats);
+
+/* Simulation::updateTime (simulation.cc:104-133): computes dt from the CFL
+ * rule with slow start and end-time clamp, steps, advances t and iters. */
+int mgpu_update_time(mgpu_engine *e, double end_time, mgpu_stats *stats);
+int mgpu_get_time(const mgpu_engine *e, double *t, double *dt, int *iters);
+int mgpu_set_time(mgpu_engine *e, double t, int iters);
+
+/* FVS::F (fluxVectorSplitting.cc:58-131) of the resident state; fnet is
+ * written in the reference layout, interior cells only (ghosts zero). */
+int mgpu_fnet(mgpu_engine *e, double *fnet);
+
+/* Model::fluxVector (srmhd.cc:72-206, srrmhd.cc:65-192) of the resident
+ * state in direction dir, written for every padded cell. */
+int mgpu_flux_vector(mgpu_engine *e, int dir, double *f);
+
+/* Model::getPrimitiveVars (srmhd.cc:390-562, srrmhd.cc:314-356) on the
+ * resident cons with the resident prims as the starting guess. */
+int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);
+
+/* Block until the engine's queued work is done. */
+int mgpu_sync(mgpu_engine *e);
+
+/* Runs nsteps steps with the given dt values (length nsteps) and reports the
+ * elapsed time of the whole sequence in milliseconds, measured with HIP
+ * events recorded on the engine's stream; kernel_ms (may be NULL, length
+ * kernel_slots) receives the average duration per launch of each kernel slot
+ * of one step, measured the same way on a separate pass, and the slot count
+ * is returned in *nslots. */
+int mgpu_timed_steps(mgpu_engine *e, int nsteps, const double *dts, double *total_ms,
+                     double *kernel_ms, int kernel_slots, int *nslots);
+
+/* Name of kernel slot i in the order reported by mgpu_timed_steps. */
+const char *mgpu_kernel_slot_name(const mgpu_engine *e, int slot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,94 @@
+"""Builds the native libraries in-tree.
+
+* ``method_amd/lib/libmethod_gpu.so`` -- HIP kernels and the C ABI of
+  ``include/method_gpu.h``, compiled for gfx950 only.
+* ``method_amd/lib/libmethod_host.so`` -- the C++ host API (Data, Simulation,
+  SRMHD, FVS, RK2, ...) layered on the C ABI.
+
+``python -m method_amd.build`` builds both.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "method_amd")
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+LIBDIR = os.path.join(PKG, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+GPU_LIB = os.path.join(LIBDIR, "libmethod_gpu.so")
+HOST_LIB = os.path.join(LIBDIR, "libmethod_host.so")
+
+GPU_SOURCES = ["mgpu_kernels.hip", "mgpu_engine.hip"]
+ARCH = "gfx950"
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, cwd=ROOT):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout[-4000:] + r.stderr[-8000:])
+    return r
+
+
+def gpu_flags():
+    return [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+        # no packed FP32 instructions in any HIP code
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-I", INCLUDE, "-I", CSRC,
+    ]
+
+
+def build_gpu() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src in GPU_SOURCES:
+        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
+        _run([hipcc()] + gpu_flags() + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = GPU_LIB + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, GPU_LIB)
+    for o in objs:
+        os.remove(o)
+    return GPU_LIB
+
+
+def build_host() -> str | None:
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cc")) if os.path.isdir(HOST) else []
+    if not srcs:
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", "-std=c++14", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-I", HOST, "-o", tmp] + srcs
+         + ["-L", LIBDIR, "-lmethod_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
+
+
+def build_all(verbose: bool = True) -> list[str]:
+    out = [build_gpu()]
+    h = build_host()
+    if h:
+        out.append(h)
+    if verbose:
+        for p in out:
+            print("built", p)
+    return out
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    sys.exit(0)

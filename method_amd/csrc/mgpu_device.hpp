@@ -1,0 +1,246 @@
+// Device-side numerics of the MI355X time-step engine: grid geometry with
+// boundary index remapping, WENO3/WENO5 reconstruction and the ideal SRMHD
+// model (flux vector, prims -> all, Newton cons2prim). FP64 throughout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mgpu {
+
+// Interior-only device layout [var][i][j][k], k fastest. 2-D grids keep a
+// unit z extent. Ghost cells are never stored: a read outside the interior is
+// redirected to the interior cell the reference's Bcs::apply would have
+// copied there (periodic wrap or outflow clamp, boundaryConds.cc:5-145,590-730).
+struct Geo {
+  int n[3];             // interior extents; n[2] == 1 in 2-D
+  long long stride[3];  // element strides of i, j, k
+  long long vstride;    // elements per variable plane = n0*n1*n2
+  int periodic[3];      // 1 periodic, 0 outflow
+  double inv_d[3];      // 1/dx, 1/dy, 1/dz
+  int dims;
+};
+
+__device__ __forceinline__ int remap(int i, int n, int periodic) {
+  if (i < 0) return periodic ? i + n : 0;
+  if (i >= n) return periodic ? i - n : n - 1;
+  return i;
+}
+
+// ------------------------------------------------------------------ WENO
+// Jiang-Shu WENO with nonlinear weights alpha_k = d_k / (eps + beta_k^2),
+// eps = 1e-16, as in the reference (wenoUpwinds.cc:5-84). The normalised
+// weights are formed with one division: sum_k d_k q_k / s_k over sum_k d_k / s_k
+// is multiplied through by s_0 s_1 (s_2), which is the same quantity up to
+// rounding. beta_k are the Jiang-Shu smoothness indicators in factored form.
+
+constexpr double kWenoEps = 1e-16;
+
+// Upwind value at the right face of v1 from (v0, v1, v2); the downwind value
+// at the left face of v1 is weno3(v2, v1, v0).
+__device__ __forceinline__ double weno3(double v0, double v1, double v2) {
+  const double a0 = v1 - v0, a1 = v2 - v1;
+  const double b0 = a0 * a0, b1 = a1 * a1;
+  const double s0 = fma(b0, b0, kWenoEps), s1 = fma(b1, b1, kWenoEps);
+  const double q0 = fma(1.5, v1, -0.5 * v0);  // (-v0 + 3 v1) / 2
+  const double q1 = 0.5 * (v1 + v2);          // (v1 + v2) / 2
+  const double t0 = (1.0 / 3.0) * s1, t1 = (2.0 / 3.0) * s0;
+  return fma(t0, q0, t1 * q1) / (t0 + t1);
+}
+
+// Upwind value at the right face of v2 from (v0..v4).
+__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4) {
+  constexpr double c13 = 13.0 / 12.0;
+  const double a0 = v0 - 2.0 * v1 + v2, e0 = v0 - 4.0 * v1 + 3.0 * v2;
+  const double a1 = v1 - 2.0 * v2 + v3, e1 = v1 - v3;
+  const double a2 = v2 - 2.0 * v3 + v4, e2 = 3.0 * v2 - 4.0 * v3 + v4;
+  const double b0 = fma(c13 * a0, a0, 0.25 * e0 * e0);
+  const double b1 = fma(c13 * a1, a1, 0.25 * e1 * e1);
+  const double b2 = fma(c13 * a2, a2, 0.25 * e2 * e2);
+  const double s0 = fma(b0, b0, kWenoEps), s1 = fma(b1, b1, kWenoEps), s2 = fma(b2, b2, kWenoEps);
+  // candidate stencils (2 v0 - 7 v1 + 11 v2)/6, (-v1 + 5 v2 + 2 v3)/6, (2 v2 + 5 v3 - v4)/6
+  const double q0 = (1.0 / 6.0) * (2.0 * v0 - 7.0 * v1 + 11.0 * v2);
+  const double q1 = (1.0 / 6.0) * (-v1 + 5.0 * v2 + 2.0 * v3);
+  const double q2 = (1.0 / 6.0) * (2.0 * v2 + 5.0 * v3 - v4);
+  const double t0 = 0.1 * (s1 * s2), t1 = 0.6 * (s0 * s2), t2 = 0.3 * (s0 * s1);
+  return fma(t0, q0, fma(t1, q1, t2 * q2)) / (t0 + t1 + t2);
+}
+
+template <int ORDER>
+struct Weno;
+
+template <>
+struct Weno<3> {
+  static constexpr int R = 2;  // stencil radius = (order + 1) / 2
+  // Face between cells R-1 and R of p[0 .. 2R-1] (upwind on fp, downwind on fm).
+  template <class LP, class LM>
+  __device__ __forceinline__ static double face(LP fp, LM fm) {
+    return weno3(fp(0), fp(1), fp(2)) + weno3(fm(3), fm(2), fm(1));
+  }
+};
+
+template <>
+struct Weno<5> {
+  static constexpr int R = 3;
+  template <class LP, class LM>
+  __device__ __forceinline__ static double face(LP fp, LM fm) {
+    return weno5(fp(0), fp(1), fp(2), fp(3), fp(4)) + weno5(fm(5), fm(4), fm(3), fm(2), fm(1));
+  }
+};
+
+// ----------------------------------------------------------------- SRMHD
+// Variable order (srmhd.cc:36-53): cons D, Sx, Sy, Sz, tau, Bx, By, Bz, phi;
+// prims rho, vx, vy, vz, p, Bx, By, Bz; aux h, W, e, c, b0, bx, by, bz, bsq,
+// vsq, BS, Bsq, Ssq.
+constexpr int kSrmhdCons = 9, kSrmhdPrims = 8, kSrmhdAux = 13;
+
+// Flux vector in direction DIR from the cell's cons and (vx, vy, vz, p)
+// (SRMHD::fluxVector, srmhd.cc:72-206). W and b^mu are formed from the
+// primitives as SRMHD::primsToAll does (srmhd.cc:596-622). QUIRK_PHI_Z: the
+// z flux of phi is By as at srmhd.cc:200.
+template <int DIR, bool QUIRK_PHI_Z>
+__device__ __forceinline__ void srmhd_flux(const double u[9], double vx, double vy, double vz, double p,
+                                           double f[9]) {
+  const double Bx = u[5], By = u[6], Bz = u[7];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double iW = sqrt(1.0 - v2);  // 1/W
+  const double W = 1.0 / iW;
+  const double b0 = W * (vx * Bx + vy * By + vz * Bz);
+  const double bx = fma(Bx, iW, b0 * vx), by = fma(By, iW, b0 * vy), bz = fma(Bz, iW, b0 * vz);
+  const double bsq = (Bx * Bx + By * By + Bz * Bz + b0 * b0) * (iW * iW);
+  const double vd = DIR == 0 ? vx : (DIR == 1 ? vy : vz);
+  const double Bd = DIR == 0 ? Bx : (DIR == 1 ? By : Bz);
+  const double ptot = p + 0.5 * bsq;
+  const double BdiW = Bd * iW;
+  f[0] = u[0] * vd;
+  f[1] = fma(u[1], vd, -bx * BdiW);
+  f[2] = fma(u[2], vd, -by * BdiW);
+  f[3] = fma(u[3], vd, -bz * BdiW);
+  f[1 + DIR] += ptot;
+  f[4] = fma(u[4] + ptot, vd, -b0 * BdiW);
+  f[5] = DIR == 0 ? u[8] : fma(Bx, vd, -Bd * vx);
+  f[6] = DIR == 1 ? u[8] : fma(By, vd, -Bd * vy);
+  f[7] = DIR == 2 ? u[8] : fma(Bz, vd, -Bd * vz);
+  f[8] = (DIR == 2 && QUIRK_PHI_Z) ? By : Bd;
+}
+
+// SRMHD::primsToAll for one cell (srmhd.cc:573-673).
+__device__ __forceinline__ void srmhd_prims_to_all(const double P[8], double gam, double U[9], double A[13]) {
+  U[5] = P[5];
+  U[6] = P[6];
+  U[7] = P[7];
+  A[11] = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+  U[8] = 0.0;
+  A[9] = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+  A[1] = 1.0 / sqrt(1.0 - A[9]);
+  A[4] = A[1] * (P[1] * P[5] + P[2] * P[6] + P[3] * P[7]);
+  A[5] = P[5] / A[1] + A[4] * P[1];
+  A[6] = P[6] / A[1] + A[4] * P[2];
+  A[7] = P[7] / A[1] + A[4] * P[3];
+  A[8] = (A[11] + A[4] * A[4]) / (A[1] * A[1]);
+  A[0] = 1.0 + P[4] / P[0] * (gam / (gam - 1.0));
+  A[2] = P[4] / (P[0] * (gam - 1.0));
+  A[3] = sqrt(A[2] * gam * (gam - 1.0) / A[0]);
+  U[0] = P[0] * A[1];
+  const double e = (P[0] * A[0] + A[8]) * A[1] * A[1];
+  U[1] = e * P[1] - A[4] * A[5];
+  U[2] = e * P[2] - A[4] * A[6];
+  U[3] = e * P[3] - A[4] * A[7];
+  A[12] = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+  A[10] = P[5] * U[1] + P[6] * U[2] + P[7] * U[3];
+  U[4] = e - (P[4] + A[8] * 0.5) - A[4] * A[4] - U[0];
+}
+
+constexpr double kNewtonTol = 1e-12;
+constexpr int kNewtonMaxIt = 60;
+
+// Newton on the 2x2 cons2prim system of srmhd.cc:269-295 in x = (v^2, Z =
+// rho h W^2), with its analytic Jacobian. Converged when the scaled residual
+// max(|f0|/(Z+B^2)^2, |f1|/(Z+B^2)) <= 1e-12; steps leaving v^2 in [0,1) or
+// Z > 0 are pulled back half way. Returns Newton steps taken or -1.
+__device__ __forceinline__ int srmhd_newton(double D, double tau, double Bsq, double Ssq, double BS, double gam,
+                                            double &v2, double &Z) {
+  if (!(v2 >= 0.0 && v2 < 1.0)) v2 = 0.0;
+  if (!(Z > 0.0)) Z = D + tau + Bsq;
+  const double gf = (gam - 1.0) / gam;
+  const double BS2 = BS * BS;
+  for (int it = 0; it <= kNewtonMaxIt; ++it) {
+    const double s = 1.0 - v2, sq = sqrt(s);
+    const double p = gf * (Z * s - D * sq);
+    const double ZB = Z + Bsq;
+    const double iZ = 1.0 / Z;
+    const double iZ2 = iZ * iZ;
+    const double f0 = ZB * ZB * v2 - (2.0 * Z + Bsq) * BS2 * iZ2 - Ssq;
+    const double f1 = ZB - p - 0.5 * Bsq * s - 0.5 * BS2 * iZ2 - D - tau;
+    const double iZB = 1.0 / ZB;
+    const double r = fmax(fabs(f0) * iZB * iZB, fabs(f1) * iZB);
+    if (r <= kNewtonTol) return it;
+    if (!(r == r) || it == kNewtonMaxIt) break;
+    const double BS2iZ3 = BS2 * iZ2 * iZ;
+    const double J00 = ZB * ZB;
+    const double J01 = 2.0 * ZB * (v2 + BS2iZ3);
+    const double J10 = gf * (Z - 0.5 * D / sq) + 0.5 * Bsq;
+    const double J11 = 1.0 - gf * s + BS2iZ3;
+    const double idet = 1.0 / (J00 * J11 - J01 * J10);
+    const double d0 = (J11 * f0 - J01 * f1) * idet;
+    const double d1 = (J00 * f1 - J10 * f0) * idet;
+    double nv2 = v2 - d0, nZ = Z - d1;
+    if (nv2 < 0.0) nv2 = 0.5 * v2;
+    if (nv2 >= 1.0) nv2 = 0.5 * (v2 + 1.0);
+    if (nZ <= 0.0) nZ = 0.5 * Z;
+    v2 = nv2;
+    Z = nZ;
+  }
+  return -1;
+}
+
+// Recovery of prims and aux from the root (srmhd.cc:509-557).
+__device__ __forceinline__ void srmhd_recover(const double U[9], double gam, double v2, double Z, double P[8],
+                                              double A[13]) {
+  const double BS = U[5] * U[1] + U[6] * U[2] + U[7] * U[3];
+  const double Bsq = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+  const double Ssq = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+  const double W = 1.0 / sqrt(1.0 - v2);
+  const double rho = U[0] / W;
+  const double h = Z / (rho * W * W);
+  const double p = (h - 1.0) * rho * (gam - 1.0) / gam;
+  const double e = p / (rho * (gam - 1.0));
+  const double den = 1.0 / (Z * (Bsq + Z));
+  const double vx = (U[5] * BS + U[1] * Z) * den;
+  const double vy = (U[6] * BS + U[2] * Z) * den;
+  const double vz = (U[7] * BS + U[3] * Z) * den;
+  const double b0 = W * (U[5] * vx + U[6] * vy + U[7] * vz);
+  const double iW = 1.0 / W;
+  P[0] = rho;
+  P[1] = vx;
+  P[2] = vy;
+  P[3] = vz;
+  P[4] = p;
+  P[5] = U[5];
+  P[6] = U[6];
+  P[7] = U[7];
+  A[0] = h;
+  A[1] = W;
+  A[2] = e;
+  A[3] = sqrt(e * gam * (gam - 1.0) / h);
+  A[4] = b0;
+  A[5] = U[5] * iW + b0 * vx;
+  A[6] = U[6] * iW + b0 * vy;
+  A[7] = U[7] * iW + b0 * vz;
+  A[8] = (Bsq + b0 * b0) * (iW * iW);
+  A[9] = vx * vx + vy * vy + vz * vz;
+  A[10] = BS;
+  A[11] = Bsq;
+  A[12] = Ssq;
+}
+
+// Plain starting guess from the previous primitives (srmhd.cc:438-442), with
+// h re-formed from p and rho as primsToAll does (srmhd.cc:625).
+__device__ __forceinline__ void srmhd_plain_guess(double rho, double vx, double vy, double vz, double p, double gam,
+                                                  double &v2, double &Z) {
+  v2 = vx * vx + vy * vy + vz * vz;
+  const double h = 1.0 + p / rho * (gam / (gam - 1.0));
+  Z = rho * h / (1.0 - v2);
+}
+
+}  // namespace mgpu

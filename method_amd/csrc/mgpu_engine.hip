@@ -1,0 +1,459 @@
+// C ABI of the time-step engine (include/method_gpu.h): device state,
+// integrator sequencing, host <-> device layout conversion and timing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "method_gpu.h"
+#include "mgpu_kernels.hpp"
+
+using namespace mgpu;
+
+namespace {
+
+thread_local std::string g_err = "";
+
+int fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HIP_OK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(_e));      \
+  } while (0)
+
+// Records a HIP event on the engine stream after every kernel of a step so
+// each launch's duration can be read back.
+struct Marks {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> slot;
+};
+
+}  // namespace
+
+struct mgpu_engine {
+  mgpu_config cfg;
+  Geo geo;
+  PadGeo pad;
+  int ncons, nprims, naux;
+  hipStream_t stream = nullptr;
+  double *U = nullptr, *V = nullptr, *A = nullptr, *U1 = nullptr, *V1 = nullptr, *fnet = nullptr;
+  double *staging = nullptr;
+  size_t staging_elems = 0;
+  C2PCounters *dstats = nullptr;
+  int *fail_count = nullptr;
+  long long *fail_list = nullptr;
+  unsigned char *fail_flag = nullptr;
+  double t = 0.0, dt = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+  int iters = 0;
+  std::vector<std::string> slot_names;
+  Marks *marks = nullptr;  // non-null while a timed pass records per-kernel events
+};
+
+namespace {
+
+int slot_of(mgpu_engine *e, const char *name) {
+  for (size_t i = 0; i < e->slot_names.size(); ++i)
+    if (e->slot_names[i] == name) return (int)i;
+  e->slot_names.push_back(name);
+  return (int)e->slot_names.size() - 1;
+}
+
+int mark(mgpu_engine *e, const char *name) {
+  if (!e->marks) return 0;
+  hipEvent_t ev;
+  HIP_OK(hipEventCreate(&ev));
+  HIP_OK(hipEventRecord(ev, e->stream));
+  e->marks->ev.push_back(ev);
+  e->marks->slot.push_back(slot_of(e, name));
+  return 0;
+}
+
+int ensure_staging(mgpu_engine *e) {
+  const size_t need = (size_t)(e->ncons + e->nprims + e->naux) * e->pad.ntot;
+  if (e->staging_elems >= need) return 0;
+  if (e->staging) HIP_OK(hipFree(e->staging));
+  e->staging = nullptr;
+  HIP_OK(hipMalloc(&e->staging, need * sizeof(double)));
+  e->staging_elems = need;
+  return 0;
+}
+
+int zero_stats(mgpu_engine *e) {
+  HIP_OK(hipMemsetAsync(e->dstats, 0, sizeof(C2PCounters), e->stream));
+  return 0;
+}
+
+int read_stats(mgpu_engine *e, mgpu_stats *st) {
+  if (!st) return 0;
+  C2PCounters h;
+  HIP_OK(hipMemcpyAsync(&h, e->dstats, sizeof h, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  st->cells = (long long)h.cells;
+  st->iterations = (long long)h.iterations;
+  st->failures = (long long)h.failures;
+  st->max_iterations = (long long)h.max_iterations;
+  return 0;
+}
+
+SweepArgs sweep_args(mgpu_engine *e, const double *U, const double *V, C2PCounters *st) {
+  SweepArgs a;
+  memset(&a, 0, sizeof a);
+  a.geo = e->geo;
+  a.U = U;
+  a.V = V;
+  a.fnet = e->fnet;
+  a.gamma = e->cfg.gamma;
+  a.stats = st;
+  a.fail_count = e->fail_count;
+  a.fail_list = e->fail_list;
+  a.fail_flag = e->fail_flag;
+  return a;
+}
+
+// Sweeps in the order z, y, x; the x sweep applies `last_mode`.
+int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
+  const int dims = e->geo.dims;
+  const int order = e->cfg.weno_order, quirk = e->cfg.quirks;
+  int first = 1;
+  for (int dir = dims - 1; dir >= 0; --dir) {
+    int mode = dir == 0 ? last_mode : (first ? kSweepWrite : kSweepAdd);
+    if (dir == 0 && (last_mode == kSweepWrite || last_mode == kSweepAdd)) mode = first ? kSweepWrite : kSweepAdd;
+    launch_sweep_srmhd(a, order, dir, mode, quirk, e->stream);
+    static const char *names[3][5] = {
+        {"sweep_x_write", "sweep_x_add", "sweep_x_stage1_c2p", "sweep_x_stage2_c2p", "sweep_x"},
+        {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y"},
+        {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z"}};
+    if (mark(e, names[dir][mode])) return -1;
+    first = 0;
+  }
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int retry(mgpu_engine *e, const double *U, double *V, double *A, C2PCounters *st) {
+  const long long last = e->cfg.quirks ? e->geo.vstride - 1 : -1;
+  launch_c2p_retry_srmhd(e->geo, U, V, A, e->cfg.gamma, e->fail_count, e->fail_list, e->fail_flag, last, st,
+                         e->stream);
+  HIP_OK(hipGetLastError());
+  return mark(e, "c2p_retry");
+}
+
+// RK2::step (RK2.cc:93-100): predictor, finalise, corrector, finalise.
+int rk2_step(mgpu_engine *e, double dt, C2PCounters *st) {
+  SweepArgs a = sweep_args(e, e->U, e->V, st);
+  a.dt = dt;
+  a.Vg = e->V;
+  a.Uout = e->U1;
+  a.Vout = e->V1;
+  a.Aout = nullptr;
+  if (run_sweeps(e, a, kSweepStage1)) return -1;
+  if (retry(e, e->U1, e->V1, nullptr, st)) return -1;
+
+  SweepArgs b = sweep_args(e, e->U1, e->V1, st);
+  b.dt = dt;
+  b.Un = e->U;
+  b.Vg = e->V;
+  b.Uout = e->U;
+  b.Vout = e->V;
+  b.Aout = e->A;
+  if (run_sweeps(e, b, kSweepStage2)) return -1;
+  if (retry(e, e->U, e->V, e->A, st)) return -1;
+  return 0;
+}
+
+int do_step(mgpu_engine *e, double dt, mgpu_stats *stats) {
+  C2PCounters *st = stats ? e->dstats : nullptr;
+  if (stats && zero_stats(e)) return -1;
+  if (rk2_step(e, dt, st)) return -1;
+  if (stats) {
+    if (read_stats(e, stats)) return -1;
+    return stats->failures > 0 ? (int)std::min<long long>(stats->failures, 1 << 30) : 0;
+  }
+  return 0;
+}
+
+double cfl_dt(const mgpu_engine *e, double end_time) {
+  // Simulation::updateTime (simulation.cc:113-124), alpha = 1.
+  const double sd = std::sqrt((double)e->geo.dims);
+  const double dtX = e->cfg.cfl * e->dx / (1.0 * sd);
+  const double dtY = e->cfg.cfl * e->dy / (1.0 * sd);
+  const double dtZ = e->cfg.cfl * e->dz / (1.0 * sd);
+  double dt = (dtX <= dtY && dtX <= dtZ) ? dtX : ((dtY < dtZ) ? dtY : dtZ);
+  if (e->iters < 15) dt *= 0.1;
+  if (e->iters >= 15 && e->iters < 30) dt *= 0.25;
+  if (e->t + dt > end_time) dt = end_time - e->t;
+  return dt;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mgpu_last_error(void) { return g_err.c_str(); }
+
+int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
+  if (!cfg || !out) return fail("mgpu_create: null argument");
+  *out = nullptr;
+  const mgpu_config &c = *cfg;
+  if (c.model != MGPU_SRMHD) return fail("mgpu_create: model %d is not available in this build", c.model);
+  if (c.integrator != MGPU_RK2) return fail("mgpu_create: integrator %d is not available for SRMHD", c.integrator);
+  if (c.weno_order != 3 && c.weno_order != 5) return fail("mgpu_create: weno_order must be 3 or 5");
+  const int shift = (c.weno_order + 1) / 2;
+  if (c.Ng < shift + 1)  // WenoBase::checkSufficientGhostZones (weno.cc:13-20)
+    return fail("You must increase number of boundary cells. (order %d needs Ng >= %d, have %d)", c.weno_order,
+                shift + 1, c.Ng);
+  if (c.nx < 1 || c.ny < 1 || c.nz < 0) return fail("mgpu_create: need nx >= 1, ny >= 1 (2-D or 3-D), nz >= 0");
+  if (c.sigma < 0.0) return fail("Conductivity must be non-negative, sigma >= 0.");  // simData.cc:100-102
+  for (int ax = 0; ax < 3; ++ax)
+    if (c.bc[ax] != MGPU_BC_PERIODIC && c.bc[ax] != MGPU_BC_OUTFLOW) return fail("mgpu_create: bad bc on axis %d", ax);
+  const int n2 = c.nz > 0 ? c.nz : 1;
+  if (c.nx < c.Ng || c.ny < c.Ng || (c.nz > 0 && c.nz < c.Ng))
+    return fail("mgpu_create: every active axis needs at least Ng interior cells");
+  HIP_OK(hipSetDevice(c.device));
+
+  mgpu_engine *e = new mgpu_engine();
+  e->cfg = c;
+  e->ncons = kSrmhdCons;
+  e->nprims = kSrmhdPrims;
+  e->naux = kSrmhdAux;
+  Geo &g = e->geo;
+  g.n[0] = c.nx;
+  g.n[1] = c.ny;
+  g.n[2] = n2;
+  g.stride[2] = 1;
+  g.stride[1] = n2;
+  g.stride[0] = (long long)c.ny * n2;
+  g.vstride = (long long)c.nx * c.ny * n2;
+  g.dims = c.nz > 0 ? 3 : 2;
+  for (int ax = 0; ax < 3; ++ax) g.periodic[ax] = c.bc[ax] == MGPU_BC_PERIODIC;
+  // Data::initData / Simulation::Simulation (simData.cc:74-79, simulation.cc:34-36)
+  e->dx = (c.xmax - c.xmin) / c.nx;
+  e->dy = (c.ymax - c.ymin) / c.ny;
+  e->dz = c.nz > 0 ? (c.zmax - c.zmin) / c.nz : INFINITY;
+  g.inv_d[0] = 1.0 / e->dx;
+  g.inv_d[1] = 1.0 / e->dy;
+  g.inv_d[2] = c.nz > 0 ? 1.0 / e->dz : 0.0;
+  PadGeo &p = e->pad;
+  p.N[0] = c.nx + 2 * c.Ng;
+  p.N[1] = c.ny + 2 * c.Ng;
+  p.N[2] = c.nz > 0 ? c.nz + 2 * c.Ng : 1;
+  p.off[0] = c.Ng;
+  p.off[1] = c.Ng;
+  p.off[2] = c.nz > 0 ? c.Ng : 0;
+  p.ntot = (long long)p.N[0] * p.N[1] * p.N[2];
+
+  auto cleanup = [&](int rc) {
+    mgpu_destroy(e);
+    return rc;
+  };
+  const size_t n = (size_t)g.vstride;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail("stream create"));
+  struct {
+    double **ptr;
+    size_t elems;
+  } bufs[] = {{&e->U, n * e->ncons},  {&e->V, n * e->nprims}, {&e->A, n * e->naux},
+              {&e->U1, n * e->ncons}, {&e->V1, n * e->nprims}, {&e->fnet, n * e->ncons}};
+  for (auto &b : bufs) {
+    if (hipMalloc(b.ptr, b.elems * sizeof(double)) != hipSuccess)
+      return cleanup(fail("hipMalloc of %zu bytes failed", b.elems * sizeof(double)));
+    if (hipMemset(*b.ptr, 0, b.elems * sizeof(double)) != hipSuccess) return cleanup(fail("hipMemset failed"));
+  }
+  if (hipMalloc(&e->dstats, sizeof(C2PCounters)) != hipSuccess || hipMalloc(&e->fail_count, sizeof(int)) != hipSuccess ||
+      hipMalloc(&e->fail_list, n * sizeof(long long)) != hipSuccess || hipMalloc(&e->fail_flag, n) != hipSuccess)
+    return cleanup(fail("hipMalloc of C2P work arrays failed"));
+  if (hipMemset(e->fail_count, 0, sizeof(int)) != hipSuccess || hipMemset(e->fail_flag, 0, n) != hipSuccess ||
+      hipMemset(e->dstats, 0, sizeof(C2PCounters)) != hipSuccess)
+    return cleanup(fail("hipMemset failed"));
+  *out = e;
+  return 0;
+}
+
+int mgpu_destroy(mgpu_engine *e) {
+  if (!e) return 0;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  double *bufs[] = {e->U, e->V, e->A, e->U1, e->V1, e->fnet, e->staging};
+  for (double *b : bufs)
+    if (b) hipFree(b);
+  if (e->dstats) hipFree(e->dstats);
+  if (e->fail_count) hipFree(e->fail_count);
+  if (e->fail_list) hipFree(e->fail_list);
+  if (e->fail_flag) hipFree(e->fail_flag);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return 0;
+}
+
+int mgpu_sizes(const mgpu_engine *e, int *ncons, int *nprims, int *naux) {
+  if (!e) return fail("null engine");
+  if (ncons) *ncons = e->ncons;
+  if (nprims) *nprims = e->nprims;
+  if (naux) *naux = e->naux;
+  return 0;
+}
+
+int mgpu_upload_state(mgpu_engine *e, const double *cons, const double *prims, const double *aux) {
+  if (!e) return fail("null engine");
+  if (ensure_staging(e)) return -1;
+  const double *src[3] = {cons, prims, aux};
+  double *dst[3] = {e->U, e->V, e->A};
+  const int nv[3] = {e->ncons, e->nprims, e->naux};
+  for (int a = 0; a < 3; ++a) {
+    if (!src[a]) continue;
+    HIP_OK(hipMemcpyAsync(e->staging, src[a], sizeof(double) * nv[a] * e->pad.ntot, hipMemcpyHostToDevice, e->stream));
+    launch_compact(e->geo, e->pad, nv[a], e->staging, dst[a], e->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(e->stream));
+  }
+  return 0;
+}
+
+int mgpu_download_state(mgpu_engine *e, double *cons, double *prims, double *aux) {
+  if (!e) return fail("null engine");
+  if (ensure_staging(e)) return -1;
+  double *dst[3] = {cons, prims, aux};
+  const double *src[3] = {e->U, e->V, e->A};
+  const int nv[3] = {e->ncons, e->nprims, e->naux};
+  for (int a = 0; a < 3; ++a) {
+    if (!dst[a]) continue;
+    launch_expand(e->geo, e->pad, nv[a], src[a], e->staging, 0, e->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(dst[a], e->staging, sizeof(double) * nv[a] * e->pad.ntot, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+  }
+  return 0;
+}
+
+int mgpu_prims_to_all(mgpu_engine *e) {
+  if (!e) return fail("null engine");
+  launch_prims_to_all_srmhd(e->geo, e->V, e->U, e->A, e->cfg.gamma, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_step(mgpu_engine *e, double dt, mgpu_stats *stats) {
+  if (!e) return fail("null engine");
+  if (!(dt > 0.0)) return fail("mgpu_step: dt must be positive");
+  return do_step(e, dt, stats);
+}
+
+int mgpu_update_time(mgpu_engine *e, double end_time, mgpu_stats *stats) {
+  if (!e) return fail("null engine");
+  const double dt = cfl_dt(e, end_time);
+  if (!(dt > 0.0)) return fail("mgpu_update_time: end time reached");
+  e->dt = dt;
+  const int rc = do_step(e, dt, stats);
+  if (rc < 0) return rc;
+  e->t += dt;
+  e->iters++;
+  return rc;
+}
+
+int mgpu_get_time(const mgpu_engine *e, double *t, double *dt, int *iters) {
+  if (!e) return fail("null engine");
+  if (t) *t = e->t;
+  if (dt) *dt = e->dt;
+  if (iters) *iters = e->iters;
+  return 0;
+}
+
+int mgpu_set_time(mgpu_engine *e, double t, int iters) {
+  if (!e) return fail("null engine");
+  e->t = t;
+  e->iters = iters;
+  return 0;
+}
+
+int mgpu_fnet(mgpu_engine *e, double *fnet) {
+  if (!e || !fnet) return fail("null argument");
+  if (ensure_staging(e)) return -1;
+  SweepArgs a = sweep_args(e, e->U, e->V, nullptr);
+  if (run_sweeps(e, a, kSweepAdd)) return -1;
+  launch_expand(e->geo, e->pad, e->ncons, e->fnet, e->staging, 1, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(fnet, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_flux_vector(mgpu_engine *e, int dir, double *f) {
+  if (!e || !f) return fail("null argument");
+  if (dir < 0 || dir >= e->geo.dims) return fail("mgpu_flux_vector: dir %d out of range", dir);
+  if (ensure_staging(e)) return -1;
+  launch_flux_vector_srmhd(e->geo, e->pad, e->U, e->V, e->staging, dir, e->cfg.quirks, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(f, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
+  if (!e) return fail("null engine");
+  if (zero_stats(e)) return -1;
+  launch_c2p_srmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->dstats, e->fail_count, e->fail_list, e->fail_flag,
+                   e->stream);
+  HIP_OK(hipGetLastError());
+  if (retry(e, e->U, e->V, e->A, e->dstats)) return -1;
+  mgpu_stats local;
+  if (read_stats(e, &local)) return -1;
+  if (stats) *stats = local;
+  return local.failures > 0 ? (int)std::min<long long>(local.failures, 1 << 30) : 0;
+}
+
+int mgpu_sync(mgpu_engine *e) {
+  if (!e) return fail("null engine");
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_timed_steps(mgpu_engine *e, int nsteps, const double *dts, double *total_ms, double *kernel_ms,
+                     int kernel_slots, int *nslots) {
+  if (!e || !dts || nsteps <= 0) return fail("mgpu_timed_steps: bad arguments");
+  HIP_OK(hipStreamSynchronize(e->stream));
+  Marks marks;
+  e->marks = &marks;
+  int rc = mark(e, "start");
+  for (int s = 0; s < nsteps && rc == 0; ++s) rc = do_step(e, dts[s], nullptr);
+  e->marks = nullptr;
+  if (rc == 0) {
+    if (hipStreamSynchronize(e->stream) != hipSuccess) rc = fail("stream sync failed");
+  }
+  if (rc == 0) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, marks.ev.front(), marks.ev.back());
+    if (total_ms) *total_ms = ms;
+    std::vector<double> sum(e->slot_names.size(), 0.0);
+    std::vector<int> cnt(e->slot_names.size(), 0);
+    for (size_t i = 1; i < marks.ev.size(); ++i) {
+      float d = 0.f;
+      hipEventElapsedTime(&d, marks.ev[i - 1], marks.ev[i]);
+      sum[marks.slot[i]] += d;
+      cnt[marks.slot[i]] += 1;
+    }
+    if (nslots) *nslots = (int)e->slot_names.size();
+    if (kernel_ms)
+      for (int i = 0; i < kernel_slots && i < (int)sum.size(); ++i) kernel_ms[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
+  }
+  for (hipEvent_t ev : marks.ev) hipEventDestroy(ev);
+  return rc;
+}
+
+const char *mgpu_kernel_slot_name(const mgpu_engine *e, int slot) {
+  if (!e || slot < 0 || slot >= (int)e->slot_names.size()) return "";
+  return e->slot_names[slot].c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,465 @@
+// HIP kernels of the SRMHD time step on gfx950 (FP64).
+//
+// One RK2 stage (RK2.cc:31-91) is two or three directional sweep launches.
+// Each sweep stages a ghost-padded tile of Lax-Friedrichs split fluxes in LDS
+// (FVS::fluxReconstruction, fluxVectorSplitting.cc:4-56), reconstructs every
+// face of the tile with WENO (weno.cc:123-229) and differences neighbouring
+// faces into the cell's net flux (FVS::F, fluxVectorSplitting.cc:79-112). The
+// first sweep writes the net flux, later ones add to it, and the last one also
+// applies the stage update and the per-cell Newton cons2prim, so the stage's
+// new cons and prims leave the kernel without another pass over HBM.
+#include "mgpu_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace mgpu {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+  // Blocks are dispatched round robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so neighbouring tiles share its L2.
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, local = bid >> 3;
+  return x * q + (x < r ? x : r) + local;
+}
+
+// Block-level reduction of C2P counters into the device stats.
+__device__ __forceinline__ void reduce_stats(C2PCounters *st, int cells, int iters, int fails, int maxit) {
+  if (!st) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    cells += __shfl_xor(cells, off);
+    iters += __shfl_xor(iters, off);
+    fails += __shfl_xor(fails, off);
+    maxit = max(maxit, __shfl_xor(maxit, off));
+  }
+  if ((threadIdx.x & 63) == 0 && cells > 0) {
+    atomicAdd(&st->cells, (unsigned long long)cells);
+    atomicAdd(&st->iterations, (unsigned long long)iters);
+    atomicAdd(&st->failures, (unsigned long long)fails);
+    atomicMax(&st->max_iterations, (unsigned long long)maxit);
+  }
+}
+
+template <int ORDER, int DAX, int CAX, bool DFAST>
+struct SweepShape {
+  static constexpr int R = Weno<ORDER>::R;
+  static constexpr int TL = DFAST ? 32 : 16;  // cells along the sweep axis
+  static constexpr int TC = DFAST ? 8 : 16;   // pencils along the column axis
+  static constexpr int TLH = TL + 2 * R;      // with halo
+  static constexpr int EAX = 3 - DAX - CAX;
+  static constexpr int kFaces = (TL + 1) * TC;
+  static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
+  static_assert(kFaces <= 2 * kThreads, "at most two faces per thread");
+};
+
+// One directional sweep of the SRMHD flux divergence.
+template <int ORDER, int DAX, int CAX, bool DFAST, int MODE, bool QUIRK>
+__global__ __launch_bounds__(kThreads) void k_sweep_srmhd(SweepArgs a) {
+  using S = SweepShape<ORDER, DAX, CAX, DFAST>;
+  constexpr int R = S::R, TL = S::TL, TC = S::TC, TLH = S::TLH, EAX = S::EAX;
+  constexpr int NV = kSrmhdCons;
+  // f+ then f- for each variable: [2*NV][TLH][TC]; reused for face fluxes.
+  __shared__ double s_f[2 * NV * TLH * TC];
+
+  const Geo &g = a.geo;
+  const int nd = g.n[DAX], nc = g.n[CAX];
+  const int tiles_d = (nd + TL - 1) / TL;
+  const int tiles_c = (nc + TC - 1) / TC;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int td = lb % tiles_d;
+  const int rest = lb / tiles_d;
+  const int tc = rest % tiles_c;
+  const int e = rest / tiles_c;
+  const int d0 = td * TL, c0 = tc * TC;
+  const long long base_e = (long long)e * g.stride[EAX];
+  const long long vs = g.vstride;
+  const int tid = threadIdx.x;
+
+  // Phase 1: split fluxes of every tile cell and its halo along DAX.
+  for (int idx = tid; idx < TLH * TC; idx += kThreads) {
+    const int l = DFAST ? idx % TLH : idx / TC;
+    const int cc = DFAST ? idx / TLH : idx % TC;
+    const int gc = c0 + cc;
+    double fp[NV], fm[NV];
+    if (gc < nc) {
+      // cells past nd + R - 1 only feed faces outside the grid; clamping them
+      // keeps the periodic image inside [0, nd) for any tile overhang
+      const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
+      const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
+      double u[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
+      const double vx = a.V[1 * vs + q], vy = a.V[2 * vs + q], vz = a.V[3 * vs + q], p = a.V[4 * vs + q];
+      double f[NV];
+      srmhd_flux<DAX, QUIRK>(u, vx, vy, vz, p, f);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        fp[v] = 0.5 * (f[v] + u[v]);
+        fm[v] = 0.5 * (f[v] - u[v]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      s_f[(v * TLH + l) * TC + cc] = fp[v];
+      s_f[((NV + v) * TLH + l) * TC + cc] = fm[v];
+    }
+  }
+  __syncthreads();
+
+  // Phase 2: WENO face values, face lf being the left face of tile cell lf.
+  double face[2][NV];
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int idx = tid + rep * kThreads;
+    if (idx < S::kFaces) {
+      const int lf = idx / TC, cc = idx % TC;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double *fpv = s_f + (v * TLH + lf) * TC + cc;
+        const double *fmv = s_f + ((NV + v) * TLH + lf) * TC + cc;
+        face[rep][v] = Weno<ORDER>::face([&](int m) { return fpv[m * TC]; }, [&](int m) { return fmv[m * TC]; });
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int idx = tid + rep * kThreads;
+    if (idx < S::kFaces) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_f[v * (TL + 1) * TC + idx] = face[rep][v];
+    }
+  }
+  __syncthreads();
+
+  // Phase 3: net flux of each tile cell, then the sweep's mode.
+  const int l = DFAST ? tid % TL : tid / TC;
+  const int cc = DFAST ? tid / TL : tid % TC;
+  const int gd = d0 + l, gc = c0 + cc;
+  const bool live = gd < nd && gc < nc;
+  int c2p_cells = 0, c2p_iters = 0, c2p_fail = 0, c2p_max = 0;
+  if (live) {
+    const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
+    const double inv_d = g.inv_d[DAX];
+    double fn[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double *fv = s_f + v * (TL + 1) * TC;
+      fn[v] = (fv[(l + 1) * TC + cc] - fv[l * TC + cc]) * inv_d;
+    }
+    if (MODE == kSweepWrite) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
+    } else {
+      if (MODE != kSweepOnly) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fn[v] += a.fnet[v * vs + q];
+      }
+      if (MODE == kSweepAdd) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
+      } else {
+        // RK2 stage update (RK2.cc:58-66, 81-90) then cons2prim
+        // (TimeIntegrator::finalise, timeInt.h:117-132).
+        double un[NV];
+        if (MODE == kSweepStage1) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) un[v] = a.U[v * vs + q] - a.dt * fn[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) un[v] = 0.5 * (a.Un[v * vs + q] + a.U[v * vs + q] - a.dt * fn[v]);
+        }
+        double v2, Z;
+        srmhd_plain_guess(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                          a.Vg[4 * vs + q], a.gamma, v2, Z);
+        const double BS = un[5] * un[1] + un[6] * un[2] + un[7] * un[3];
+        const double Bsq = un[5] * un[5] + un[6] * un[6] + un[7] * un[7];
+        const double Ssq = un[1] * un[1] + un[2] * un[2] + un[3] * un[3];
+        const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
+        double P[kSrmhdPrims], A[kSrmhdAux];
+        srmhd_recover(un, a.gamma, v2, Z, P, A);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.Uout[v * vs + q] = un[v];
+#pragma unroll
+        for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+        if (a.Aout) {
+#pragma unroll
+          for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+        }
+        c2p_cells = 1;
+        if (its < 0) {
+          c2p_fail = 1;
+          const int slot = atomicAdd(a.fail_count, 1);
+          a.fail_list[slot] = q;
+          a.fail_flag[q] = 1;
+        } else {
+          c2p_iters = its;
+          c2p_max = its;
+        }
+      }
+    }
+  }
+  if (MODE == kSweepStage1 || MODE == kSweepStage2) reduce_stats(a.stats, c2p_cells, c2p_iters, c2p_fail, c2p_max);
+}
+
+// SRMHD::primsToAll over interior cells (srmhd.cc:573-673).
+__global__ __launch_bounds__(kThreads) void k_prims_to_all_srmhd(Geo g, const double *V, double *U, double *A,
+                                                                  double gam) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  const long long vs = g.vstride;
+  double P[8], u[9], aux[13];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) P[v] = V[v * vs + q];
+  srmhd_prims_to_all(P, gam, u, aux);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) U[v * vs + q] = u[v];
+#pragma unroll
+  for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+}
+
+// SRMHD::getPrimitiveVars on resident cons with resident prims as the guess
+// (srmhd.cc:390-562).
+__global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, double *V, double *A, double gam,
+                                                         C2PCounters *st, int *fail_count, long long *fail_list,
+                                                         unsigned char *fail_flag) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  int cells = 0, iters = 0, fails = 0, maxit = 0;
+  if (q < g.vstride) {
+    const long long vs = g.vstride;
+    double u[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
+    double v2, Z;
+    srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
+    const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
+    const double Bsq = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+    const double Ssq = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
+    const int its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
+    double P[8], aux[13];
+    srmhd_recover(u, gam, v2, Z, P, aux);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
+#pragma unroll
+    for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+    cells = 1;
+    if (its < 0) {
+      fails = 1;
+      const int slot = atomicAdd(fail_count, 1);
+      fail_list[slot] = q;
+      fail_flag[q] = 1;
+    } else {
+      iters = its;
+      maxit = its;
+    }
+  }
+  reduce_stats(st, cells, iters, fails, maxit);
+}
+
+// Retry of failed cells from the mean root of converged interior face
+// neighbours (srmhd.cc:465-507). With `last_cell` >= 0 the system solved is
+// that cell's, as the reference's retry reuses the last cell's arguments.
+__global__ __launch_bounds__(kThreads) void k_c2p_retry_srmhd(Geo g, const double *U, double *V, double *A,
+                                                               double gam, const int *fail_count,
+                                                               const long long *fail_list, unsigned char *fail_flag,
+                                                               long long last_cell, C2PCounters *st) {
+  const int nfail = *fail_count;
+  const long long vs = g.vstride;
+  for (int f = blockIdx.x * kThreads + threadIdx.x; f < nfail; f += gridDim.x * kThreads) {
+    const long long q = fail_list[f];
+    const int i = (int)(q / g.stride[0]);
+    const int j = (int)((q / g.stride[1]) % g.n[1]);
+    const int k = (int)(q % g.n[2]);
+    const int pos[3] = {i, j, k};
+    double sv2 = 0.0, sZ = 0.0;
+    int cnt = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+      for (int s = -1; s <= 1; s += 2) {
+        const int np = pos[ax] + s;
+        if (np < 0 || np >= g.n[ax] || g.n[ax] == 1) continue;
+        const long long m = q + s * g.stride[ax];
+        if (fail_flag[m]) continue;
+        double rho = V[m], vx = V[vs + m], vy = V[2 * vs + m], vz = V[3 * vs + m], p = V[4 * vs + m];
+        double v2, Z;
+        srmhd_plain_guess(rho, vx, vy, vz, p, gam, v2, Z);
+        sv2 += v2;
+        sZ += Z;
+        ++cnt;
+      }
+    }
+    double v2 = cnt ? sv2 / cnt : 0.0, Z = cnt ? sZ / cnt : 0.0;
+    const long long src = last_cell >= 0 ? last_cell : q;
+    double u[9];
+    for (int v = 0; v < 9; ++v) u[v] = U[v * vs + src];
+    const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
+    const double Bsq = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+    const double Ssq = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
+    const int its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
+    // recovery formulas use this cell's own cons, as srmhd.cc:509-557 does
+    for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
+    double P[8], aux[13];
+    srmhd_recover(u, gam, v2, Z, P, aux);
+    for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
+    if (A)
+      for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+    if (st && its < 0) atomicAdd(&st->retry_failures, 1ull);
+  }
+}
+
+__global__ void k_clear_fail_flags(const int *fail_count, const long long *fail_list, unsigned char *fail_flag) {
+  const int nfail = *fail_count;
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nfail; f += gridDim.x * blockDim.x)
+    fail_flag[fail_list[f]] = 0;
+}
+
+__global__ void k_zero_int(int *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// Padded reference layout -> interior device layout for nvar variables.
+__global__ __launch_bounds__(kThreads) void k_compact(Geo g, PadGeo pg, int nvar, const double *src, double *dst) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  const int i = (int)(q / g.stride[0]);
+  const int j = (int)((q / g.stride[1]) % g.n[1]);
+  const int k = (int)(q % g.n[2]);
+  const long long p = ((long long)(i + pg.off[0]) * pg.N[1] + (j + pg.off[1])) * pg.N[2] + (k + pg.off[2]);
+  for (int v = 0; v < nvar; ++v) dst[v * g.vstride + q] = src[v * pg.ntot + p];
+}
+
+// Interior device layout -> padded reference layout with every ghost cell
+// holding the value Bcs::apply gives it (per-axis remap, x then y then z).
+// ghosts_zero: leave ghosts at zero instead (FVS::F writes interior only).
+__global__ __launch_bounds__(kThreads) void k_expand(Geo g, PadGeo pg, int nvar, const double *src, double *dst,
+                                                      int ghosts_zero) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const int I = (int)(p / ((long long)pg.N[1] * pg.N[2]));
+  const int J = (int)((p / pg.N[2]) % pg.N[1]);
+  const int K = (int)(p % pg.N[2]);
+  const int c[3] = {I - pg.off[0], J - pg.off[1], K - pg.off[2]};
+  bool ghost = false;
+  long long q = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    int x = c[ax];
+    if (x < 0 || x >= g.n[ax]) ghost = true;
+    x = remap(x, g.n[ax], g.periodic[ax]);
+    q += (long long)x * g.stride[ax];
+  }
+  for (int v = 0; v < nvar; ++v) dst[v * pg.ntot + p] = (ghost && ghosts_zero) ? 0.0 : src[v * g.vstride + q];
+}
+
+// Model::fluxVector of the resident state for every padded cell (the padded
+// cell's value is its boundary image's, as after Bcs::apply).
+template <int DIR, bool QUIRK>
+__global__ __launch_bounds__(kThreads) void k_flux_vector_srmhd(Geo g, PadGeo pg, const double *U, const double *V,
+                                                                 double *dst) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const int I = (int)(p / ((long long)pg.N[1] * pg.N[2]));
+  const int J = (int)((p / pg.N[2]) % pg.N[1]);
+  const int K = (int)(p % pg.N[2]);
+  const int c[3] = {I - pg.off[0], J - pg.off[1], K - pg.off[2]};
+  long long q = 0;
+  for (int ax = 0; ax < 3; ++ax) q += (long long)remap(c[ax], g.n[ax], g.periodic[ax]) * g.stride[ax];
+  const long long vs = g.vstride;
+  double u[9], f[9];
+  for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
+  srmhd_flux<DIR, QUIRK>(u, V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], f);
+  for (int v = 0; v < 9; ++v) dst[v * pg.ntot + p] = f[v];
+}
+
+template <int ORDER, int DAX, int CAX, bool DFAST, int MODE>
+void launch_sweep_t(const SweepArgs &a, int quirk, hipStream_t s) {
+  using S = SweepShape<ORDER, DAX, CAX, DFAST>;
+  const int tiles = ((a.geo.n[DAX] + S::TL - 1) / S::TL) * ((a.geo.n[CAX] + S::TC - 1) / S::TC) * a.geo.n[S::EAX];
+  if (quirk)
+    hipLaunchKernelGGL((k_sweep_srmhd<ORDER, DAX, CAX, DFAST, MODE, true>), dim3(tiles), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep_srmhd<ORDER, DAX, CAX, DFAST, MODE, false>), dim3(tiles), dim3(kThreads), 0, s, a);
+}
+
+template <int ORDER, int MODE>
+void launch_sweep_o(const SweepArgs &a, int dir, int quirk, hipStream_t s) {
+  if (a.geo.dims == 3) {
+    if (dir == 0) launch_sweep_t<ORDER, 0, 2, false, MODE>(a, quirk, s);
+    else if (dir == 1) launch_sweep_t<ORDER, 1, 2, false, MODE>(a, quirk, s);
+    else launch_sweep_t<ORDER, 2, 1, true, MODE>(a, quirk, s);
+  } else {
+    if (dir == 0) launch_sweep_t<ORDER, 0, 1, false, MODE>(a, quirk, s);
+    else launch_sweep_t<ORDER, 1, 0, true, MODE>(a, quirk, s);
+  }
+}
+
+template <int ORDER>
+void launch_sweep_m(const SweepArgs &a, int dir, int mode, int quirk, hipStream_t s) {
+  switch (mode) {
+    case kSweepWrite: launch_sweep_o<ORDER, kSweepWrite>(a, dir, quirk, s); break;
+    case kSweepAdd: launch_sweep_o<ORDER, kSweepAdd>(a, dir, quirk, s); break;
+    case kSweepStage1: launch_sweep_o<ORDER, kSweepStage1>(a, dir, quirk, s); break;
+    case kSweepStage2: launch_sweep_o<ORDER, kSweepStage2>(a, dir, quirk, s); break;
+    default: break;
+  }
+}
+
+inline unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+void launch_sweep_srmhd(const SweepArgs &a, int order, int dir, int mode, int quirk, hipStream_t s) {
+  if (order == 3)
+    launch_sweep_m<3>(a, dir, mode, quirk, s);
+  else
+    launch_sweep_m<5>(a, dir, mode, quirk, s);
+}
+
+void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double *A, double gam, hipStream_t s) {
+  hipLaunchKernelGGL(k_prims_to_all_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, V, U, A, gam);
+}
+
+void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
+                      int *fail_count, long long *fail_list, unsigned char *fail_flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_c2p_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, gam, st, fail_count,
+                     fail_list, fail_flag);
+}
+
+void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, const int *fail_count,
+                            const long long *fail_list, unsigned char *fail_flag, long long last_cell,
+                            C2PCounters *st, hipStream_t s) {
+  hipLaunchKernelGGL(k_c2p_retry_srmhd, dim3(64), dim3(kThreads), 0, s, g, U, V, A, gam, fail_count, fail_list,
+                     fail_flag, last_cell, st);
+  hipLaunchKernelGGL(k_clear_fail_flags, dim3(64), dim3(kThreads), 0, s, fail_count, fail_list, fail_flag);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, const_cast<int *>(fail_count));
+}
+
+void launch_zero_int(int *p, hipStream_t s) { hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p); }
+
+void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_compact, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, pg, nvar, src, dst);
+}
+
+void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_expand, dim3(blocks_for(pg.ntot)), dim3(kThreads), 0, s, g, pg, nvar, src, dst, ghosts_zero);
+}
+
+void launch_flux_vector_srmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst, int dir,
+                              int quirk, hipStream_t s) {
+  const dim3 grid(blocks_for(pg.ntot)), block(kThreads);
+  if (dir == 0)
+    hipLaunchKernelGGL((k_flux_vector_srmhd<0, false>), grid, block, 0, s, g, pg, U, V, dst);
+  else if (dir == 1)
+    hipLaunchKernelGGL((k_flux_vector_srmhd<1, false>), grid, block, 0, s, g, pg, U, V, dst);
+  else if (quirk)
+    hipLaunchKernelGGL((k_flux_vector_srmhd<2, true>), grid, block, 0, s, g, pg, U, V, dst);
+  else
+    hipLaunchKernelGGL((k_flux_vector_srmhd<2, false>), grid, block, 0, s, g, pg, U, V, dst);
+}
+
+}  // namespace mgpu

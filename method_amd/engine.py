@@ -1,0 +1,248 @@
+"""Python front end of the engine's C ABI (include/method_gpu.h) via ctypes.
+
+The engine is the GPU body behind the reference's time-step API
+(Project/CPU/Include/simulation.h:84-115, timeInt.h:61-132, model.h:55-133,
+flux.h:47-72). Arrays cross the boundary as numpy float64 in the reference
+layout ``[var][i][j][k]`` over the ghost-padded grid (simData.h:10).
+
+There is no CPU fallback: if ``libmethod_gpu.so`` is missing or no GPU is
+present, constructing an Engine raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmethod_gpu.so")
+
+SRMHD, SRRMHD = 0, 1
+RK2, SSP2 = 0, 1
+OUTFLOW, PERIODIC = 0, 1
+
+EXPORTED = [
+    "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
+    "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
+    "mgpu_flux_vector", "mgpu_c2p", "mgpu_sync", "mgpu_timed_steps", "mgpu_kernel_slot_name",
+]
+
+
+class MgpuConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("Ng", ctypes.c_int),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double), ("ymin", ctypes.c_double),
+        ("ymax", ctypes.c_double), ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("sigma", ctypes.c_double), ("cp", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("model", ctypes.c_int), ("weno_order", ctypes.c_int), ("integrator", ctypes.c_int),
+        ("bc", ctypes.c_int * 3), ("quirks", ctypes.c_int), ("device", ctypes.c_int),
+    ]
+
+
+class MgpuStats(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_longlong), ("iterations", ctypes.c_longlong),
+                ("failures", ctypes.c_longlong), ("max_iterations", ctypes.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libmethod_gpu.so and declares the C ABI; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(f"HIP engine library not built: {path} (run `python -m method_amd.build`)")
+    L = ctypes.CDLL(path)
+    E = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    sp = ctypes.POINTER(MgpuStats)
+    L.mgpu_last_error.restype = ctypes.c_char_p
+    L.mgpu_create.argtypes = [ctypes.POINTER(MgpuConfig), ctypes.POINTER(E)]
+    L.mgpu_destroy.argtypes = [E]
+    L.mgpu_sizes.argtypes = [E, ctypes.POINTER(ctypes.c_int)] * 1 + [ctypes.POINTER(ctypes.c_int)] * 2
+    L.mgpu_upload_state.argtypes = [E, dp, dp, dp]
+    L.mgpu_download_state.argtypes = [E, dp, dp, dp]
+    L.mgpu_prims_to_all.argtypes = [E]
+    L.mgpu_step.argtypes = [E, ctypes.c_double, sp]
+    L.mgpu_update_time.argtypes = [E, ctypes.c_double, sp]
+    L.mgpu_get_time.argtypes = [E, dp, dp, ctypes.POINTER(ctypes.c_int)]
+    L.mgpu_set_time.argtypes = [E, ctypes.c_double, ctypes.c_int]
+    L.mgpu_fnet.argtypes = [E, dp]
+    L.mgpu_flux_vector.argtypes = [E, ctypes.c_int, dp]
+    L.mgpu_c2p.argtypes = [E, sp]
+    L.mgpu_sync.argtypes = [E]
+    L.mgpu_timed_steps.argtypes = [E, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mgpu_kernel_slot_name.argtypes = [E, ctypes.c_int]
+    L.mgpu_kernel_slot_name.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class Config:
+    """Grid and scheme of one simulation (the reference Data constructor,
+    simData.h:205-218, plus the model / WENO / integrator / Bcs choice)."""
+
+    nx: int
+    ny: int
+    nz: int
+    xmin: float
+    xmax: float
+    ymin: float
+    ymax: float
+    zmin: float = 0.0
+    zmax: float = 1.0
+    Ng: int = 4
+    gamma: float = 4.0 / 3.0
+    sigma: float = 0.0
+    cp: float = 0.1
+    cfl: float = 0.4
+    model: int = SRMHD
+    weno_order: int = 3
+    integrator: int = RK2
+    bc: tuple = (PERIODIC, OUTFLOW, PERIODIC)
+    quirks: int = 1
+    device: int = 0
+
+    def to_c(self) -> MgpuConfig:
+        c = MgpuConfig()
+        for f in ("nx", "ny", "nz", "Ng", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "gamma", "sigma",
+                  "cp", "cfl", "model", "weno_order", "integrator", "quirks", "device"):
+            setattr(c, f, getattr(self, f))
+        for a in range(3):
+            c.bc[a] = int(self.bc[a])
+        return c
+
+    @property
+    def shape(self):
+        return (self.nx + 2 * self.Ng, self.ny + 2 * self.Ng, self.nz + 2 * self.Ng if self.nz > 0 else 1)
+
+    @property
+    def interior_cells(self) -> int:
+        return self.nx * self.ny * max(self.nz, 1)
+
+
+class Engine:
+    """One device-resident simulation driven through the C ABI."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.L = load_library()
+        self._h = ctypes.c_void_p()
+        c = cfg.to_c()
+        self._check(self.L.mgpu_create(ctypes.byref(c), ctypes.byref(self._h)), "mgpu_create")
+        nc, np_, na = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.mgpu_sizes(self._h, ctypes.byref(nc), ctypes.byref(np_), ctypes.byref(na)), "mgpu_sizes")
+        self.ncons, self.nprims, self.naux = nc.value, np_.value, na.value
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise EngineError(f"{what}: {self.L.mgpu_last_error().decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.L.mgpu_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state ---------------------------------------------------------------
+    def zeros(self, nvar: int) -> np.ndarray:
+        return np.zeros((nvar,) + self.cfg.shape)
+
+    def upload(self, cons=None, prims=None, aux=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (cons, prims, aux)]
+        self._check(self.L.mgpu_upload_state(self._h, *[_dp(a) for a in arrs]), "mgpu_upload_state")
+
+    def download(self):
+        cons, prims, aux = self.zeros(self.ncons), self.zeros(self.nprims), self.zeros(self.naux)
+        self._check(self.L.mgpu_download_state(self._h, _dp(cons), _dp(prims), _dp(aux)), "mgpu_download_state")
+        return cons, prims, aux
+
+    def set_prims(self, prims: np.ndarray):
+        """Simulation::set: upload primitives, primsToAll, boundaries (simulation.cc:94-96)."""
+        self.upload(prims=prims)
+        self._check(self.L.mgpu_prims_to_all(self._h), "mgpu_prims_to_all")
+
+    # time stepping -------------------------------------------------------
+    def step(self, dt: float, stats: bool = False):
+        st = MgpuStats() if stats else None
+        rc = self._check(self.L.mgpu_step(self._h, dt, ctypes.byref(st) if st else None), "mgpu_step")
+        return (rc, st.as_dict()) if stats else rc
+
+    def update_time(self, end_time: float = 1e10, stats: bool = False):
+        st = MgpuStats() if stats else None
+        rc = self._check(self.L.mgpu_update_time(self._h, end_time, ctypes.byref(st) if st else None),
+                         "mgpu_update_time")
+        return (rc, st.as_dict()) if stats else rc
+
+    def time(self):
+        t, dt, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(self.L.mgpu_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(it)), "get_time")
+        return t.value, dt.value, it.value
+
+    def set_time(self, t: float, iters: int):
+        self._check(self.L.mgpu_set_time(self._h, t, iters), "mgpu_set_time")
+
+    def sync(self):
+        self._check(self.L.mgpu_sync(self._h), "mgpu_sync")
+
+    # model / flux-method calls on the resident state ----------------------
+    def fnet(self) -> np.ndarray:
+        out = self.zeros(self.ncons)
+        self._check(self.L.mgpu_fnet(self._h, _dp(out)), "mgpu_fnet")
+        return out
+
+    def flux_vector(self, dir: int) -> np.ndarray:
+        out = self.zeros(self.ncons)
+        self._check(self.L.mgpu_flux_vector(self._h, dir, _dp(out)), "mgpu_flux_vector")
+        return out
+
+    def c2p(self):
+        st = MgpuStats()
+        rc = self._check(self.L.mgpu_c2p(self._h, ctypes.byref(st)), "mgpu_c2p")
+        return rc, st.as_dict()
+
+    # timing ----------------------------------------------------------------
+    def timed_steps(self, dts, max_slots: int = 32):
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        total = ctypes.c_double()
+        kms = np.zeros(max_slots)
+        ns = ctypes.c_int()
+        self._check(self.L.mgpu_timed_steps(self._h, len(dts), _dp(dts), ctypes.byref(total), _dp(kms), max_slots,
+                                            ctypes.byref(ns)), "mgpu_timed_steps")
+        slots = {}
+        for i in range(ns.value):
+            name = self.L.mgpu_kernel_slot_name(self._h, i).decode()
+            if name != "start":
+                slots[name] = kms[i]
+        return total.value, slots

@@ -1,0 +1,238 @@
+"""Parity of the HIP SRMHD time step with the reference (golden fixtures) and
+with the CPU oracle, through the C ABI. Needs an MI355X."""
+import sys
+
+import numpy as np
+import pytest
+
+from method_amd import engine as eng
+from method_amd import initdata as idt
+from oracle import oracle as orc
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+KH2D = dict(nx=64, ny=64, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+KH3D16 = dict(nx=16, ny=16, nz=16, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def engine(weno=3, bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), quirks=1, **grid):
+    return eng.Engine(eng.Config(weno_order=weno, bc=bc, quirks=quirks, **grid))
+
+
+def oracle_for(weno=3, periodic=(1, 0, 1), quirks=1, **grid):
+    return orc.Oracle(orc.Case(weno_order=weno, periodic=periodic, quirks=quirks, **grid))
+
+
+def check_rows(rows, tol, use_group=False):
+    for n, dm, fm, rf, rg, pw in rows:
+        err = rg if use_group else rf
+        assert err <= tol, f"{n}: {err:.3e} > {tol:.1e}"
+        if fm == 0.0:
+            assert dm == 0.0, f"{n} must stay exactly zero"
+
+
+def test_set_matches_oracle_including_ghosts(golden):
+    z = golden("kh2d_64_weno3_rk2")
+    say("gpu: set_prims KH 2-D 64^2")
+    with engine(**KH2D) as e:
+        e.set_prims(z["prims0"])
+        c, p, a = e.download()
+    oc, op, oa = oracle_for(**KH2D).set(z["prims0"])
+    for got, ref, names in ((c, oc, parity.SRMHD_CONS), (p, op, parity.SRMHD_PRIMS), (a, oa, parity.SRMHD_AUX)):
+        rows = parity.field_errors(got, ref, names)  # whole padded grid, ghosts included
+        say(parity.report("gpu vs oracle after Simulation::set (padded grid)", rows))
+        check_rows(rows, 1e-14, use_group=True)
+
+
+@pytest.mark.parametrize("dir", [0, 1])
+def test_flux_vector_matches_oracle_2d(golden, dir):
+    z = golden("kh2d_64_weno3_rk2")
+    say(f"gpu: flux_vector dir={dir}")
+    with engine(**KH2D) as e:
+        e.set_prims(z["prims0"])
+        f = e.flux_vector(dir)
+    o = oracle_for(**KH2D)
+    c, p, a = o.set(z["prims0"])
+    ref = o.flux_vector(c, p, a, dir)
+    rows = parity.field_errors(f, ref, parity.SRMHD_CONS)
+    check_rows(rows, 1e-13, use_group=True)
+
+
+@pytest.mark.parametrize("quirks", [1, 0])
+def test_flux_vector_z_phi_quirk(quirks):
+    g = idt.kh3d_grid(16)
+    prims0 = idt.kh_prims(g, seed=5)
+    prims0[5] = 0.03  # Bx != 0 so B has all components
+    prims0[6] = -0.02
+    say(f"gpu: flux_vector z quirks={quirks}")
+    with engine(weno=5, quirks=quirks, **KH3D16) as e:
+        e.set_prims(prims0)
+        f = e.flux_vector(2)
+    o = oracle_for(weno=5, quirks=quirks, **KH3D16)
+    c, p, a = o.set(prims0)
+    ref = o.flux_vector(c, p, a, 2)
+    check_rows(parity.field_errors(f, ref, parity.SRMHD_CONS), 1e-13, use_group=True)
+    assert np.allclose(f[8], p[6] if quirks else p[7], rtol=0, atol=0)
+
+
+def test_fnet_matches_reference_2d_weno3(golden):
+    z = golden("kh2d_64_weno3_rk2")
+    say("gpu: fnet KH 2-D WENO3")
+    with engine(**KH2D) as e:
+        e.set_prims(z["prims0"])
+        f = e.fnet()
+    rows = parity.field_errors(parity.interior(f, 4, 2), z["fnet0"], parity.SRMHD_CONS)
+    say(parity.report("gpu vs reference FVS::F, KH 2-D WENO3", rows))
+    check_rows(rows, 1e-12, use_group=True)
+    # ghost cells of fnet stay zero, as FVS::F writes the interior only
+    assert np.all(f[:, :4] == 0) and np.all(f[:, -4:] == 0)
+
+
+def test_fnet_matches_reference_3d_weno5(golden):
+    z = golden("kh3d_16_weno5_rk2")
+    say("gpu: fnet KH 3-D WENO5")
+    with engine(weno=5, **KH3D16) as e:
+        e.set_prims(z["prims0"])
+        f = e.fnet()
+    rows = parity.field_errors(parity.interior(f, 4, 3), z["fnet0"], parity.SRMHD_CONS)
+    say(parity.report("gpu vs reference FVS::F, KH 3-D WENO5", rows))
+    check_rows(rows, 1e-12, use_group=True)
+
+
+def test_c2p_vectors_round_trip_and_reference(golden):
+    z = golden("srmhd_c2p_vectors")
+    n = int(z["n"][0])
+    # a 2-D strip n x 4 holding the 1-D vectors in its first row is not the
+    # same grid; use an n x Ng grid whose every row repeats the vectors
+    ny = 4
+    shape = (n + 8, ny + 8, 1)
+
+    def widen(a):
+        out = np.zeros((a.shape[0],) + shape)
+        out[:, :, :, 0] = a[:, :, 0, 0][:, :, None]
+        return out
+
+    say("gpu: c2p vectors")
+    with eng.Engine(eng.Config(nx=n, ny=ny, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0)) as e:
+        e.upload(cons=widen(z["cons"]), prims=widen(z["guess_prims"]), aux=widen(z["guess_aux"]))
+        rc, st = e.c2p()
+        _, p, _ = e.download()
+    assert rc == 0 and st["failures"] == 0 and st["cells"] == n * ny
+    got = p[:, 4:-4, 4, 0]
+    pt, pr = z["prims_true"][:, 4:-4, 0, 0], z["prims_ref"][:, 4:-4, 0, 0]
+    scale = np.abs(pt).max(axis=1, keepdims=True)
+    say(f"c2p round trip max rel {np.max(np.abs(got - pt) / scale):.3e}, vs reference hybrd "
+        f"{np.max(np.abs(got - pr) / scale):.3e}, mean Newton steps {st['iterations'] / st['cells']:.3f}")
+    assert (np.abs(got - pt) / scale).max() < 1e-11
+    assert (np.abs(got - pr) / scale).max() < 1e-7
+
+
+def test_kh2d_ten_steps_match_reference(golden):
+    """The parity gate: KH 2-D 64^2 WENO3 RK2 cfl 0.4, 10 updateTime calls."""
+    z = golden("kh2d_64_weno3_rk2")
+    say("gpu: KH 2-D 64^2 10 steps vs reference")
+    with engine(**KH2D) as e:
+        e.set_prims(z["prims0"])
+        total = {"cells": 0, "iterations": 0, "failures": 0}
+        for s in range(10):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0
+            for k in total:
+                total[k] += st[k]
+            t, dt, it = e.time()
+            assert dt == z["dts"][s]
+        c, p, a = e.download()
+    say(f"C2P: {total['cells']} cells, {total['iterations'] / total['cells']:.3f} Newton steps/cell, "
+        f"{total['failures']} failures")
+    for got, ref, names in ((c, z["cons10"], parity.SRMHD_CONS), (p, z["prims10"], parity.SRMHD_PRIMS),
+                            (a, z["aux10"], parity.SRMHD_AUX)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        say(parity.report("gpu vs reference, KH 2-D 64^2 WENO3 RK2, 10 steps", rows))
+        check_rows(rows, 1e-9)
+    # ghost cells hold what the reference's Flow boundaries leave there
+    for got, ref in ((c, z["cons10"]), (p, z["prims10"])):
+        assert np.max(np.abs(got - ref)) <= 1e-9 * max(1.0, np.abs(ref).max())
+
+
+def test_kh3d_weno5_three_steps_match_reference(golden):
+    z = golden("kh3d_16_weno5_rk2")
+    say("gpu: KH 3-D 16^3 WENO5 3 steps vs reference")
+    with engine(weno=5, **KH3D16) as e:
+        e.set_prims(z["prims0"])
+        for s in range(3):
+            assert e.update_time() == 0
+        c, p, a = e.download()
+    for got, ref, names in ((c, z["cons3"], parity.SRMHD_CONS), (p, z["prims3"], parity.SRMHD_PRIMS),
+                            (a, z["aux3"], parity.SRMHD_AUX)):
+        rows = parity.field_errors(parity.interior(got, 4, 3), ref, names)
+        say(parity.report("gpu vs reference, KH 3-D 16^3 WENO5 RK2, 3 steps", rows))
+        check_rows(rows, 1e-9, use_group=True)
+
+
+@pytest.mark.parametrize("n, weno, bc", [(40, 5, (1, 0, 1)), (37, 3, (0, 1, 0))])
+def test_3d_multi_tile_matches_oracle(n, weno, bc):
+    """Grids spanning several tiles (and ragged ones) against the oracle, 2 steps."""
+    g = idt.Grid(n, n + 3, n - 2, -0.5, 0.5, -1.0, 1.0, 0.0, 1.0)
+    prims0 = idt.kh_prims(g, seed=11)
+    grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=g.xmin, xmax=g.xmax, ymin=g.ymin, ymax=g.ymax, zmin=0.0, zmax=1.0)
+    say(f"gpu: 3-D {g.nx}x{g.ny}x{g.nz} WENO{weno} bc={bc} vs oracle")
+    o = oracle_for(weno=weno, periodic=bc, **grid)
+    oc, op, oa, _, _ = o.run(prims0, 2)
+    with engine(weno=weno, bc=tuple(bc), **grid) as e:
+        e.set_prims(prims0)
+        for _ in range(2):
+            assert e.update_time() == 0
+        c, p, a = e.download()
+    for got, ref, names in ((c, oc, parity.SRMHD_CONS), (p, op, parity.SRMHD_PRIMS)):
+        rows = parity.field_errors(got, ref, names)  # padded, ghosts included
+        say(parity.report(f"gpu vs oracle {g.nx}x{g.ny}x{g.nz}", rows))
+        check_rows(rows, 1e-11, use_group=True)
+
+
+def test_uniform_state_is_stationary():
+    # test_simulation.cc:78-125: a homogeneous state is unchanged by evolve
+    g = idt.kh3d_grid(32)
+    prims0 = np.zeros((8,) + g.shape)
+    prims0[0], prims0[1], prims0[2], prims0[3], prims0[4] = 0.8, 0.3, -0.2, 0.1, 1.3
+    prims0[5], prims0[6], prims0[7] = 0.2, -0.1, 0.4
+    say("gpu: uniform state 32^3")
+    with engine(weno=5, nx=32, ny=32, nz=32, xmin=-0.5, xmax=0.5, ymin=-1, ymax=1, zmin=0, zmax=1) as e:
+        e.set_prims(prims0)
+        c0, p0, _ = e.download()
+        for _ in range(3):
+            e.update_time()
+        c, p, _ = e.download()
+    assert np.max(np.abs(c - c0)) <= 1e-14 * np.abs(c0).max()
+    assert np.max(np.abs(p - p0)) <= 1e-13 * np.abs(p0).max()
+
+
+def test_full_size_periodic_shift_invariance():
+    """256^3 (the benchmark grid): shifting the initial data periodically in x
+    and z shifts the result exactly; no cell fails its Newton solve."""
+    g = idt.kh3d_grid(256)
+    prims0 = idt.kh_prims(g, seed=1234)
+    grid = dict(nx=256, ny=256, nz=256, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    sx, sz = 37, 101
+    inner = prims0[:, 4:-4, :, 4:-4]
+    shifted = prims0.copy()
+    shifted[:, 4:-4, :, 4:-4] = np.roll(np.roll(inner, sx, axis=1), sz, axis=3)
+    outs = []
+    for k, p0 in enumerate((prims0, shifted)):
+        say(f"gpu: 256^3 WENO5 run {k}")
+        with engine(weno=5, **grid) as e:
+            e.set_prims(p0)
+            for _ in range(2):
+                rc, st = e.update_time(stats=True)
+                assert rc == 0 and st["failures"] == 0
+            c, _, _ = e.download()
+            outs.append(c[:, 4:-4, 4:-4, 4:-4].copy())
+            del c
+    a, b = outs
+    assert np.isfinite(a).all()
+    assert np.array_equal(np.roll(np.roll(a, sx, axis=1), sz, axis=3), b)

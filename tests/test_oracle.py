@@ -1,0 +1,217 @@
+"""The CPU oracle, pinned before it is trusted as the checker.
+
+Pins: the reference's own known-answer tests (Tests/CPU/Src/test_weno.cc,
+test_srmhd.cc, test_simulation.cc, test_boundaryConds.cc), the reference build's
+outputs stored under tests/golden/ (make_fixtures.py), and closed forms.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests import parity
+
+KH2D = dict(nx=64, ny=64, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+KH3D = dict(nx=16, ny=16, nz=16, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0)
+
+
+# --------------------------------------------------------------------- WENO
+def test_weno_linear_kats():
+    # test_weno.cc:37-82 (WENO3 up 1.5, down 0.5) and :161-210 (WENO5 up 2.5, down 1.5)
+    assert abs(orc.weno3(0.0, 1.0, 2.0) - 1.5) < 1e-15
+    assert abs(orc.weno3(2.0, 1.0, 0.0) - 0.5) < 1e-15
+    assert abs(orc.weno5(0.0, 1.0, 2.0, 3.0, 4.0) - 2.5) < 1e-15
+    assert abs(orc.weno5(4.0, 3.0, 2.0, 1.0, 0.0) - 1.5) < 1e-15
+
+
+def _jiang_shu5(v):
+    """Jiang-Shu WENO5 from its textbook definition, in exact rationals then float."""
+    from fractions import Fraction as Fr
+
+    v = [Fr(x) for x in v]
+    b = [Fr(13, 12) * (v[0] - 2 * v[1] + v[2]) ** 2 + Fr(1, 4) * (v[0] - 4 * v[1] + 3 * v[2]) ** 2,
+         Fr(13, 12) * (v[1] - 2 * v[2] + v[3]) ** 2 + Fr(1, 4) * (v[1] - v[3]) ** 2,
+         Fr(13, 12) * (v[2] - 2 * v[3] + v[4]) ** 2 + Fr(1, 4) * (3 * v[2] - 4 * v[3] + v[4]) ** 2]
+    q = [(2 * v[0] - 7 * v[1] + 11 * v[2]) / 6, (-v[1] + 5 * v[2] + 2 * v[3]) / 6, (2 * v[2] + 5 * v[3] - v[4]) / 6]
+    d = [Fr(1, 10), Fr(6, 10), Fr(3, 10)]
+    eps = Fr(1, 10 ** 16)
+    a = [d[k] / (eps + b[k] ** 2) for k in range(3)]
+    return float(sum(a[k] * q[k] for k in range(3)) / sum(a))
+
+
+def _jiang_shu3(v):
+    from fractions import Fraction as Fr
+
+    v = [Fr(x) for x in v]
+    b = [(v[1] - v[0]) ** 2, (v[2] - v[1]) ** 2]
+    q = [(-v[0] + 3 * v[1]) / 2, (v[1] + v[2]) / 2]
+    d = [Fr(1, 3), Fr(2, 3)]
+    eps = Fr(1, 10 ** 16)
+    a = [d[k] / (eps + b[k] ** 2) for k in range(2)]
+    return float(sum(a[k] * q[k] for k in range(2)) / sum(a))
+
+
+def test_weno_against_exact_rational_definition():
+    rng = np.random.default_rng(3)
+    for scale in (1e-3, 1.0, 10.0):
+        for _ in range(50):
+            v = (rng.normal(size=5) * scale).tolist()
+            ref5 = _jiang_shu5(v)
+            assert abs(orc.weno5(*v) - ref5) <= 1e-13 * max(1.0, abs(ref5)) * max(1.0, scale)
+            ref3 = _jiang_shu3(v[:3])
+            assert abs(orc.weno3(*v[:3]) - ref3) <= 1e-13 * max(1.0, abs(ref3)) * max(1.0, scale)
+    # a discontinuity picks the smooth side (essentially non-oscillatory)
+    assert abs(orc.weno5(0, 0, 0, 1, 1) - 0.0) < 1e-6
+
+
+# ------------------------------------------------------------ primsToAll
+def test_prims_to_all_closed_form():
+    # test_srmhd.cc:228-282: prims = 1e-16, gamma = 5/3 -> h 3.5, W 1, e 1.5, c sqrt(10/21)
+    case = orc.Case(10, 10, 10, 0, 1, 0, 1, 0, 1, gamma=5.0 / 3.0)
+    o = orc.Oracle(case)
+    _, p, _ = case.zeros()
+    p[:] = 1e-16
+    cons, prims, aux = o.prims_to_all(p)
+    assert np.abs(cons).max() <= 1e-15
+    assert np.all(aux[0] == 3.5)
+    assert np.abs(aux[1] - 1.0).max() <= 1e-15
+    assert np.all(aux[2] == 1.5)
+    assert np.all(aux[3] == math.sqrt(10.0 / 21.0))
+    assert np.abs(aux[4:]).max() <= 1e-15
+
+
+# --------------------------------------------------------------------- dt
+def test_timestep_closed_forms():
+    # test_simulation.cc:34-72: 100x10x2 on [0,1]x[-0.5,0.5]x[-0.1,0.1], cfl 0.5 -> 0.002886751346
+    case = orc.Case(100, 10, 2, 0.0, 1.0, -0.5, 0.5, -0.1, 0.1, cfl=0.5, Ng=4)
+    o = orc.Oracle(case)
+    assert abs(o.timestep(0.0, 100) - 0.002886751346) < 1e-11
+    assert abs(o.timestep(0.0, 0) - 0.1 * 0.002886751346) < 1e-12
+    assert abs(o.timestep(0.0, 20) - 0.25 * 0.002886751346) < 1e-12
+    assert o.timestep(1.0, 100, end_time=1.0001) == pytest.approx(1e-4)
+    # 2-D KH 64^2: 0.1 * 0.4 * (1/64) / sqrt(2)
+    o2 = orc.Oracle(orc.Case(**KH2D))
+    assert o2.timestep(0.0, 0) == 0.1 * (0.4 * (1.0 / 64) / math.sqrt(2))
+
+
+# -------------------------------------------------------------------- BCs
+def test_boundary_index_patterns():
+    # test_boundaryConds.cc: periodic copies nx+i -> i and Ng+i -> nx+Ng+i; outflow copies the edge cell
+    case = orc.Case(10, 10, 10, 0, 1, 0, 1, 0, 1, periodic=(1, 0, 1))
+    o = orc.Oracle(case)
+    Nx, Ny, Nz = case.shape
+    a = np.arange(Nx * Ny * Nz, dtype=np.float64).reshape(1, Nx, Ny, Nz).copy()
+    b = a.copy()
+    o.apply_bcs(b)
+    Ng, nx, ny = 4, 10, 10
+    for i in range(Ng):
+        assert np.array_equal(b[0, i, Ng:-Ng, Ng:-Ng], a[0, nx + i, Ng:-Ng, Ng:-Ng])
+        assert np.array_equal(b[0, nx + Ng + i, Ng:-Ng, Ng:-Ng], a[0, Ng + i, Ng:-Ng, Ng:-Ng])
+        assert np.array_equal(b[0, Ng:-Ng, i, Ng:-Ng], a[0, Ng:-Ng, Ng, Ng:-Ng])
+        assert np.array_equal(b[0, Ng:-Ng, ny + Ng + i, Ng:-Ng], a[0, Ng:-Ng, ny + Ng - 1, Ng:-Ng])
+    # corner: composition of the per-axis maps
+    assert b[0, 0, 0, 0] == a[0, nx + 0, Ng, 10 + 0]
+
+
+# -------------------------------------------------------------------- C2P
+def test_c2p_round_trip_and_reference_vectors(golden):
+    z = golden("srmhd_c2p_vectors")
+    n = int(z["n"][0])
+    case = orc.Case(n, 0, 0, 0.0, 1.0, 0.0, 1.0, periodic=(0, 0, 0))
+    o = orc.Oracle(case)
+    p, a = z["guess_prims"].copy(), z["guess_aux"].copy()
+    fails, st = o.c2p(z["cons"], p, a)
+    assert fails == 0 and st.failures == 0
+    sl = (slice(None), slice(4, -4))
+    pt, pr = z["prims_true"][sl], z["prims_ref"][sl]
+    got = p[sl]
+    scale = np.abs(pt).max(axis=1, keepdims=True)
+    # round trip to the exact primitives
+    assert (np.abs(got - pt) / np.maximum(scale, 1e-300)).max() < 1e-11
+    # the reference's hybrd roots are within its own tolerance of the same root
+    assert (np.abs(got - pr) / np.maximum(scale, 1e-300)).max() < 1e-7
+
+
+def test_srmhd_newton_special_cases():
+    L = orc.lib()
+    import ctypes
+
+    g = 4.0 / 3.0
+    # rest state (v = 0): Z = rho h
+    rho, p = 0.7, 0.3
+    h = 1 + p / rho * g / (g - 1)
+    D, tau = rho, rho * h - p - rho
+    x = (ctypes.c_double * 2)(0.2, 1.0)
+    its = L.oc_srmhd_newton(D, 0.0, 0.0, 0.0, tau, 0.0, 0.0, 0.0, g, x)
+    assert its >= 0 and abs(x[0]) < 1e-13 and abs(x[1] - rho * h) < 1e-12
+    # hydro (B = 0): closed form cons from prims, root equals rho h W^2
+    v = 0.6
+    W = 1 / math.sqrt(1 - v * v)
+    D = rho * W
+    S = rho * h * W * W * v
+    tau = rho * h * W * W - p - D
+    x = (ctypes.c_double * 2)(0.3, 2.0)
+    its = L.oc_srmhd_newton(D, S, 0.0, 0.0, tau, 0.0, 0.0, 0.0, g, x)
+    assert its >= 0 and abs(x[0] - v * v) < 1e-13 and abs(x[1] - rho * h * W * W) < 1e-12
+
+
+# ------------------------------------------------- reference trajectories
+def test_kh2d_fnet_matches_reference(golden):
+    z = golden("kh2d_64_weno3_rk2")
+    o = orc.Oracle(orc.Case(**KH2D, weno_order=3, periodic=(1, 0, 1)))
+    c, p, a = o.set(z["prims0"])
+    f = parity.interior(o.fnet(c, p, a), 4, 2)
+    assert np.array_equal(f, z["fnet0"])
+
+
+def test_kh3d_fnet_matches_reference(golden):
+    z = golden("kh3d_16_weno5_rk2")
+    o = orc.Oracle(orc.Case(**KH3D, weno_order=5, periodic=(1, 0, 1)))
+    c, p, a = o.set(z["prims0"])
+    f = parity.interior(o.fnet(c, p, a), 4, 3)
+    assert np.array_equal(f, z["fnet0"])
+
+
+def test_kh2d_ten_steps_match_reference(golden):
+    z = golden("kh2d_64_weno3_rk2")
+    o = orc.Oracle(orc.Case(**KH2D, weno_order=3, periodic=(1, 0, 1)))
+    c, p, a, dts, st = o.run(z["prims0"], 10)
+    assert np.array_equal(np.array(dts), z["dts"])
+    assert st.failures == 0
+    for got, ref, names in ((c, z["cons10"], parity.SRMHD_CONS), (p, z["prims10"], parity.SRMHD_PRIMS),
+                            (a, z["aux10"], parity.SRMHD_AUX)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        print(parity.report("oracle vs reference, KH 2-D 64^2 WENO3 RK2, 10 steps", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rf <= 1e-9, n
+            if fm == 0.0:
+                assert dm == 0.0, n
+
+
+def test_kh3d_weno5_steps_match_reference(golden):
+    z = golden("kh3d_16_weno5_rk2")
+    o = orc.Oracle(orc.Case(**KH3D, weno_order=5, periodic=(1, 0, 1)))
+    c, p, a, dts, st = o.run(z["prims0"], 3)
+    for got, ref, names in ((c, z["cons3"], parity.SRMHD_CONS), (p, z["prims3"], parity.SRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 3), ref, names)
+        print(parity.report("oracle vs reference, KH 3-D 16^3 WENO5 RK2, 3 steps", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-9, n
+
+
+def test_quirk_flag_changes_only_phi_z_flux():
+    case = orc.Case(**KH3D, weno_order=5)
+    o1 = orc.Oracle(case)
+    case0 = orc.Case(**KH3D, weno_order=5, quirks=0)
+    o0 = orc.Oracle(case0)
+    rng = np.random.default_rng(0)
+    c, p, a = case.zeros()
+    p[0] = 1.0
+    p[4] = 1.0
+    p[5:8] = rng.uniform(-0.1, 0.1, size=p[5:8].shape)
+    c1, p1, a1 = o1.prims_to_all(p)
+    f1 = o1.flux_vector(c1, p1, a1, 2)
+    f0 = o0.flux_vector(c1, p1, a1, 2)
+    assert np.array_equal(f1[:8], f0[:8])
+    assert np.array_equal(f1[8], p1[6]) and np.array_equal(f0[8], p1[7])
