@@ -81,6 +81,15 @@ def test_flux_vector_z_phi_quirk(quirks):
     assert np.allclose(f[8], p[6] if quirks else p[7], rtol=0, atol=0)
 
 
+# FVS::F is compared at 1e-9 of the field scale, not at rounding level: the
+# reference forms the WENO smoothness indicators as expanded monomials
+# (wenoUpwinds.cc:48-67), whose cancellation leaves ~1e-15 absolute noise in
+# beta; where beta^2 is near eps = 1e-16 (variations ~1e-4, as in these
+# states) that moves its weights by ~1e-7 relative, i.e. face fluxes by ~1e-11.
+# The engine uses the factored Jiang-Shu form, accurate to rounding.
+F_TOL = 1e-9
+
+
 def test_fnet_matches_reference_2d_weno3(golden):
     z = golden("kh2d_64_weno3_rk2")
     say("gpu: fnet KH 2-D WENO3")
@@ -89,7 +98,7 @@ def test_fnet_matches_reference_2d_weno3(golden):
         f = e.fnet()
     rows = parity.field_errors(parity.interior(f, 4, 2), z["fnet0"], parity.SRMHD_CONS)
     say(parity.report("gpu vs reference FVS::F, KH 2-D WENO3", rows))
-    check_rows(rows, 1e-12, use_group=True)
+    check_rows(rows, F_TOL, use_group=True)
     # ghost cells of fnet stay zero, as FVS::F writes the interior only
     assert np.all(f[:, :4] == 0) and np.all(f[:, -4:] == 0)
 
@@ -102,7 +111,7 @@ def test_fnet_matches_reference_3d_weno5(golden):
         f = e.fnet()
     rows = parity.field_errors(parity.interior(f, 4, 3), z["fnet0"], parity.SRMHD_CONS)
     say(parity.report("gpu vs reference FVS::F, KH 3-D WENO5", rows))
-    check_rows(rows, 1e-12, use_group=True)
+    check_rows(rows, F_TOL, use_group=True)
 
 
 def test_c2p_vectors_round_trip_and_reference(golden):
