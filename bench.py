@@ -35,7 +35,25 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=96, help="cells per side of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy kernel of known size: 2^27 doubles read and written per launch.
+    Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those counters."""
+    import torch
+
+    n = 1 << 27
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": 8 * n, "bytes_written_per_launch": 8 * n,
+                      "launches": 3}))
 
 
 def cpu_baseline(n: int, steps: int):
@@ -62,6 +80,9 @@ def cpu_baseline(n: int, steps: int):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU halo exchange "
                                    "is a design note only)"}))

@@ -107,6 +107,10 @@ int mgpu_fnet(mgpu_engine *e, double *fnet);
  * state in direction dir, written for every padded cell. */
 int mgpu_flux_vector(mgpu_engine *e, int dir, double *f);
 
+/* Model::sourceTerm (srmhd.cc:233-250, srrmhd.cc:209-254) of the resident
+ * state, written for every padded cell. */
+int mgpu_source(mgpu_engine *e, double *source);
+
 /* Model::getPrimitiveVars (srmhd.cc:390-562, srrmhd.cc:314-356) on the
  * resident cons with the resident prims as the starting guess. */
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);
@@ -118,7 +122,8 @@ int mgpu_sync(mgpu_engine *e);
  * elapsed time of the whole sequence in milliseconds, measured with HIP
  * events recorded on the engine's stream; kernel_ms (may be NULL, length
  * kernel_slots) receives the average duration per launch of each kernel slot
- * of one step, measured the same way on a separate pass, and the slot count
+ * of one step, from events recorded between the launches of the same pass,
+ * and the slot count
  * is returned in *nslots. */
 int mgpu_timed_steps(mgpu_engine *e, int nsteps, const double *dts, double *total_ms,
                      double *kernel_ms, int kernel_slots, int *nslots);

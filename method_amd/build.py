@@ -72,10 +72,29 @@ def build_host() -> str | None:
         return None
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = HOST_LIB + ".tmp"
-    _run(["g++", "-std=c++14", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-I", HOST, "-o", tmp] + srcs
+    _run(["g++", "-std=c++14", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-I", HOST, "-I", os.path.join(HOST, "include"), "-o", tmp] + srcs
          + ["-L", LIBDIR, "-lmethod_gpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HOST_LIB)
     return HOST_LIB
+
+
+EXAMPLES = os.path.join(HOST, "examples")
+
+
+def build_examples() -> list[str]:
+    """Host-API drivers (written like the reference's example mains)."""
+    out = []
+    if not os.path.isdir(EXAMPLES):
+        return out
+    for f in sorted(os.listdir(EXAMPLES)):
+        if not f.endswith(".cc"):
+            continue
+        exe = os.path.join(LIBDIR, f[:-3])
+        _run(["g++", "-std=c++14", "-O2", "-Wall", "-I", INCLUDE, "-I", os.path.join(HOST, "include"),
+              os.path.join(EXAMPLES, f), "-o", exe, "-L", LIBDIR, "-lmethod_host", "-lmethod_gpu",
+              "-Wl,-rpath,$ORIGIN"])
+        out.append(exe)
+    return out
 
 
 def build_all(verbose: bool = True) -> list[str]:
@@ -83,6 +102,7 @@ def build_all(verbose: bool = True) -> list[str]:
     h = build_host()
     if h:
         out.append(h)
+        out += build_examples()
     if verbose:
         for p in out:
             print("built", p)

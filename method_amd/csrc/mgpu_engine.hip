@@ -2,6 +2,7 @@
 // integrator sequencing, host <-> device layout conversion and timing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -92,19 +93,28 @@ int ensure_staging(mgpu_engine *e) {
 }
 
 int zero_stats(mgpu_engine *e) {
-  HIP_OK(hipMemsetAsync(e->dstats, 0, sizeof(C2PCounters), e->stream));
+  HIP_OK(hipMemsetAsync(e->dstats, 0, sizeof(C2PCounters) * kStatBuckets, e->stream));
   return 0;
 }
 
 int read_stats(mgpu_engine *e, mgpu_stats *st) {
   if (!st) return 0;
-  C2PCounters h;
-  HIP_OK(hipMemcpyAsync(&h, e->dstats, sizeof h, hipMemcpyDeviceToHost, e->stream));
+  C2PCounters h[kStatBuckets];
+  HIP_OK(hipMemcpyAsync(h, e->dstats, sizeof h, hipMemcpyDeviceToHost, e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
-  st->cells = (long long)h.cells;
-  st->iterations = (long long)h.iterations;
-  st->failures = (long long)h.failures;
-  st->max_iterations = (long long)h.max_iterations;
+  long long cells = 0, iters = 0, fails = 0, maxit = 0, retry_fails = 0;
+  for (int b = 0; b < kStatBuckets; ++b) {
+    cells += (long long)h[b].cells;
+    iters += (long long)h[b].iterations;
+    fails += (long long)h[b].failures;
+    retry_fails += (long long)h[b].retry_failures;
+    maxit = std::max(maxit, (long long)h[b].max_iterations);
+  }
+  st->cells = cells;
+  st->iterations = iters;
+  st->failures = fails;
+  st->max_iterations = maxit;
+  (void)retry_fails;
   return 0;
 }
 
@@ -271,11 +281,11 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
       return cleanup(fail("hipMalloc of %zu bytes failed", b.elems * sizeof(double)));
     if (hipMemset(*b.ptr, 0, b.elems * sizeof(double)) != hipSuccess) return cleanup(fail("hipMemset failed"));
   }
-  if (hipMalloc(&e->dstats, sizeof(C2PCounters)) != hipSuccess || hipMalloc(&e->fail_count, sizeof(int)) != hipSuccess ||
+  if (hipMalloc(&e->dstats, sizeof(C2PCounters) * kStatBuckets) != hipSuccess || hipMalloc(&e->fail_count, sizeof(int)) != hipSuccess ||
       hipMalloc(&e->fail_list, n * sizeof(long long)) != hipSuccess || hipMalloc(&e->fail_flag, n) != hipSuccess)
     return cleanup(fail("hipMalloc of C2P work arrays failed"));
   if (hipMemset(e->fail_count, 0, sizeof(int)) != hipSuccess || hipMemset(e->fail_flag, 0, n) != hipSuccess ||
-      hipMemset(e->dstats, 0, sizeof(C2PCounters)) != hipSuccess)
+      hipMemset(e->dstats, 0, sizeof(C2PCounters) * kStatBuckets) != hipSuccess)
     return cleanup(fail("hipMemset failed"));
   *out = e;
   return 0;
@@ -396,6 +406,17 @@ int mgpu_flux_vector(mgpu_engine *e, int dir, double *f) {
   launch_flux_vector_srmhd(e->geo, e->pad, e->U, e->V, e->staging, dir, e->cfg.quirks, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(f, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_source(mgpu_engine *e, double *source) {
+  if (!e || !source) return fail("null argument");
+  if (ensure_staging(e)) return -1;
+  launch_source_srmhd(e->geo, e->pad, e->U, e->staging, e->cfg.cp, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(source, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost,
+                        e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
   return 0;
 }

@@ -26,20 +26,39 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
   return x * q + (x < r ? x : r) + local;
 }
 
-// Block-level reduction of C2P counters into the device stats.
+// Block-level reduction of C2P counters into one of the stats buckets.
 __device__ __forceinline__ void reduce_stats(C2PCounters *st, int cells, int iters, int fails, int maxit) {
   if (!st) return;
+  __shared__ int s_red[kThreads / 64][4];
   for (int off = 32; off > 0; off >>= 1) {
     cells += __shfl_xor(cells, off);
     iters += __shfl_xor(iters, off);
     fails += __shfl_xor(fails, off);
     maxit = max(maxit, __shfl_xor(maxit, off));
   }
-  if ((threadIdx.x & 63) == 0 && cells > 0) {
-    atomicAdd(&st->cells, (unsigned long long)cells);
-    atomicAdd(&st->iterations, (unsigned long long)iters);
-    atomicAdd(&st->failures, (unsigned long long)fails);
-    atomicMax(&st->max_iterations, (unsigned long long)maxit);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[w][0] = cells;
+    s_red[w][1] = iters;
+    s_red[w][2] = fails;
+    s_red[w][3] = maxit;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0, it = 0, f = 0, m = 0;
+    for (int k = 0; k < kThreads / 64; ++k) {
+      c += s_red[k][0];
+      it += s_red[k][1];
+      f += s_red[k][2];
+      m = max(m, s_red[k][3]);
+    }
+    if (c > 0) {
+      C2PCounters *b = st + (blockIdx.x & (kStatBuckets - 1));
+      atomicAdd(&b->cells, (unsigned long long)c);
+      atomicAdd(&b->iterations, (unsigned long long)it);
+      if (f) atomicAdd(&b->failures, (unsigned long long)f);
+      atomicMax(&b->max_iterations, (unsigned long long)m);
+    }
   }
 }
 
@@ -375,6 +394,22 @@ __global__ __launch_bounds__(kThreads) void k_flux_vector_srmhd(Geo g, PadGeo pg
   for (int v = 0; v < 9; ++v) dst[v * pg.ntot + p] = f[v];
 }
 
+// Model::sourceTerm of the resident state for every padded cell: only phi
+// has a source, -phi / cp^2 (srmhd.cc:233-250).
+__global__ __launch_bounds__(kThreads) void k_source_srmhd(Geo g, PadGeo pg, const double *U, double *dst,
+                                                            double cp) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const int I = (int)(p / ((long long)pg.N[1] * pg.N[2]));
+  const int J = (int)((p / pg.N[2]) % pg.N[1]);
+  const int K = (int)(p % pg.N[2]);
+  const int c[3] = {I - pg.off[0], J - pg.off[1], K - pg.off[2]};
+  long long q = 0;
+  for (int ax = 0; ax < 3; ++ax) q += (long long)remap(c[ax], g.n[ax], g.periodic[ax]) * g.stride[ax];
+  for (int v = 0; v < 8; ++v) dst[v * pg.ntot + p] = 0.0;
+  dst[8 * pg.ntot + p] = -U[8 * g.vstride + q] / (cp * cp);
+}
+
 template <int ORDER, int DAX, int CAX, bool DFAST, int MODE>
 void launch_sweep_t(const SweepArgs &a, int quirk, hipStream_t s) {
   using S = SweepShape<ORDER, DAX, CAX, DFAST>;
@@ -447,6 +482,10 @@ void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src,
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s) {
   hipLaunchKernelGGL(k_expand, dim3(blocks_for(pg.ntot)), dim3(kThreads), 0, s, g, pg, nvar, src, dst, ghosts_zero);
+}
+
+void launch_source_srmhd(const Geo &g, const PadGeo &pg, const double *U, double *dst, double cp, hipStream_t s) {
+  hipLaunchKernelGGL(k_source_srmhd, dim3(blocks_for(pg.ntot)), dim3(kThreads), 0, s, g, pg, U, dst, cp);
 }
 
 void launch_flux_vector_srmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst, int dir,

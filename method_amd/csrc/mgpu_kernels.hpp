@@ -15,9 +15,12 @@ struct PadGeo {
   long long ntot;
 };
 
+// Cons2prim counters, spread over kStatBuckets copies so blocks of one launch
+// rarely hit the same address; the host sums the buckets.
 struct C2PCounters {
   unsigned long long cells, iterations, failures, max_iterations, retry_failures;
 };
+constexpr int kStatBuckets = 128;
 
 enum SweepMode { kSweepWrite = 0, kSweepAdd = 1, kSweepStage1 = 2, kSweepStage2 = 3, kSweepOnly = 4 };
 
@@ -48,6 +51,7 @@ void launch_zero_int(int *p, hipStream_t s);
 void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s);
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s);
+void launch_source_srmhd(const Geo &g, const PadGeo &pg, const double *U, double *dst, double cp, hipStream_t s);
 void launch_flux_vector_srmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst, int dir,
                               int quirk, hipStream_t s);
 

@@ -26,7 +26,7 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_c2p", "mgpu_sync", "mgpu_timed_steps", "mgpu_kernel_slot_name",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_sync", "mgpu_timed_steps", "mgpu_kernel_slot_name",
 ]
 
 
@@ -80,6 +80,7 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_set_time.argtypes = [E, ctypes.c_double, ctypes.c_int]
     L.mgpu_fnet.argtypes = [E, dp]
     L.mgpu_flux_vector.argtypes = [E, ctypes.c_int, dp]
+    L.mgpu_source.argtypes = [E, dp]
     L.mgpu_c2p.argtypes = [E, sp]
     L.mgpu_sync.argtypes = [E]
     L.mgpu_timed_steps.argtypes = [E, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
@@ -225,6 +226,11 @@ class Engine:
     def flux_vector(self, dir: int) -> np.ndarray:
         out = self.zeros(self.ncons)
         self._check(self.L.mgpu_flux_vector(self._h, dir, _dp(out)), "mgpu_flux_vector")
+        return out
+
+    def source(self) -> np.ndarray:
+        out = self.zeros(self.ncons)
+        self._check(self.L.mgpu_source(self._h, _dp(out)), "mgpu_source")
         return out
 
     def c2p(self):

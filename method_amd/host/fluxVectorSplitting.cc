@@ -1,0 +1,33 @@
+// FVS and WENO objects of the host API (fluxVectorSplitting.cc:58-131, weno.cc:7-20).
+#include "fluxVectorSplitting.h"
+
+#include <cstdio>
+#include <stdexcept>
+
+#include "gpu_link.h"
+
+WenoBase::WenoBase(Data *data, int order) : data(data), order(order) {
+  shift = (order + 1) / 2;
+  checkSufficientGhostZones();
+  data->gpu.wenoOrder = order;
+}
+
+void WenoBase::checkSufficientGhostZones() {
+  if (data->Ng < shift + 1) {
+    printf("This order Weno reconstruction requires at least %d ghost zones, you have %d...", shift + 1, data->Ng);
+    throw std::invalid_argument("You must increase number of boundary cells.");
+  }
+}
+
+void FVS::fluxReconstruction(double *, double *, double *, double *, double *, int, int) {
+  throw std::logic_error("FVS::fluxReconstruction: the engine reconstructs inside its flux sweeps; call FVS::F");
+}
+
+void FVS::F(double *cons, double *prims, double *aux, double *f, double *fnet) {
+  (void)aux;
+  (void)f;
+  mgpu_engine *e = method_host::engine(data);
+  method_host::check(mgpu_upload_state(e, cons, prims, nullptr), "FVS::F");
+  method_host::check(mgpu_fnet(e, fnet), "FVS::F");
+  data->gpu.deviceCurrent = false;
+}

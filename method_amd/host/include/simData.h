@@ -9,9 +9,8 @@
 #include <string>
 #include <vector>
 
+#include "method_gpu.h"
 #include "platformEnv.h"
-
-struct mgpu_engine;
 
 #define ID(variable, idx, jdx, kdx) \
   ((variable) * (d->Nx) * (d->Ny) * (d->Nz) + (idx) * (d->Ny) * (d->Nz) + (jdx) * (d->Nz) + (kdx))
@@ -24,6 +23,7 @@ struct GpuLink {
   int quirks = 1;       // reproduce the reference's flux-index quirks
   int device = 0;
   mgpu_engine *engine = nullptr;
+  mgpu_config created;         // configuration the engine was created with
   bool deviceCurrent = false;  // device state equals d->cons/prims/aux
   bool hostStale = false;      // device ahead of the host arrays
   bool mirrorEachStep = true;  // copy the state back to the host after each step

@@ -1,0 +1,47 @@
+// Time integrators of the host API (timeInt.h:117-132, RK2.cc:31-100).
+#include <cstdio>
+#include <stdexcept>
+
+#include "RK2.h"
+#include "gpu_link.h"
+
+using method_host::check;
+using method_host::engine;
+
+void TimeIntegrator::finalise(double *cons, double *prims, double *aux) {
+  try {
+    model->getPrimitiveVars(cons, prims, aux);
+  } catch (const std::exception &e) {
+    printf("Time integration raises exception with following message:\n%s\n", e.what());
+    throw;
+  }
+  model->finalise(cons, prims, aux);
+  bcs->apply(cons, prims, aux);
+}
+
+RK2::RK2(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : TimeIntegrator(data, model, bcs, fluxMethod, ext) {
+  data->gpu.integrator = MGPU_RK2;
+}
+
+// Both stages run inside the engine on the device-resident state. The host
+// arrays are uploaded first unless the device already holds them, and copied
+// back afterwards unless the caller (Simulation::evolve) defers that.
+void RK2::step(double *cons, double *prims, double *aux, double dt) {
+  Data *d = data;
+  if (dt <= 0) dt = d->dt;
+  mgpu_engine *e = engine(d);
+  const bool own = cons == d->cons && prims == d->prims && aux == d->aux;
+  if (!(own && d->gpu.deviceCurrent)) check(mgpu_upload_state(e, cons, prims, aux), "RK2::step");
+  mgpu_stats st;
+  const int rc = check(mgpu_step(e, dt, &st), "RK2::step");
+  if (rc > 0) printf("Smart guessing did not work for %d cells\n", rc);
+  if (own && !d->gpu.mirrorEachStep) {
+    d->gpu.deviceCurrent = true;
+    d->gpu.hostStale = true;
+    return;
+  }
+  check(mgpu_download_state(e, cons, prims, aux), "RK2::step");
+  d->gpu.deviceCurrent = own;
+  d->gpu.hostStale = false;
+}
