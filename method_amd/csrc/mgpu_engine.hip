@@ -51,6 +51,7 @@ struct mgpu_engine {
   int ncons, nprims, naux;
   hipStream_t stream = nullptr;
   double *U = nullptr, *V = nullptr, *A = nullptr, *U1 = nullptr, *V1 = nullptr, *fnet = nullptr;
+  double *U2 = nullptr, *V2 = nullptr, *F1 = nullptr, *S1 = nullptr, *S2 = nullptr;  // SSP2 only
   double *staging = nullptr;
   size_t staging_elems = 0;
   C2PCounters *dstats = nullptr;
@@ -126,6 +127,8 @@ SweepArgs sweep_args(mgpu_engine *e, const double *U, const double *V, C2PCounte
   a.V = V;
   a.fnet = e->fnet;
   a.gamma = e->cfg.gamma;
+  a.sigma = e->cfg.sigma;
+  a.cp = e->cfg.cp;
   a.stats = st;
   a.fail_count = e->fail_count;
   a.fail_list = e->fail_list;
@@ -141,11 +144,12 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
   for (int dir = dims - 1; dir >= 0; --dir) {
     int mode = dir == 0 ? last_mode : (first ? kSweepWrite : kSweepAdd);
     if (dir == 0 && (last_mode == kSweepWrite || last_mode == kSweepAdd)) mode = first ? kSweepWrite : kSweepAdd;
-    launch_sweep_srmhd(a, order, dir, mode, quirk, e->stream);
-    static const char *names[3][5] = {
-        {"sweep_x_write", "sweep_x_add", "sweep_x_stage1_c2p", "sweep_x_stage2_c2p", "sweep_x"},
-        {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y"},
-        {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z"}};
+    launch_sweep(a, e->cfg.model, order, dir, mode, quirk, e->stream);
+    static const char *names[3][6] = {
+        {"sweep_x_write", "sweep_x_add", "sweep_x_stage1_c2p", "sweep_x_stage2_c2p", "sweep_x_imex2", "sweep_x_final"},
+        {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y_imex2", "sweep_y_final"},
+        {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z_imex2",
+         "sweep_z_final"}};
     if (mark(e, names[dir][mode])) return -1;
     first = 0;
   }
@@ -154,6 +158,7 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
 }
 
 int retry(mgpu_engine *e, const double *U, double *V, double *A, C2PCounters *st) {
+  if (e->cfg.model != MGPU_SRMHD) return 0;  // the resistive recovery has no retry (srrmhd.cc:574-579)
   const long long last = e->cfg.quirks ? e->geo.vstride - 1 : -1;
   launch_c2p_retry_srmhd(e->geo, U, V, A, e->cfg.gamma, e->fail_count, e->fail_list, e->fail_flag, last, st,
                          e->stream);
@@ -184,10 +189,54 @@ int rk2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   return 0;
 }
 
+// SSP2::step (SSP2.cc:53-173): implicit stage one, flux1 sweeps ending in the
+// stage-two solve, flux2 sweeps ending in the final combination.
+int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
+  SweepArgs a = sweep_args(e, e->U, e->V, st);
+  a.dt = dt;
+  a.Un = e->U;
+  a.Vg = e->V;
+  a.Uout = e->U1;
+  a.Vout = e->V1;
+  a.Aout = nullptr;
+  a.S2 = e->S1;  // stage one writes source1
+  launch_imex1_srrmhd(a, e->stream);
+  HIP_OK(hipGetLastError());
+  if (mark(e, "imex_stage1_c2p")) return -1;
+
+  SweepArgs b = sweep_args(e, e->U1, e->V1, st);
+  b.dt = dt;
+  b.Un = e->U;
+  b.Vg = e->V1;
+  b.Uout = e->U2;
+  b.Vout = e->V2;
+  b.Aout = nullptr;
+  b.F1 = e->F1;
+  b.S1 = e->S1;
+  b.S2 = e->S2;
+  if (run_sweeps(e, b, kSweepImex2)) return -1;
+
+  SweepArgs c = sweep_args(e, e->U2, e->V2, st);
+  c.dt = dt;
+  c.Un = e->U;
+  c.Vg = e->V2;
+  c.Uout = e->U;
+  c.Vout = e->V;
+  c.Aout = e->A;
+  c.F1 = e->F1;
+  c.S1 = e->S1;
+  c.S2 = e->S2;
+  return run_sweeps(e, c, kSweepImexFinal);
+}
+
 int do_step(mgpu_engine *e, double dt, mgpu_stats *stats) {
   C2PCounters *st = stats ? e->dstats : nullptr;
   if (stats && zero_stats(e)) return -1;
-  if (rk2_step(e, dt, st)) return -1;
+  if (e->cfg.integrator == MGPU_SSP2) {
+    if (ssp2_step(e, dt, st)) return -1;
+  } else if (rk2_step(e, dt, st)) {
+    return -1;
+  }
   if (stats) {
     if (read_stats(e, stats)) return -1;
     return stats->failures > 0 ? (int)std::min<long long>(stats->failures, 1 << 30) : 0;
@@ -218,8 +267,11 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   if (!cfg || !out) return fail("mgpu_create: null argument");
   *out = nullptr;
   const mgpu_config &c = *cfg;
-  if (c.model != MGPU_SRMHD) return fail("mgpu_create: model %d is not available in this build", c.model);
-  if (c.integrator != MGPU_RK2) return fail("mgpu_create: integrator %d is not available for SRMHD", c.integrator);
+  if (c.model != MGPU_SRMHD && c.model != MGPU_SRRMHD) return fail("mgpu_create: unknown model %d", c.model);
+  if (c.integrator != MGPU_RK2 && c.integrator != MGPU_SSP2)
+    return fail("mgpu_create: unknown integrator %d", c.integrator);
+  if (c.model == MGPU_SRMHD && c.integrator == MGPU_SSP2)
+    return fail("mgpu_create: SSP2 is provided with the SRRMHD model only");
   if (c.weno_order != 3 && c.weno_order != 5) return fail("mgpu_create: weno_order must be 3 or 5");
   const int shift = (c.weno_order + 1) / 2;
   if (c.Ng < shift + 1)  // WenoBase::checkSufficientGhostZones (weno.cc:13-20)
@@ -236,9 +288,10 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
 
   mgpu_engine *e = new mgpu_engine();
   e->cfg = c;
-  e->ncons = kSrmhdCons;
-  e->nprims = kSrmhdPrims;
-  e->naux = kSrmhdAux;
+  const bool res = c.model == MGPU_SRRMHD;
+  e->ncons = res ? kSrrmhdCons : kSrmhdCons;
+  e->nprims = res ? kSrrmhdPrims : kSrmhdPrims;
+  e->naux = res ? kSrrmhdAux : kSrmhdAux;
   Geo &g = e->geo;
   g.n[0] = c.nx;
   g.n[1] = c.ny;
@@ -275,8 +328,12 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
     double **ptr;
     size_t elems;
   } bufs[] = {{&e->U, n * e->ncons},  {&e->V, n * e->nprims}, {&e->A, n * e->naux},
-              {&e->U1, n * e->ncons}, {&e->V1, n * e->nprims}, {&e->fnet, n * e->ncons}};
-  for (auto &b : bufs) {
+              {&e->U1, n * e->ncons}, {&e->V1, n * e->nprims}, {&e->fnet, n * e->ncons},
+              {&e->U2, n * e->ncons}, {&e->V2, n * e->nprims}, {&e->F1, n * e->ncons},
+              {&e->S1, n * 5},        {&e->S2, n * 5}};
+  const int nbufs = c.integrator == MGPU_SSP2 ? 11 : 6;
+  for (int bi = 0; bi < nbufs; ++bi) {
+    auto &b = bufs[bi];
     if (hipMalloc(b.ptr, b.elems * sizeof(double)) != hipSuccess)
       return cleanup(fail("hipMalloc of %zu bytes failed", b.elems * sizeof(double)));
     if (hipMemset(*b.ptr, 0, b.elems * sizeof(double)) != hipSuccess) return cleanup(fail("hipMemset failed"));
@@ -294,7 +351,7 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
 int mgpu_destroy(mgpu_engine *e) {
   if (!e) return 0;
   if (e->stream) hipStreamSynchronize(e->stream);
-  double *bufs[] = {e->U, e->V, e->A, e->U1, e->V1, e->fnet, e->staging};
+  double *bufs[] = {e->U, e->V, e->A, e->U1, e->V1, e->fnet, e->staging, e->U2, e->V2, e->F1, e->S1, e->S2};
   for (double *b : bufs)
     if (b) hipFree(b);
   if (e->dstats) hipFree(e->dstats);
@@ -348,7 +405,10 @@ int mgpu_download_state(mgpu_engine *e, double *cons, double *prims, double *aux
 
 int mgpu_prims_to_all(mgpu_engine *e) {
   if (!e) return fail("null engine");
-  launch_prims_to_all_srmhd(e->geo, e->V, e->U, e->A, e->cfg.gamma, e->stream);
+  if (e->cfg.model == MGPU_SRMHD)
+    launch_prims_to_all_srmhd(e->geo, e->V, e->U, e->A, e->cfg.gamma, e->stream);
+  else
+    launch_prims_to_all_srrmhd(e->geo, e->V, e->U, e->A, e->cfg.gamma, e->cfg.sigma, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(e->stream));
   return 0;
@@ -403,7 +463,11 @@ int mgpu_flux_vector(mgpu_engine *e, int dir, double *f) {
   if (!e || !f) return fail("null argument");
   if (dir < 0 || dir >= e->geo.dims) return fail("mgpu_flux_vector: dir %d out of range", dir);
   if (ensure_staging(e)) return -1;
-  launch_flux_vector_srmhd(e->geo, e->pad, e->U, e->V, e->staging, dir, e->cfg.quirks, e->stream);
+  if (e->cfg.model == MGPU_SRMHD)
+    launch_flux_vector_srmhd(e->geo, e->pad, e->U, e->V, e->staging, dir, e->cfg.quirks, e->stream);
+  else
+    launch_flux_vector_srrmhd(e->geo, e->pad, e->U, e->V, e->staging, dir, e->cfg.quirks, e->cfg.gamma,
+                              e->cfg.sigma, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(f, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost, e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
@@ -413,7 +477,10 @@ int mgpu_flux_vector(mgpu_engine *e, int dir, double *f) {
 int mgpu_source(mgpu_engine *e, double *source) {
   if (!e || !source) return fail("null argument");
   if (ensure_staging(e)) return -1;
-  launch_source_srmhd(e->geo, e->pad, e->U, e->staging, e->cfg.cp, e->stream);
+  if (e->cfg.model == MGPU_SRMHD)
+    launch_source_srmhd(e->geo, e->pad, e->U, e->staging, e->cfg.cp, e->stream);
+  else
+    launch_source_srrmhd(e->geo, e->pad, e->U, e->V, e->staging, e->cfg.gamma, e->cfg.sigma, e->cfg.cp, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(source, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost,
                         e->stream));
@@ -424,8 +491,11 @@ int mgpu_source(mgpu_engine *e, double *source) {
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
   if (!e) return fail("null engine");
   if (zero_stats(e)) return -1;
-  launch_c2p_srmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->dstats, e->fail_count, e->fail_list, e->fail_flag,
-                   e->stream);
+  if (e->cfg.model == MGPU_SRMHD)
+    launch_c2p_srmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->dstats, e->fail_count, e->fail_list, e->fail_flag,
+                     e->stream);
+  else
+    launch_c2p_srrmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->cfg.sigma, e->dstats, e->stream);
   HIP_OK(hipGetLastError());
   if (retry(e, e->U, e->V, e->A, e->dstats)) return -1;
   mgpu_stats local;

@@ -62,7 +62,27 @@ __device__ __forceinline__ void reduce_stats(C2PCounters *st, int cells, int ite
   }
 }
 
-template <int ORDER, int DAX, int CAX, bool DFAST>
+// Model traits used by the sweep: conserved count, which primitives the flux
+// needs and the flux itself.
+struct SrmhdM {
+  static constexpr int NC = kSrmhdCons, NP = kSrmhdPrims, NA = kSrmhdAux;
+  static constexpr int P0 = 1, NPL = 4;  // vx, vy, vz, p
+  template <int DIR, bool Q>
+  __device__ __forceinline__ static void flux(const double *u, const double *pl, const SweepArgs &, double *f) {
+    srmhd_flux<DIR, Q>(u, pl[0], pl[1], pl[2], pl[3], f);
+  }
+};
+
+struct SrrmhdM {
+  static constexpr int NC = kSrrmhdCons, NP = kSrrmhdPrims, NA = kSrrmhdAux;
+  static constexpr int P0 = 0, NPL = 5;  // rho, vx, vy, vz, p
+  template <int DIR, bool Q>
+  __device__ __forceinline__ static void flux(const double *u, const double *pl, const SweepArgs &a, double *f) {
+    srrmhd_flux<DIR, Q>(u, pl[0], pl[1], pl[2], pl[3], pl[4], a.gamma, a.sigma, f);
+  }
+};
+
+template <class M, int ORDER, int DAX, int CAX, bool DFAST>
 struct SweepShape {
   static constexpr int R = Weno<ORDER>::R;
   static constexpr int TL = DFAST ? 32 : 16;  // cells along the sweep axis
@@ -70,18 +90,88 @@ struct SweepShape {
   static constexpr int TLH = TL + 2 * R;      // with halo
   static constexpr int EAX = 3 - DAX - CAX;
   static constexpr int kFaces = (TL + 1) * TC;
+  static constexpr int kLds = 2 * M::NC * TLH * TC;
   static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
   static_assert(kFaces <= 2 * kThreads, "at most two faces per thread");
 };
 
-// One directional sweep of the SRMHD flux divergence.
-template <int ORDER, int DAX, int CAX, bool DFAST, int MODE, bool QUIRK>
-__global__ __launch_bounds__(kThreads) void k_sweep_srmhd(SweepArgs a) {
-  using S = SweepShape<ORDER, DAX, CAX, DFAST>;
+// Counters of one cell's cons2prim, accumulated over a block.
+struct CellC2P {
+  int cells = 0, iters = 0, fails = 0, maxit = 0;
+  __device__ void add(int its) {
+    cells = 1;
+    if (its < 0) {
+      fails = 1;
+    } else {
+      iters = its;
+      maxit = its;
+    }
+  }
+};
+
+// RK2 stage end for SRMHD: update, Newton C2P, write.
+__device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, const double *un, SrmhdM,
+                                           CellC2P &cs) {
+  const long long vs = a.geo.vstride;
+  double v2, Z;
+  srmhd_plain_guess(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q], a.Vg[4 * vs + q],
+                    a.gamma, v2, Z);
+  const double BS = un[5] * un[1] + un[6] * un[2] + un[7] * un[3];
+  const double Bsq = un[5] * un[5] + un[6] * un[6] + un[7] * un[7];
+  const double Ssq = un[1] * un[1] + un[2] * un[2] + un[3] * un[3];
+  const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
+  double P[kSrmhdPrims], A[kSrmhdAux];
+  srmhd_recover(un, a.gamma, v2, Z, P, A);
+#pragma unroll
+  for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+#pragma unroll
+  for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+  if (a.Aout) {
+#pragma unroll
+    for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+  }
+  cs.add(its);
+  if (its < 0) {
+    const int slot = atomicAdd(a.fail_count, 1);
+    a.fail_list[slot] = q;
+    a.fail_flag[q] = 1;
+  }
+}
+
+// Cons2prim of a resistive stage value with the secant rule, writing cons,
+// prims and (if requested) aux; returns the recovered aux in A.
+__device__ __forceinline__ void c2p_store_srrmhd(const SweepArgs &a, long long q, const double *un, double *A,
+                                                 CellC2P &cs) {
+  const long long vs = a.geo.vstride;
+  const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                                   a.Vg[4 * vs + q], a.gamma);
+  double P[kSrrmhdPrims];
+  const int its = srrmhd_c2p_cell(un, a.gamma, a.sigma, Zg, P, A);
+#pragma unroll
+  for (int v = 0; v < kSrrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+  if (a.Aout) {
+#pragma unroll
+    for (int v = 0; v < kSrrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+  }
+  cs.add(its);
+}
+
+__device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, const double *un, SrrmhdM,
+                                           CellC2P &cs) {
+  double A[kSrrmhdAux];
+  c2p_store_srrmhd(a, q, un, A, cs);
+}
+
+// One directional sweep of the flux divergence of model M.
+template <class M, int ORDER, int DAX, int CAX, bool DFAST, int MODE, bool QUIRK>
+__global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
+  using S = SweepShape<M, ORDER, DAX, CAX, DFAST>;
   constexpr int R = S::R, TL = S::TL, TC = S::TC, TLH = S::TLH, EAX = S::EAX;
-  constexpr int NV = kSrmhdCons;
+  constexpr int NV = M::NC;
   // f+ then f- for each variable: [2*NV][TLH][TC]; reused for face fluxes.
-  __shared__ double s_f[2 * NV * TLH * TC];
+  __shared__ double s_f[S::kLds];
 
   const Geo &g = a.geo;
   const int nd = g.n[DAX], nc = g.n[CAX];
@@ -108,12 +198,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep_srmhd(SweepArgs a) {
       // keeps the periodic image inside [0, nd) for any tile overhang
       const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
       const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
-      double u[NV];
+      double u[NV], pl[M::NPL];
 #pragma unroll
       for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
-      const double vx = a.V[1 * vs + q], vy = a.V[2 * vs + q], vz = a.V[3 * vs + q], p = a.V[4 * vs + q];
+#pragma unroll
+      for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
       double f[NV];
-      srmhd_flux<DAX, QUIRK>(u, vx, vy, vz, p, f);
+      M::template flux<DAX, QUIRK>(u, pl, a, f);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         fp[v] = 0.5 * (f[v] + u[v]);
@@ -162,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_srmhd(SweepArgs a) {
   const int cc = DFAST ? tid / TL : tid % TC;
   const int gd = d0 + l, gc = c0 + cc;
   const bool live = gd < nd && gc < nc;
-  int c2p_cells = 0, c2p_iters = 0, c2p_fail = 0, c2p_max = 0;
+  CellC2P cs;
   if (live) {
     const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
     const double inv_d = g.inv_d[DAX];
@@ -176,16 +267,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep_srmhd(SweepArgs a) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
     } else {
-      if (MODE != kSweepOnly) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) fn[v] += a.fnet[v * vs + q];
-      }
+      for (int v = 0; v < NV; ++v) fn[v] += a.fnet[v * vs + q];
       if (MODE == kSweepAdd) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
-      } else {
-        // RK2 stage update (RK2.cc:58-66, 81-90) then cons2prim
-        // (TimeIntegrator::finalise, timeInt.h:117-132).
+      } else if (MODE == kSweepStage1 || MODE == kSweepStage2) {
         double un[NV];
         if (MODE == kSweepStage1) {
 #pragma unroll
@@ -194,37 +281,39 @@ __global__ __launch_bounds__(kThreads) void k_sweep_srmhd(SweepArgs a) {
 #pragma unroll
           for (int v = 0; v < NV; ++v) un[v] = 0.5 * (a.Un[v * vs + q] + a.U[v * vs + q] - a.dt * fn[v]);
         }
-        double v2, Z;
-        srmhd_plain_guess(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
-                          a.Vg[4 * vs + q], a.gamma, v2, Z);
-        const double BS = un[5] * un[1] + un[6] * un[2] + un[7] * un[3];
-        const double Bsq = un[5] * un[5] + un[6] * un[6] + un[7] * un[7];
-        const double Ssq = un[1] * un[1] + un[2] * un[2] + un[3] * un[3];
-        const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
-        double P[kSrmhdPrims], A[kSrmhdAux];
-        srmhd_recover(un, a.gamma, v2, Z, P, A);
+        finish_rk2(a, q, un, M(), cs);
+      } else if (MODE == kSweepImex2) {
+        // flux1 complete: keep it, then solve stage two from
+        // c = cons - dt (flux1 - om2gam source1) (SSP2.cc:112-148)
+        double c[kSrrmhdCons], x[kSrrmhdCons], A[kSrrmhdAux];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.Uout[v * vs + q] = un[v];
-#pragma unroll
-        for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
-        if (a.Aout) {
-#pragma unroll
-          for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+        for (int v = 0; v < kSrrmhdCons; ++v) {
+          a.F1[v * vs + q] = fn[v];
+          c[v] = a.Un[v * vs + q] - a.dt * fn[v];
         }
-        c2p_cells = 1;
-        if (its < 0) {
-          c2p_fail = 1;
-          const int slot = atomicAdd(a.fail_count, 1);
-          a.fail_list[slot] = q;
-          a.fail_flag[q] = 1;
-        } else {
-          c2p_iters = its;
-          c2p_max = its;
-        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) c[8 + m] += a.dt * kSsp2Om2Gam * a.S1[m * vs + q];
+        const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                                         a.Vg[4 * vs + q], a.gamma);
+        const int its = imex_stage(c, a.dt, a.gamma, a.sigma, a.cp, Zg, x);
+        c2p_store_srrmhd(a, q, x, A, cs);
+        if (its < 0) cs.fails = 1;
+        double s5[5];
+        srrmhd_source5(x, A, a.cp, s5);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a.S2[m * vs + q] = s5[m];
+      } else if (MODE == kSweepImexFinal) {
+        // cons += -dt/2 (flux1 + flux2 - source1 - source2) (SSP2.cc:156-166)
+        double un[kSrrmhdCons], A[kSrrmhdAux];
+#pragma unroll
+        for (int v = 0; v < kSrrmhdCons; ++v) un[v] = a.Un[v * vs + q] - 0.5 * a.dt * (a.F1[v * vs + q] + fn[v]);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) un[8 + m] += 0.5 * a.dt * (a.S1[m * vs + q] + a.S2[m * vs + q]);
+        c2p_store_srrmhd(a, q, un, A, cs);
       }
     }
   }
-  if (MODE == kSweepStage1 || MODE == kSweepStage2) reduce_stats(a.stats, c2p_cells, c2p_iters, c2p_fail, c2p_max);
+  if (MODE >= kSweepStage1) reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
 }
 
 // SRMHD::primsToAll over interior cells (srmhd.cc:573-673).
@@ -410,36 +499,171 @@ __global__ __launch_bounds__(kThreads) void k_source_srmhd(Geo g, PadGeo pg, con
   dst[8 * pg.ntot + p] = -U[8 * g.vstride + q] / (cp * cp);
 }
 
-template <int ORDER, int DAX, int CAX, bool DFAST, int MODE>
-void launch_sweep_t(const SweepArgs &a, int quirk, hipStream_t s) {
-  using S = SweepShape<ORDER, DAX, CAX, DFAST>;
-  const int tiles = ((a.geo.n[DAX] + S::TL - 1) / S::TL) * ((a.geo.n[CAX] + S::TC - 1) / S::TC) * a.geo.n[S::EAX];
-  if (quirk)
-    hipLaunchKernelGGL((k_sweep_srmhd<ORDER, DAX, CAX, DFAST, MODE, true>), dim3(tiles), dim3(kThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_sweep_srmhd<ORDER, DAX, CAX, DFAST, MODE, false>), dim3(tiles), dim3(kThreads), 0, s, a);
+// ------------------------------------------------------------- SRRMHD
+
+// SSP2 stage one for every interior cell (SSP2.cc:72-105): solve
+// x - cons - dt gam psi(x) = 0, then finalise (cons2prim from the cell's
+// previous rho h W^2) and keep source1's non-zero components.
+__global__ __launch_bounds__(kThreads) void k_imex1_srrmhd(SweepArgs a) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  CellC2P cs;
+  if (q < a.geo.vstride) {
+    const long long vs = a.geo.vstride;
+    double c[kSrrmhdCons], x[kSrrmhdCons], A[kSrrmhdAux];
+#pragma unroll
+    for (int v = 0; v < kSrrmhdCons; ++v) c[v] = a.Un[v * vs + q];
+    const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                                     a.Vg[4 * vs + q], a.gamma);
+    const int its = imex_stage(c, a.dt, a.gamma, a.sigma, a.cp, Zg, x);
+    c2p_store_srrmhd(a, q, x, A, cs);
+    if (its < 0) cs.fails = 1;
+    double s5[5];
+    srrmhd_source5(x, A, a.cp, s5);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.S2[m * vs + q] = s5[m];
+  }
+  reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
 }
 
-template <int ORDER, int MODE>
+// SRRMHD::primsToAll over interior cells (srrmhd.cc:358-510). The charge
+// density is the 4th-order central divergence of E; its stencil reads the
+// neighbours' boundary images.
+__global__ __launch_bounds__(kThreads) void k_prims_to_all_srrmhd(Geo g, const double *V, double *U, double *A,
+                                                                   double gam, double sigma) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  const long long vs = g.vstride;
+  const int pos[3] = {(int)(q / g.stride[0]), (int)((q / g.stride[1]) % g.n[1]), (int)(q % g.n[2])};
+  double P[kSrrmhdPrims], u[kSrrmhdCons], aux[kSrrmhdAux];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdPrims; ++v) P[v] = V[v * vs + q];
+  srrmhd_prims_to_all_local(P, gam, u, aux);
+  double div = 0.0;
+  for (int ax = 0; ax < g.dims; ++ax) {
+    long long qq[4];
+    const int offs[4] = {2, 1, -1, -2};
+    for (int m = 0; m < 4; ++m) {
+      long long r = q - (long long)pos[ax] * g.stride[ax];
+      r += (long long)remap(pos[ax] + offs[m], g.n[ax], g.periodic[ax]) * g.stride[ax];
+      qq[m] = r;
+    }
+    const double *E = V + (8 + ax) * vs;
+    const double term = (-E[qq[0]] + 8 * E[qq[1]] - 8 * E[qq[2]] + E[qq[3]]) * (g.inv_d[ax] / 12.0);
+    div = ax == 0 ? term : div + term;
+  }
+  u[13] = div;
+  double J[3];
+  srrmhd_current(u, P, aux, sigma, J);
+  aux[4] = J[0];
+  aux[5] = J[1];
+  aux[6] = J[2];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdCons; ++v) U[v * vs + q] = u[v];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdAux; ++v) A[v * vs + q] = aux[v];
+}
+
+// SRRMHD::getPrimitiveVars (srrmhd.cc:314-356): secant from the resident
+// prims' rho h W^2.
+__global__ __launch_bounds__(kThreads) void k_c2p_srrmhd(Geo g, const double *U, double *V, double *A, double gam,
+                                                          double sigma, C2PCounters *st) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  CellC2P cs;
+  if (q < g.vstride) {
+    const long long vs = g.vstride;
+    double u[kSrrmhdCons], P[kSrrmhdPrims], aux[kSrrmhdAux];
+#pragma unroll
+    for (int v = 0; v < kSrrmhdCons; ++v) u[v] = U[v * vs + q];
+    const double Zg = srrmhd_guess_Z(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam);
+    cs.add(srrmhd_c2p_cell(u, gam, sigma, Zg, P, aux));
+#pragma unroll
+    for (int v = 0; v < kSrrmhdPrims; ++v) V[v * vs + q] = P[v];
+#pragma unroll
+    for (int v = 0; v < kSrrmhdAux; ++v) A[v * vs + q] = aux[v];
+  }
+  reduce_stats(st, cs.cells, cs.iters, cs.fails, cs.maxit);
+}
+
+__device__ __forceinline__ long long padded_image(const Geo &g, const PadGeo &pg, long long p) {
+  const int I = (int)(p / ((long long)pg.N[1] * pg.N[2]));
+  const int J = (int)((p / pg.N[2]) % pg.N[1]);
+  const int K = (int)(p % pg.N[2]);
+  const int c[3] = {I - pg.off[0], J - pg.off[1], K - pg.off[2]};
+  long long q = 0;
+  for (int ax = 0; ax < 3; ++ax) q += (long long)remap(c[ax], g.n[ax], g.periodic[ax]) * g.stride[ax];
+  return q;
+}
+
+// SRRMHD::sourceTerm (srrmhd.cc:209-254) of the resident state, every padded cell.
+__global__ __launch_bounds__(kThreads) void k_source_srrmhd(Geo g, PadGeo pg, const double *U, const double *V,
+                                                             double *dst, double gam, double sigma, double cp) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const long long q = padded_image(g, pg, p), vs = g.vstride;
+  double u[kSrrmhdCons], P[kSrrmhdPrims], aux[kSrrmhdAux];
+  for (int v = 0; v < kSrrmhdCons; ++v) u[v] = U[v * vs + q];
+  for (int v = 0; v < kSrrmhdPrims; ++v) P[v] = V[v * vs + q];
+  double uu[kSrrmhdCons];
+  srrmhd_prims_to_all_local(P, gam, uu, aux);
+  double J[3];
+  srrmhd_current(u, P, aux, sigma, J);
+  aux[4] = J[0];
+  aux[5] = J[1];
+  aux[6] = J[2];
+  double s5[5];
+  srrmhd_source5(u, aux, cp, s5);
+  for (int v = 0; v < 8; ++v) dst[v * pg.ntot + p] = 0.0;
+  for (int m = 0; m < 5; ++m) dst[(8 + m) * pg.ntot + p] = s5[m];
+  dst[13 * pg.ntot + p] = 0.0;
+}
+
+template <int DIR, bool QUIRK>
+__global__ __launch_bounds__(kThreads) void k_flux_vector_srrmhd(Geo g, PadGeo pg, const double *U, const double *V,
+                                                                  double *dst, double gam, double sigma) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const long long q = padded_image(g, pg, p), vs = g.vstride;
+  double u[kSrrmhdCons], f[kSrrmhdCons];
+  for (int v = 0; v < kSrrmhdCons; ++v) u[v] = U[v * vs + q];
+  srrmhd_flux<DIR, QUIRK>(u, V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, sigma, f);
+  for (int v = 0; v < kSrrmhdCons; ++v) dst[v * pg.ntot + p] = f[v];
+}
+
+template <class M, int ORDER, int DAX, int CAX, bool DFAST, int MODE>
+void launch_sweep_t(const SweepArgs &a, int quirk, hipStream_t s) {
+  using S = SweepShape<M, ORDER, DAX, CAX, DFAST>;
+  const int tiles = ((a.geo.n[DAX] + S::TL - 1) / S::TL) * ((a.geo.n[CAX] + S::TC - 1) / S::TC) * a.geo.n[S::EAX];
+  if (quirk && DAX == 2)
+    hipLaunchKernelGGL((k_sweep<M, ORDER, DAX, CAX, DFAST, MODE, true>), dim3(tiles), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep<M, ORDER, DAX, CAX, DFAST, MODE, false>), dim3(tiles), dim3(kThreads), 0, s, a);
+}
+
+template <class M, int ORDER, int MODE>
 void launch_sweep_o(const SweepArgs &a, int dir, int quirk, hipStream_t s) {
   if (a.geo.dims == 3) {
-    if (dir == 0) launch_sweep_t<ORDER, 0, 2, false, MODE>(a, quirk, s);
-    else if (dir == 1) launch_sweep_t<ORDER, 1, 2, false, MODE>(a, quirk, s);
-    else launch_sweep_t<ORDER, 2, 1, true, MODE>(a, quirk, s);
+    if (dir == 0) launch_sweep_t<M, ORDER, 0, 2, false, MODE>(a, quirk, s);
+    else if (dir == 1) launch_sweep_t<M, ORDER, 1, 2, false, MODE>(a, quirk, s);
+    else launch_sweep_t<M, ORDER, 2, 1, true, MODE>(a, quirk, s);
   } else {
-    if (dir == 0) launch_sweep_t<ORDER, 0, 1, false, MODE>(a, quirk, s);
-    else launch_sweep_t<ORDER, 1, 0, true, MODE>(a, quirk, s);
+    if (dir == 0) launch_sweep_t<M, ORDER, 0, 1, false, MODE>(a, quirk, s);
+    else launch_sweep_t<M, ORDER, 1, 0, true, MODE>(a, quirk, s);
   }
 }
 
-template <int ORDER>
+template <class M, int ORDER>
 void launch_sweep_m(const SweepArgs &a, int dir, int mode, int quirk, hipStream_t s) {
   switch (mode) {
-    case kSweepWrite: launch_sweep_o<ORDER, kSweepWrite>(a, dir, quirk, s); break;
-    case kSweepAdd: launch_sweep_o<ORDER, kSweepAdd>(a, dir, quirk, s); break;
-    case kSweepStage1: launch_sweep_o<ORDER, kSweepStage1>(a, dir, quirk, s); break;
-    case kSweepStage2: launch_sweep_o<ORDER, kSweepStage2>(a, dir, quirk, s); break;
-    default: break;
+    case kSweepWrite: launch_sweep_o<M, ORDER, kSweepWrite>(a, dir, quirk, s); break;
+    case kSweepAdd: launch_sweep_o<M, ORDER, kSweepAdd>(a, dir, quirk, s); break;
+    case kSweepStage1: launch_sweep_o<M, ORDER, kSweepStage1>(a, dir, quirk, s); break;
+    case kSweepStage2: launch_sweep_o<M, ORDER, kSweepStage2>(a, dir, quirk, s); break;
+    default:
+      if constexpr (M::NC == kSrrmhdCons) {
+        if (mode == kSweepImex2) launch_sweep_o<M, ORDER, kSweepImex2>(a, dir, quirk, s);
+        else if (mode == kSweepImexFinal) launch_sweep_o<M, ORDER, kSweepImexFinal>(a, dir, quirk, s);
+      }
+      break;
   }
 }
 
@@ -447,11 +671,18 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) /
 
 }  // namespace
 
-void launch_sweep_srmhd(const SweepArgs &a, int order, int dir, int mode, int quirk, hipStream_t s) {
-  if (order == 3)
-    launch_sweep_m<3>(a, dir, mode, quirk, s);
-  else
-    launch_sweep_m<5>(a, dir, mode, quirk, s);
+void launch_sweep(const SweepArgs &a, int model, int order, int dir, int mode, int quirk, hipStream_t s) {
+  if (model == 0) {
+    if (order == 3)
+      launch_sweep_m<SrmhdM, 3>(a, dir, mode, quirk, s);
+    else
+      launch_sweep_m<SrmhdM, 5>(a, dir, mode, quirk, s);
+  } else {
+    if (order == 3)
+      launch_sweep_m<SrrmhdM, 3>(a, dir, mode, quirk, s);
+    else
+      launch_sweep_m<SrrmhdM, 5>(a, dir, mode, quirk, s);
+  }
 }
 
 void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double *A, double gam, hipStream_t s) {
@@ -482,6 +713,40 @@ void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src,
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s) {
   hipLaunchKernelGGL(k_expand, dim3(blocks_for(pg.ntot)), dim3(kThreads), 0, s, g, pg, nvar, src, dst, ghosts_zero);
+}
+
+void launch_imex1_srrmhd(const SweepArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_imex1_srrmhd, dim3(blocks_for(a.geo.vstride)), dim3(kThreads), 0, s, a);
+}
+
+void launch_prims_to_all_srrmhd(const Geo &g, const double *V, double *U, double *A, double gam, double sigma,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_prims_to_all_srrmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, V, U, A, gam,
+                     sigma);
+}
+
+void launch_c2p_srrmhd(const Geo &g, const double *U, double *V, double *A, double gam, double sigma,
+                       C2PCounters *st, hipStream_t s) {
+  hipLaunchKernelGGL(k_c2p_srrmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, gam, sigma, st);
+}
+
+void launch_source_srrmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst,
+                          double gam, double sigma, double cp, hipStream_t s) {
+  hipLaunchKernelGGL(k_source_srrmhd, dim3(blocks_for(pg.ntot)), dim3(kThreads), 0, s, g, pg, U, V, dst, gam, sigma,
+                     cp);
+}
+
+void launch_flux_vector_srrmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst,
+                               int dir, int quirk, double gam, double sigma, hipStream_t s) {
+  const dim3 grid(blocks_for(pg.ntot)), block(kThreads);
+  if (dir == 0)
+    hipLaunchKernelGGL((k_flux_vector_srrmhd<0, false>), grid, block, 0, s, g, pg, U, V, dst, gam, sigma);
+  else if (dir == 1)
+    hipLaunchKernelGGL((k_flux_vector_srrmhd<1, false>), grid, block, 0, s, g, pg, U, V, dst, gam, sigma);
+  else if (quirk)
+    hipLaunchKernelGGL((k_flux_vector_srrmhd<2, true>), grid, block, 0, s, g, pg, U, V, dst, gam, sigma);
+  else
+    hipLaunchKernelGGL((k_flux_vector_srrmhd<2, false>), grid, block, 0, s, g, pg, U, V, dst, gam, sigma);
 }
 
 void launch_source_srmhd(const Geo &g, const PadGeo &pg, const double *U, double *dst, double cp, hipStream_t s) {

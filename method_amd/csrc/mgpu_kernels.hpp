@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mgpu_device.hpp"
+#include "mgpu_srrmhd.hpp"
 
 namespace mgpu {
 
@@ -22,7 +23,13 @@ struct C2PCounters {
 };
 constexpr int kStatBuckets = 128;
 
-enum SweepMode { kSweepWrite = 0, kSweepAdd = 1, kSweepStage1 = 2, kSweepStage2 = 3, kSweepOnly = 4 };
+// What the last phase of a sweep does with a cell's net flux: write it, add
+// it to the accumulator, finish an RK2 stage (RK2.cc:58-66 / 81-90, with
+// cons2prim), finish SSP2 stage two (flux1 complete: store it and solve the
+// stage-two implicit equation, SSP2.cc:112-152) or finish the SSP2 step
+// (flux2 complete: final combination and cons2prim, SSP2.cc:156-170).
+enum SweepMode { kSweepWrite = 0, kSweepAdd = 1, kSweepStage1 = 2, kSweepStage2 = 3, kSweepImex2 = 4,
+                 kSweepImexFinal = 5 };
 
 struct SweepArgs {
   Geo geo;
@@ -33,14 +40,26 @@ struct SweepArgs {
   const double *Un;  // stage 2: cons at t^n
   const double *Vg;  // prims giving the cons2prim starting guess
   double *Uout, *Vout, *Aout;
-  double dt, gamma;
+  double dt, gamma, sigma, cp;
+  double *F1;        // SSP2: flux1 store
+  const double *S1;  // SSP2: source1 (components 8..12)
+  double *S2;        // SSP2: source2 (components 8..12)
   C2PCounters *stats;
   int *fail_count;
   long long *fail_list;
   unsigned char *fail_flag;
 };
 
-void launch_sweep_srmhd(const SweepArgs &a, int order, int dir, int mode, int quirk, hipStream_t s);
+void launch_sweep(const SweepArgs &a, int model, int order, int dir, int mode, int quirk, hipStream_t s);
+void launch_imex1_srrmhd(const SweepArgs &a, hipStream_t s);
+void launch_prims_to_all_srrmhd(const Geo &g, const double *V, double *U, double *A, double gam, double sigma,
+                                hipStream_t s);
+void launch_c2p_srrmhd(const Geo &g, const double *U, double *V, double *A, double gam, double sigma,
+                       C2PCounters *st, hipStream_t s);
+void launch_source_srrmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst,
+                          double gam, double sigma, double cp, hipStream_t s);
+void launch_flux_vector_srrmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst,
+                               int dir, int quirk, double gam, double sigma, hipStream_t s);
 void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double *A, double gam, hipStream_t s);
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
                       int *fail_count, long long *fail_list, unsigned char *fail_flag, hipStream_t s);

@@ -885,14 +885,174 @@ long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, d
   return fails;
 }
 
-/* SSP2 IMEX step: implemented with the SRRMHD model below. */
-long oc_ssp2_step_impl(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
+/* ----------------------------------------------------------- SSP2 IMEX */
 
-long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
-  return oc_ssp2_step_impl(c, cons, prims, aux, dt, st);
+/* IMEX SSP2(222) constants as the reference's SSP2 object ends up holding them
+ * (IMEX2Args.h:24-26: the default constructor's values survive the assignment
+ * in SSP2::SSP2, SSP2.cc:16, because operator= does not copy them). */
+#define SSP2_GAM 0.2928932188134525
+#define SSP2_OM2GAM 0.4142135623730950
+
+/* Stage equation of IMEX2Residual1/2 (SSP2.cc:194-264) reduced to its stiff
+ * part. Only E, psi and phi have sources (srrmhd.cc:194-207), so every other
+ * component of the root equals the explicit part c; psi and phi are linear:
+ *   x_psi (1 + dtg/cp^2) = c_psi + dtg x_qch,   x_phi (1 + dtg/cp^2) = c_phi,
+ * and E solves E + dtg J(x) = c_E, where J (srrmhd.cc:303-311) depends on x
+ * through the primitive recovery. The recovery's own equation
+ * (srrmhd.cc:512-537) is solved jointly, in y = (Ex, Ey, Ez, Z). */
+static int imex_residual(const double *c, double dtg, double gam, double sigma, const double *y, double *F) {
+  const double Ex = y[0], Ey = y[1], Ez = y[2], Z = y[3];
+  const double Bx = c[5], By = c[6], Bz = c[7];
+  const double Sbx = c[1] - (Ey * Bz - Ez * By);
+  const double Sby = c[2] - (Ez * Bx - Ex * Bz);
+  const double Sbz = c[3] - (Ex * By - Ey * Bx);
+  const double tauBar = c[4] - 0.5 * (Bx * Bx + By * By + Bz * Bz + Ex * Ex + Ey * Ey + Ez * Ez);
+  if (!(Z > 0)) return -1;
+  const double vx = Sbx / Z, vy = Sby / Z, vz = Sbz / Z;
+  const double vsq = vx * vx + vy * vy + vz * vz;
+  if (!(vsq < 1.0)) return -1;
+  const double W = 1.0 / sqrt(1.0 - vsq);
+  const double D = c[0];
+  F[3] = (1 - (gam - 1) / (W * W * gam)) * Z + ((gam - 1) / (W * gam) - 1) * D - tauBar;
+  const double vE = vx * Ex + vy * Ey + vz * Ez;
+  const double q = c[13];
+  const double Jx = q * vx + W * sigma * (Ex + (vy * Bz - vz * By) - vE * vx);
+  const double Jy = q * vy + W * sigma * (Ey + (vz * Bx - vx * Bz) - vE * vy);
+  const double Jz = q * vz + W * sigma * (Ez + (vx * By - vy * Bx) - vE * vz);
+  F[0] = Ex - c[8] + dtg * Jx;
+  F[1] = Ey - c[9] + dtg * Jy;
+  F[2] = Ez - c[10] + dtg * Jz;
+  return 0;
 }
 
-long oc_ssp2_step_impl(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
-  (void)c; (void)cons; (void)prims; (void)aux; (void)dt; (void)st;
-  return -1; /* filled in with the implicit-stage restatement */
+/* Newton on y = (E, Z) with a forward-difference Jacobian (steps 1e-7 of
+ * max(|y_j|, 1e-3)) and Gaussian elimination with partial pivoting; steps
+ * that leave the physical region are halved. Converged when the largest step
+ * component is <= 1e-14 max(1, |y|). Returns iterations or -1. */
+static int imex_solve(const double *c, double dtg, double gam, double sigma, double *y) {
+  double F[4], Fp[4], Jm[4][5];
+  for (int it = 0; it < 60; it++) {
+    if (imex_residual(c, dtg, gam, sigma, y, F)) return -1;
+    for (int j = 0; j < 4; j++) {
+      double yj = y[j], h = 1e-7 * fmax(fabs(yj), 1e-3);
+      y[j] = yj + h;
+      if (imex_residual(c, dtg, gam, sigma, y, Fp)) {
+        h = -h;
+        y[j] = yj + h;
+        if (imex_residual(c, dtg, gam, sigma, y, Fp)) return -1;
+      }
+      y[j] = yj;
+      for (int i = 0; i < 4; i++) Jm[i][j] = (Fp[i] - F[i]) / h;
+    }
+    for (int i = 0; i < 4; i++) Jm[i][4] = -F[i];
+    for (int k = 0; k < 4; k++) {
+      int piv = k;
+      for (int i = k + 1; i < 4; i++)
+        if (fabs(Jm[i][k]) > fabs(Jm[piv][k])) piv = i;
+      if (piv != k)
+        for (int j = 0; j < 5; j++) {
+          double t = Jm[k][j];
+          Jm[k][j] = Jm[piv][j];
+          Jm[piv][j] = t;
+        }
+      if (Jm[k][k] == 0.0) return -1;
+      for (int i = k + 1; i < 4; i++) {
+        double m = Jm[i][k] / Jm[k][k];
+        for (int j = k; j < 5; j++) Jm[i][j] -= m * Jm[k][j];
+      }
+    }
+    double d[4];
+    for (int k = 3; k >= 0; k--) {
+      double sum = Jm[k][4];
+      for (int j = k + 1; j < 4; j++) sum -= Jm[k][j] * d[j];
+      d[k] = sum / Jm[k][k];
+    }
+    double ymax = 1.0, dmax = 0.0;
+    for (int j = 0; j < 4; j++) {
+      ymax = fmax(ymax, fabs(y[j]));
+      dmax = fmax(dmax, fabs(d[j]));
+    }
+    double lam = 1.0, yn[4];
+    for (int tries = 0; tries < 40; tries++) {
+      for (int j = 0; j < 4; j++) yn[j] = y[j] + lam * d[j];
+      if (imex_residual(c, dtg, gam, sigma, yn, Fp) == 0) break;
+      lam *= 0.5;
+    }
+    for (int j = 0; j < 4; j++) y[j] = yn[j];
+    if (dmax <= 1e-14 * ymax) return it + 1;
+  }
+  return -1;
+}
+
+/* One implicit stage for one cell: c holds the explicit part (cons for stage
+ * 1, cons - dt (flux1 - om2gam source1) for stage 2), Zg the starting guess
+ * of Z; x receives the stage value. */
+static int imex_stage_cell(const double *c, double dt, double gam, double sigma, double cp, double Zg, double *x) {
+  const double dtg = dt * SSP2_GAM;
+  double y[4] = {c[8], c[9], c[10], Zg};
+  int its = imex_solve(c, dtg, gam, sigma, y);
+  for (int v = 0; v < 14; v++) x[v] = c[v];
+  x[8] = y[0];
+  x[9] = y[1];
+  x[10] = y[2];
+  const double k = 1.0 + dtg / (cp * cp);
+  x[11] = (c[11] + dtg * x[13]) / k;
+  x[12] = c[12] / k;
+  return its;
+}
+
+/* SSP2::step (SSP2.cc:53-173) with the SRRMHD model. */
+long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
+  geom g = mkgeom(c);
+  if (c->model != OC_SRRMHD) return -1;
+  const int nc = 14;
+  const long n = g.ntot;
+  double *U1 = (double *)calloc(n * nc, sizeof(double));
+  double *U2 = (double *)calloc(n * nc, sizeof(double));
+  double *s1 = (double *)calloc(n * nc, sizeof(double));
+  double *f1 = (double *)calloc(n * nc, sizeof(double));
+  double *s2 = (double *)calloc(n * nc, sizeof(double));
+  double *f2 = (double *)calloc(n * nc, sizeof(double));
+  long fails = 0;
+  double cc[14], x[14];
+  /* stage one: x - cons - dt gam psi(x) = 0, guess from the cell's aux Z */
+  for (int i = g.is; i < g.ie; i++)
+    for (int j = g.js; j < g.je; j++)
+      for (int k = g.ks; k < g.ke; k++) {
+        for (int v = 0; v < nc; v++) cc[v] = cons[IX(v, i, j, k)];
+        if (imex_stage_cell(cc, dt, c->gamma, c->sigma, c->cp, aux[IX(10, i, j, k)], x) < 0) fails++;
+        for (int v = 0; v < nc; v++) U1[IX(v, i, j, k)] = x[v];
+      }
+  fails += finalise(c, U1, prims, aux, st);
+  oc_source(c, U1, prims, aux, s1);
+  oc_fvs_F(c, U1, prims, aux, f1);
+  /* stage two: x - cons + dt (flux1 - om2gam source1 - gam psi(x)) = 0 */
+  for (int i = g.is; i < g.ie; i++)
+    for (int j = g.js; j < g.je; j++)
+      for (int k = g.ks; k < g.ke; k++) {
+        for (int v = 0; v < nc; v++) {
+          long q = IX(v, i, j, k);
+          cc[v] = cons[q] - dt * (f1[q] - SSP2_OM2GAM * s1[q]);
+        }
+        if (imex_stage_cell(cc, dt, c->gamma, c->sigma, c->cp, aux[IX(10, i, j, k)], x) < 0) fails++;
+        for (int v = 0; v < nc; v++) U2[IX(v, i, j, k)] = x[v];
+      }
+  fails += finalise(c, U2, prims, aux, st);
+  oc_source(c, U2, prims, aux, s2);
+  oc_fvs_F(c, U2, prims, aux, f2);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) {
+          long q = IX(v, i, j, k);
+          cons[q] += -0.5 * dt * (f1[q] + f2[q] - s1[q] - s2[q]);
+        }
+  fails += finalise(c, cons, prims, aux, st);
+  free(U1);
+  free(U2);
+  free(s1);
+  free(f1);
+  free(s2);
+  free(f2);
+  return fails;
 }

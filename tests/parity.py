@@ -11,9 +11,9 @@ SRRMHD_CONS = ["D", "Sx", "Sy", "Sz", "tau", "Bx", "By", "Bz", "Ex", "Ey", "Ez",
 SRRMHD_PRIMS = ["rho", "vx", "vy", "vz", "p", "Bx", "By", "Bz", "Ex", "Ey", "Ez"]
 
 # Components of one vector share that vector's scale; divergence-cleaning
-# scalars take the magnetic scale.
+# scalars take the magnetic scale and the charge density (div E) the electric one.
 _GROUPS = [("Sx", "Sy", "Sz"), ("vx", "vy", "vz"), ("Bx", "By", "Bz", "phi", "psi"), ("bx", "by", "bz"),
-           ("Ex", "Ey", "Ez")]
+           ("Ex", "Ey", "Ez", "qch")]
 
 
 def interior(a, Ng, dims):

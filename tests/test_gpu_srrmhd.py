@@ -1,0 +1,151 @@
+"""Parity of the HIP resistive SRMHD model and the SSP2 IMEX step with the
+reference (golden fixture) and the CPU oracle, through the C ABI. Needs an MI355X."""
+import sys
+
+import numpy as np
+import pytest
+
+from method_amd import engine as eng
+from method_amd import initdata as idt
+from oracle import oracle as orc
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+OT32 = dict(nx=32, ny=32, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0)
+RES = dict(sigma=1e3, cp=0.1, cfl=0.5)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def engine(integrator=eng.SSP2, weno=3, bc=(eng.PERIODIC,) * 3, quirks=1, **grid):
+    return eng.Engine(eng.Config(model=eng.SRRMHD, integrator=integrator, weno_order=weno, bc=bc, quirks=quirks,
+                                 **RES, **grid))
+
+
+def oracle_for(weno=3, periodic=(1, 1, 1), quirks=1, **grid):
+    return orc.Oracle(orc.Case(model=orc.SRRMHD, weno_order=weno, periodic=periodic, quirks=quirks, **RES, **grid))
+
+
+def check_rows(rows, tol):
+    for n, dm, fm, rf, rg, pw in rows:
+        assert rg <= tol, f"{n}: {rg:.3e} > {tol:.1e}"
+
+
+def ot3d_prims(n):
+    """Orszag-Tang in x-y with a z-dependent vz, on a 3-D periodic box."""
+    g = idt.Grid(n, n + 2, n - 2, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    p = idt.ot_prims_resistive(g)
+    _, _, z = g.centres()
+    p[3] = 0.05 * np.sin(2 * np.pi * z)[None, None, :]
+    p[7] = 0.1
+    return g, p
+
+
+def test_set_matches_oracle(golden):
+    z = golden("ot2d_32_weno3_ssp2")
+    say("gpu: SRRMHD set OT 32^2")
+    with engine(**OT32) as e:
+        e.set_prims(z["prims0"])
+        c, p, a = e.download()
+    oc, op, oa = oracle_for(**OT32).set(z["prims0"])
+    rows = parity.field_errors(parity.interior(c, 4, 2), parity.interior(oc, 4, 2), parity.SRRMHD_CONS)
+    say(parity.report("gpu vs oracle, SRRMHD primsToAll", rows))
+    check_rows(rows, 1e-13)
+    assert np.max(np.abs(a - oa)) <= 1e-11 * np.abs(oa).max()
+
+
+def test_fnet_and_source_match(golden):
+    z = golden("ot2d_32_weno3_ssp2")
+    say("gpu: SRRMHD fnet/source OT 32^2")
+    with engine(**OT32) as e:
+        e.set_prims(z["prims0"])
+        f = e.fnet()
+        s = e.source()
+    rows = parity.field_errors(parity.interior(f, 4, 2), z["fnet0"], parity.SRRMHD_CONS)
+    say(parity.report("gpu vs reference FVS::F, SRRMHD OT", rows))
+    check_rows(rows, 1e-9)
+    o = oracle_for(**OT32)
+    oc, op, oa = o.set(z["prims0"])
+    os_ = o.source(oc, op, oa)
+    # for this state E = -v x B, so J and the E source are rounding noise of
+    # size sigma |E| eps in both codes: compare on that scale
+    scale = RES["sigma"] * np.abs(oc[8:11]).max()
+    assert np.max(np.abs(parity.interior(s - os_, 4, 2))) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("quirks", [1, 0])
+def test_flux_vectors_3d_match_oracle(quirks):
+    g, p0 = ot3d_prims(12)
+    grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    say(f"gpu: SRRMHD flux vectors 3-D quirks={quirks}")
+    o = oracle_for(quirks=quirks, **grid)
+    oc, op, oa = o.set(p0)
+    with engine(quirks=quirks, **grid) as e:
+        e.set_prims(p0)
+        buf = None
+        for d in range(3):
+            f = e.flux_vector(d)
+            # the oracle's quirky z flux keeps the y flux in Sz, as the reference's shared buffer does
+            buf = o.flux_vector(oc, op, oa, d, f=buf.copy() if buf is not None else None)
+            rows = parity.field_errors(parity.interior(f, 4, 3), parity.interior(buf, 4, 3), parity.SRRMHD_CONS)
+            check_rows(rows, 1e-11)
+
+
+def test_ssp2_three_steps_match_reference(golden):
+    """OT 2-D 32^2 SRRMHD, sigma 1e3, SSP2 IMEX, WENO3: 3 updateTime calls."""
+    z = golden("ot2d_32_weno3_ssp2")
+    say("gpu: SRRMHD SSP2 OT 32^2 3 steps vs reference")
+    with engine(**OT32) as e:
+        e.set_prims(z["prims0"])
+        for s in range(3):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+            assert e.time()[1] == z["dts"][s]
+        c, p, a = e.download()
+    for got, ref, names in ((c, z["cons3"], parity.SRRMHD_CONS), (p, z["prims3"], parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        say(parity.report("gpu vs reference, OT 2-D 32^2 SRRMHD SSP2, 3 steps", rows))
+        check_rows(rows, 1e-9)
+
+
+@pytest.mark.parametrize("weno, bc, integrator", [(5, (1, 1, 1), eng.SSP2), (3, (1, 0, 1), eng.SSP2),
+                                                  (3, (1, 1, 1), eng.RK2)])
+def test_3d_matches_oracle(weno, bc, integrator):
+    g, p0 = ot3d_prims(20)
+    grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    name = "ssp2" if integrator == eng.SSP2 else "rk2"
+    say(f"gpu: SRRMHD 3-D {g.nx}x{g.ny}x{g.nz} WENO{weno} bc={bc} {name} vs oracle")
+    o = oracle_for(weno=weno, periodic=bc, **grid)
+    oc, op, oa, _, _ = o.run(p0, 2, integrator=name)
+    with engine(integrator=integrator, weno=weno, bc=tuple(bc), **grid) as e:
+        e.set_prims(p0)
+        for _ in range(2):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+        c, p, a = e.download()
+    for got, ref, names in ((c, oc, parity.SRRMHD_CONS), (p, op, parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 3), parity.interior(ref, 4, 3), names)
+        say(parity.report("gpu vs oracle SRRMHD 3-D", rows))
+        check_rows(rows, 1e-9)
+
+
+def test_ot_1024_two_steps_stay_finite_and_symmetric():
+    """The config-2 grid: 1024^2, two SSP2 steps, every cell converges, and the
+    Orszag-Tang point symmetry (x, y) -> (1-x, 1-y) is kept."""
+    g = idt.ot2d_grid(1024)
+    p0 = idt.ot_prims_resistive(g)
+    say("gpu: SRRMHD OT 1024^2 2 steps")
+    with engine(nx=1024, ny=1024, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0) as e:
+        e.set_prims(p0)
+        for _ in range(2):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+        c, p, a = e.download()
+    ci = parity.interior(c, 4, 2)[:, :, :, 0]
+    assert np.isfinite(ci).all()
+    rho = ci[0]
+    assert np.max(np.abs(rho - rho[::-1, ::-1])) <= 1e-10 * rho.max()

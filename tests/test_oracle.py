@@ -215,3 +215,21 @@ def test_quirk_flag_changes_only_phi_z_flux():
     f0 = o0.flux_vector(c1, p1, a1, 2)
     assert np.array_equal(f1[:8], f0[:8])
     assert np.array_equal(f1[8], p1[6]) and np.array_equal(f0[8], p1[7])
+
+
+def test_ot2d_ssp2_steps_match_reference(golden):
+    """SSP2 IMEX with the resistive model: the oracle solves each implicit stage
+    on its stiff part (E and the recovery's Z jointly) instead of the
+    reference's 14-unknown hybrd (SSP2.cc:85,131); the roots agree."""
+    z = golden("ot2d_32_weno3_ssp2")
+    o = orc.Oracle(orc.Case(32, 32, 0, 0.0, 1.0, 0.0, 1.0, weno_order=3, periodic=(1, 1, 1), model=orc.SRRMHD,
+                            sigma=1e3, cp=0.1, cfl=0.5))
+    c, p, a = o.set(z["prims0"])
+    assert np.array_equal(parity.interior(o.fnet(c, p, a), 4, 2), z["fnet0"])
+    c, p, a, dts, st = o.run(z["prims0"], 3, integrator="ssp2")
+    assert np.array_equal(np.array(dts), z["dts"]) and st.failures == 0
+    for got, ref, names in ((c, z["cons3"], parity.SRRMHD_CONS), (p, z["prims3"], parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        print(parity.report("oracle vs reference, OT 2-D 32^2 SRRMHD SSP2, 3 steps", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-9, n
