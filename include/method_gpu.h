@@ -52,12 +52,15 @@ typedef struct {
   int device;      /* HIP device ordinal */
 } mgpu_config;
 
+enum { MGPU_GUESS_PLAIN = 0, MGPU_GUESS_LEARNED = 1, MGPU_GUESS_COLD = 2 };
+
 /* Cons2prim work counters, summed over the call. */
 typedef struct {
   long long cells;          /* cells solved */
   long long iterations;     /* Newton (SRMHD) or secant (SRRMHD) steps */
   long long failures;       /* cells that did not converge */
   long long max_iterations; /* largest per-cell count */
+  long long fallbacks;      /* learned guess only: cells restarted from the previous primitives */
 } mgpu_stats;
 
 typedef struct mgpu_engine mgpu_engine;
@@ -114,6 +117,37 @@ int mgpu_source(mgpu_engine *e, double *source);
 /* Model::getPrimitiveVars (srmhd.cc:390-562, srrmhd.cc:314-356) on the
  * resident cons with the resident prims as the starting guess. */
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);
+
+/* Starting guess of mgpu_c2p's SRMHD Newton solve. The reference starts from
+ * the previous primitives (srmhd.cc:438-442): MGPU_GUESS_PLAIN, the default.
+ * MGPU_GUESS_LEARNED (no reference counterpart) evaluates an 8-64-64-2 GELU
+ * network inside the kernel with BF16 weights on MFMA; params are its 4,866
+ * FP32 parameters in the layout of mgpu_mlp_get_params and ymean_ystd its
+ * output scales; a cell whose solve does not converge restarts from the
+ * previous primitives (counted in mgpu_stats.fallbacks). MGPU_GUESS_COLD
+ * starts from v^2 = 0, Z = tau + D, for comparison. */
+int mgpu_set_c2p_guess(mgpu_engine *e, int mode, const float *params, const float *ymean_ystd);
+
+/* ---- training the learned guess (no reference counterpart) ----
+ * The network maps the conserved (D, Sx, Sy, Sz, tau, Bx, By, Bz) of a cell
+ * to (log W, log(p / (tau + D))). Training runs on the GPU with FP32 master
+ * weights, FP32 GEMM / GELU / MSE kernels and Adam. */
+typedef struct mgpu_mlp mgpu_mlp;
+int mgpu_mlp_create(int device, unsigned long long seed, mgpu_mlp **out);
+int mgpu_mlp_destroy(mgpu_mlp *m);
+/* n samples: cons8 row-major n x 8 (D, Sx, Sy, Sz, tau, Bx, By, Bz), w_p
+ * row-major n x 2 (W, p). Sets the output normalisation from the data. */
+int mgpu_mlp_set_data(mgpu_mlp *m, const double *cons8, const double *w_p, long long n);
+/* steps Adam steps over consecutive batches of the data; *loss_out gets the
+ * last batch's mean squared error (normalised outputs). */
+int mgpu_mlp_train(mgpu_mlp *m, int steps, int batch, float lr, float *loss_out);
+/* One forward/backward pass on given normalised inputs x (n x 8) and targets
+ * t (n x 2): the 4,866 parameter gradients of the mean squared error. */
+int mgpu_mlp_gradients(mgpu_mlp *m, const float *x, const float *t, int n, float *grads, float *loss);
+int mgpu_mlp_get_params(mgpu_mlp *m, float *params, float *ymean_ystd);
+int mgpu_mlp_set_params(mgpu_mlp *m, const float *params, const float *ymean_ystd);
+/* Inference through the BF16 MFMA kernel: normalised outputs y (n x 2). */
+int mgpu_mlp_predict(mgpu_mlp *m, const double *cons8, long long n, float *y);
 
 /* Block until the engine's queued work is done. */
 int mgpu_sync(mgpu_engine *e);

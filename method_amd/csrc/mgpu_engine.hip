@@ -44,6 +44,18 @@ struct Marks {
 
 }  // namespace
 
+namespace mgpu {
+int engine_fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+}  // namespace mgpu
+
 struct mgpu_engine {
   mgpu_config cfg;
   Geo geo;
@@ -61,7 +73,9 @@ struct mgpu_engine {
   double t = 0.0, dt = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
   int iters = 0;
   std::vector<std::string> slot_names;
-  Marks *marks = nullptr;  // non-null while a timed pass records per-kernel events
+  Marks *marks = nullptr;
+  int guess_mode = MGPU_GUESS_PLAIN;  // starting guess of mgpu_c2p
+  MlpDev *mlp = nullptr;              // learned guess network (device)  // non-null while a timed pass records per-kernel events
 };
 
 namespace {
@@ -103,19 +117,19 @@ int read_stats(mgpu_engine *e, mgpu_stats *st) {
   C2PCounters h[kStatBuckets];
   HIP_OK(hipMemcpyAsync(h, e->dstats, sizeof h, hipMemcpyDeviceToHost, e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
-  long long cells = 0, iters = 0, fails = 0, maxit = 0, retry_fails = 0;
+  long long cells = 0, iters = 0, fails = 0, maxit = 0, fallbacks = 0;
   for (int b = 0; b < kStatBuckets; ++b) {
     cells += (long long)h[b].cells;
     iters += (long long)h[b].iterations;
     fails += (long long)h[b].failures;
-    retry_fails += (long long)h[b].retry_failures;
+    fallbacks += (long long)h[b].retry_failures;
     maxit = std::max(maxit, (long long)h[b].max_iterations);
   }
   st->cells = cells;
   st->iterations = iters;
   st->failures = fails;
   st->max_iterations = maxit;
-  (void)retry_fails;
+  st->fallbacks = fallbacks;
   return 0;
 }
 
@@ -358,6 +372,7 @@ int mgpu_destroy(mgpu_engine *e) {
   if (e->fail_count) hipFree(e->fail_count);
   if (e->fail_list) hipFree(e->fail_list);
   if (e->fail_flag) hipFree(e->fail_flag);
+  if (e->mlp) hipFree(e->mlp);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
   return 0;
@@ -491,9 +506,11 @@ int mgpu_source(mgpu_engine *e, double *source) {
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
   if (!e) return fail("null engine");
   if (zero_stats(e)) return -1;
-  if (e->cfg.model == MGPU_SRMHD)
+  if (e->cfg.model == MGPU_SRMHD && e->guess_mode == MGPU_GUESS_LEARNED)
+    launch_c2p_learned_srmhd(e->geo, e->U, e->V, e->A, e->mlp, e->cfg.gamma, e->dstats, e->stream);
+  else if (e->cfg.model == MGPU_SRMHD)
     launch_c2p_srmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->dstats, e->fail_count, e->fail_list, e->fail_flag,
-                     e->stream);
+                     e->guess_mode == MGPU_GUESS_COLD, e->stream);
   else
     launch_c2p_srrmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->cfg.sigma, e->dstats, e->stream);
   HIP_OK(hipGetLastError());
@@ -502,6 +519,23 @@ int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
   if (read_stats(e, &local)) return -1;
   if (stats) *stats = local;
   return local.failures > 0 ? (int)std::min<long long>(local.failures, 1 << 30) : 0;
+}
+
+int mgpu_set_c2p_guess(mgpu_engine *e, int mode, const float *params, const float *ymean_ystd) {
+  if (!e) return fail("null engine");
+  if (mode != MGPU_GUESS_PLAIN && mode != MGPU_GUESS_LEARNED && mode != MGPU_GUESS_COLD)
+    return fail("mgpu_set_c2p_guess: unknown mode %d", mode);
+  if (mode != MGPU_GUESS_PLAIN && e->cfg.model != MGPU_SRMHD)
+    return fail("mgpu_set_c2p_guess: only the SRMHD cons2prim takes a learned or cold guess");
+  if (mode == MGPU_GUESS_LEARNED) {
+    if (!params || !ymean_ystd) return fail("mgpu_set_c2p_guess: learned mode needs parameters");
+    MlpDev pack;
+    mlp_pack(params, ymean_ystd, ymean_ystd + 2, &pack);
+    if (!e->mlp) HIP_OK(hipMalloc(&e->mlp, sizeof(MlpDev)));
+    HIP_OK(hipMemcpy(e->mlp, &pack, sizeof pack, hipMemcpyHostToDevice));
+  }
+  e->guess_mode = mode;
+  return 0;
 }
 
 int mgpu_sync(mgpu_engine *e) {

@@ -334,9 +334,10 @@ __global__ __launch_bounds__(kThreads) void k_prims_to_all_srmhd(Geo g, const do
 
 // SRMHD::getPrimitiveVars on resident cons with resident prims as the guess
 // (srmhd.cc:390-562).
+// cold: start from v^2 = 0, Z = tau + D instead of the previous primitives.
 __global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, double *V, double *A, double gam,
                                                          C2PCounters *st, int *fail_count, long long *fail_list,
-                                                         unsigned char *fail_flag) {
+                                                         unsigned char *fail_flag, int cold) {
   const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
   int cells = 0, iters = 0, fails = 0, maxit = 0;
   if (q < g.vstride) {
@@ -345,7 +346,12 @@ __global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, 
 #pragma unroll
     for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
     double v2, Z;
-    srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
+    if (cold) {
+      v2 = 0.0;
+      Z = u[4] + u[0];
+    } else {
+      srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
+    }
     const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
     const double Bsq = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
     const double Ssq = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
@@ -690,9 +696,9 @@ void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double 
 }
 
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
-                      int *fail_count, long long *fail_list, unsigned char *fail_flag, hipStream_t s) {
+                      int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold, hipStream_t s) {
   hipLaunchKernelGGL(k_c2p_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, gam, st, fail_count,
-                     fail_list, fail_flag);
+                     fail_list, fail_flag, cold);
 }
 
 void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, const int *fail_count,

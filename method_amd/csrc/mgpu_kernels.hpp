@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mgpu_device.hpp"
+#include "mgpu_mlp.hpp"
 #include "mgpu_srrmhd.hpp"
 
 namespace mgpu {
@@ -62,7 +63,7 @@ void launch_flux_vector_srrmhd(const Geo &g, const PadGeo &pg, const double *U, 
                                int dir, int quirk, double gam, double sigma, hipStream_t s);
 void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double *A, double gam, hipStream_t s);
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
-                      int *fail_count, long long *fail_list, unsigned char *fail_flag, hipStream_t s);
+                      int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold, hipStream_t s);
 void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, const int *fail_count,
                             const long long *fail_list, unsigned char *fail_flag, long long last_cell,
                             C2PCounters *st, hipStream_t s);

@@ -26,8 +26,13 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_sync", "mgpu_timed_steps", "mgpu_kernel_slot_name",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_set_c2p_guess", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
+    "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
 ]
+
+GUESS_PLAIN, GUESS_LEARNED, GUESS_COLD = 0, 1, 2
+MLP_PARAMS = 4866
 
 
 class MgpuConfig(ctypes.Structure):
@@ -43,7 +48,8 @@ class MgpuConfig(ctypes.Structure):
 
 class MgpuStats(ctypes.Structure):
     _fields_ = [("cells", ctypes.c_longlong), ("iterations", ctypes.c_longlong),
-                ("failures", ctypes.c_longlong), ("max_iterations", ctypes.c_longlong)]
+                ("failures", ctypes.c_longlong), ("max_iterations", ctypes.c_longlong),
+                ("fallbacks", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -82,12 +88,29 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_flux_vector.argtypes = [E, ctypes.c_int, dp]
     L.mgpu_source.argtypes = [E, dp]
     L.mgpu_c2p.argtypes = [E, sp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.mgpu_set_c2p_guess.argtypes = [E, ctypes.c_int, fp, fp]
     L.mgpu_sync.argtypes = [E]
+    L.mgpu_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(E)]
+    L.mgpu_mlp_destroy.argtypes = [E]
+    L.mgpu_mlp_set_data.argtypes = [E, dp, dp, ctypes.c_longlong]
+    L.mgpu_mlp_train.argtypes = [E, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
+    L.mgpu_mlp_gradients.argtypes = [E, fp, fp, ctypes.c_int, fp, fp]
+    L.mgpu_mlp_get_params.argtypes = [E, fp, fp]
+    L.mgpu_mlp_set_params.argtypes = [E, fp, fp]
+    L.mgpu_mlp_predict.argtypes = [E, dp, ctypes.c_longlong, fp]
     L.mgpu_timed_steps.argtypes = [E, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mgpu_kernel_slot_name.argtypes = [E, ctypes.c_int]
     L.mgpu_kernel_slot_name.restype = ctypes.c_char_p
     _lib = L
     return L
+
+
+def _fp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _dp(a):
@@ -238,6 +261,14 @@ class Engine:
         rc = self._check(self.L.mgpu_c2p(self._h, ctypes.byref(st)), "mgpu_c2p")
         return rc, st.as_dict()
 
+    def set_c2p_guess(self, mode: int, params=None, ymean_ystd=None):
+        """Starting guess of c2p(): GUESS_PLAIN (previous primitives, the
+        reference's), GUESS_LEARNED (network parameters from Mlp.params()) or
+        GUESS_COLD."""
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        n = None if ymean_ystd is None else np.ascontiguousarray(ymean_ystd, dtype=np.float32)
+        self._check(self.L.mgpu_set_c2p_guess(self._h, mode, _fp(p), _fp(n)), "mgpu_set_c2p_guess")
+
     # timing ----------------------------------------------------------------
     def timed_steps(self, dts, max_slots: int = 32):
         dts = np.ascontiguousarray(dts, dtype=np.float64)
@@ -252,3 +283,66 @@ class Engine:
             if name != "start":
                 slots[name] = kms[i]
         return total.value, slots
+
+
+class Mlp:
+    """The learned cons2prim guess network, trained on the GPU (C ABI mgpu_mlp_*)."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        self.L = load_library()
+        self._h = ctypes.c_void_p()
+        rc = self.L.mgpu_mlp_create(device, seed, ctypes.byref(self._h))
+        if rc < 0:
+            raise EngineError(f"mgpu_mlp_create: {self.L.mgpu_last_error().decode()}")
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise EngineError(f"{what}: {self.L.mgpu_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.L.mgpu_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_data(self, cons8: np.ndarray, w_p: np.ndarray):
+        cons8 = np.ascontiguousarray(cons8, dtype=np.float64)
+        w_p = np.ascontiguousarray(w_p, dtype=np.float64)
+        assert cons8.shape[1] == 8 and w_p.shape == (cons8.shape[0], 2)
+        self._check(self.L.mgpu_mlp_set_data(self._h, _dp(cons8), _dp(w_p), cons8.shape[0]), "mgpu_mlp_set_data")
+
+    def train(self, steps: int, batch: int, lr: float) -> float:
+        loss = ctypes.c_float()
+        self._check(self.L.mgpu_mlp_train(self._h, steps, batch, lr, ctypes.byref(loss)), "mgpu_mlp_train")
+        return loss.value
+
+    def gradients(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self.L.mgpu_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0], _fp(g), _fp(loss)),
+                    "mgpu_mlp_gradients")
+        return g, float(loss[0])
+
+    def params(self):
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        n = np.zeros(4, dtype=np.float32)
+        self._check(self.L.mgpu_mlp_get_params(self._h, _fp(p), _fp(n)), "mgpu_mlp_get_params")
+        return p, n
+
+    def set_params(self, p, n=None):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        n = None if n is None else np.ascontiguousarray(n, dtype=np.float32)
+        self._check(self.L.mgpu_mlp_set_params(self._h, _fp(p), _fp(n)), "mgpu_mlp_set_params")
+
+    def predict(self, cons8: np.ndarray) -> np.ndarray:
+        cons8 = np.ascontiguousarray(cons8, dtype=np.float64)
+        y = np.zeros((cons8.shape[0], 2), dtype=np.float32)
+        self._check(self.L.mgpu_mlp_predict(self._h, _dp(cons8), cons8.shape[0], _fp(y)), "mgpu_mlp_predict")
+        return y
