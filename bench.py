@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=96, help="cells per side of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-c2p", action="store_true", help="skip the cons2prim guess comparison")
+    ap.add_argument("--no-srrmhd", action="store_true", help="skip the SRRMHD SSP2 1024^2 leg")
+    ap.add_argument("--srrmhd-steps", type=int, default=10)
+    ap.add_argument("--mlp-steps", type=int, default=12000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -76,6 +80,71 @@ def cpu_baseline(n: int, steps: int):
     return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
             "sample": f"C oracle (serial restatement of the reference RK2 step), KH 3-D {n}^3 WENO5 RK2, "
                       f"{steps} steps, {el:.1f} s on 1 host core"}
+
+
+def c2p_comparison(e, dt, args):
+    """SRMHD cons2prim of the 256^3 state with the plain, learned and cold
+    guesses: kernel time (HIP events), Newton steps, fallbacks, and the
+    learned/cold results against the plain one."""
+    from method_amd import c2p_guess
+    from method_amd import engine as eng
+
+    print("bench: training the cons2prim guess network", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    params, scales, loss = c2p_guess.train_guess(steps=args.mlp_steps)
+    train_s = time.perf_counter() - t0
+    _, p_prev, _ = e.download()
+    _, st_step = e.step(dt, stats=True)
+    out = {"workload": "SRMHD cons2prim of every cell of the 256^3 KH state; the guess state is one step older",
+           "in_step_newton_steps_per_cell": st_step["iterations"] / max(st_step["cells"], 1),
+           "training": {"steps": args.mlp_steps, "final_loss": loss, "seconds": train_s,
+                        "network": "8-64-64-2 GELU, BF16 weights on MFMA, FP32 accumulate"}}
+    prims = {}
+    for name, mode in (("plain", eng.GUESS_PLAIN), ("learned", eng.GUESS_LEARNED), ("cold", eng.GUESS_COLD)):
+        e.upload(prims=p_prev)
+        e.set_c2p_guess(mode, params if mode == eng.GUESS_LEARNED else None,
+                        scales if mode == eng.GUESS_LEARNED else None)
+        ms, st, rc = e.timed_c2p(reps=5)
+        _, prims[name], _ = e.download()
+        cells = st["cells"]
+        out[name] = {"cells_per_s": cells / (ms * 1e-3), "kernel_ms": ms,
+                     "newton_steps_per_cell": st["iterations"] / max(cells, 1), "max_newton_steps": st["max_iterations"],
+                     "fallback_rate": st["fallbacks"] / max(cells, 1), "failures": st["failures"]}
+    e.set_c2p_guess(eng.GUESS_PLAIN)
+    ref = prims["plain"]
+    scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    for name in ("learned", "cold"):
+        d = np.abs(prims[name] - ref).reshape(ref.shape[0], -1).max(axis=1)
+        out[name]["max_rel_diff_vs_plain"] = float(np.max(d / np.maximum(scale, 1e-300)))
+    out["learned_vs_plain_speedup"] = out["learned"]["cells_per_s"] / out["plain"]["cells_per_s"]
+    out["learned_vs_cold_speedup"] = out["learned"]["cells_per_s"] / out["cold"]["cells_per_s"]
+    return out
+
+
+def srrmhd_leg(steps: int):
+    """Resistive SRMHD Orszag-Tang 1024^2, sigma 1e3, SSP2 IMEX, WENO3."""
+    from method_amd import engine as eng
+    from method_amd import initdata as idt
+
+    g = idt.ot2d_grid(1024)
+    cfg = eng.Config(nx=1024, ny=1024, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, model=eng.SRRMHD,
+                     integrator=eng.SSP2, weno_order=3, bc=(eng.PERIODIC,) * 3, sigma=1e3, cp=0.1, cfl=0.5)
+    with eng.Engine(cfg) as e:
+        e.set_prims(idt.ot_prims_resistive(g))
+        for _ in range(2):
+            e.update_time()
+        base = 0.5 * (1.0 / 1024) / np.sqrt(2.0)
+        dts = [base * 0.1] * steps
+        e.sync()
+        w0 = time.perf_counter()
+        ev_ms, slots = e.timed_steps(dts)
+        wall = time.perf_counter() - w0
+        _, st = e.step(dts[-1], stats=True)
+    cells = 1024 * 1024
+    return {"workload": "resistive SRMHD Orszag-Tang 1024^2, sigma 1e3, SSP2 IMEX, WENO3, FP64",
+            "zone_updates_per_s": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "event_ms_per_step": ev_ms / steps, "launch_ms": slots,
+            "secant_steps_per_cell_solve": st["iterations"] / max(st["cells"], 1), "failures": st["failures"]}
 
 
 def main():
@@ -120,16 +189,9 @@ def main():
     wall = time.perf_counter() - w0
     e.set_time(t + float(np.sum(dts)), iters + args.steps)
 
-    # cons2prim throughput with the plain guess: previous-step prims against
-    # the next step's cons, as inside the time step
-    _, p_prev, _ = e.download()
-    _, st_step = e.step(dts[-1], stats=True)
-    e.upload(prims=p_prev)
-    del p_prev
-    e.sync()
-    c0 = time.perf_counter()
-    _, st_c2p = e.c2p()
-    c2p_s = time.perf_counter() - c0
+    # cons2prim of the same state with three starting guesses (plain = the
+    # previous step's primitives, as the reference; learned; cold)
+    c2p = c2p_comparison(e, dts[-1], args) if not args.no_c2p else None
 
     ms_per_step = 1e3 * wall / args.steps
     value = cells * args.steps / wall
@@ -170,16 +232,12 @@ def main():
             "launch_ms": slots,
             "launch_ms_sum": step_kernel_ms,
         },
-        "c2p": {
-            "guess": "plain (previous-step primitives)",
-            "cells_per_s": st_c2p["cells"] / c2p_s,
-            "newton_steps_per_cell": st_c2p["iterations"] / max(st_c2p["cells"], 1),
-            "max_newton_steps": st_c2p["max_iterations"],
-            "failures": st_c2p["failures"],
-            "in_step_newton_steps_per_cell": st_step["iterations"] / max(st_step["cells"], 1),
-        },
+        "c2p": c2p,
     }
     e.close()
+    if not args.no_srrmhd:
+        print("bench: SRRMHD SSP2 1024^2", file=sys.stderr, flush=True)
+        out["srrmhd_ssp2"] = srrmhd_leg(args.srrmhd_steps)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)

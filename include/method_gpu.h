@@ -128,6 +128,13 @@ int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);
  * starts from v^2 = 0, Z = tau + D, for comparison. */
 int mgpu_set_c2p_guess(mgpu_engine *e, int mode, const float *params, const float *ymean_ystd);
 
+/* Times the cons2prim kernel alone: reps calls of mgpu_c2p's kernel on the
+ * resident cons, each starting from the resident prims as they were on entry
+ * (restored outside the timed window); the average kernel time from HIP
+ * events on the engine stream goes to *ms_per_call and the last call's
+ * counters to *stats. Leaves the last call's prims resident. */
+int mgpu_timed_c2p(mgpu_engine *e, int reps, double *ms_per_call, mgpu_stats *stats);
+
 /* ---- training the learned guess (no reference counterpart) ----
  * The network maps the conserved (D, Sx, Sy, Sz, tau, Bx, By, Bz) of a cell
  * to (log W, log(p / (tau + D))). Training runs on the GPU with FP32 master

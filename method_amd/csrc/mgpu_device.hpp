@@ -36,46 +36,71 @@ __device__ __forceinline__ int remap(int i, int n, int periodic) {
 
 constexpr double kWenoEps = 1e-16;
 
-// Upwind value at the right face of v1 from (v0, v1, v2); the downwind value
-// at the left face of v1 is weno3(v2, v1, v0).
-__device__ __forceinline__ double weno3(double v0, double v1, double v2) {
+// Upwind reconstruction at the right face of v1 from (v0, v1, v2), as a
+// numerator/denominator pair (value = n / d); the downwind value at the left
+// face of v1 is the same function of (v2, v1, v0).
+__device__ __forceinline__ void weno3_nd(double v0, double v1, double v2, double &n, double &d) {
   const double a0 = v1 - v0, a1 = v2 - v1;
   const double b0 = a0 * a0, b1 = a1 * a1;
   const double s0 = fma(b0, b0, kWenoEps), s1 = fma(b1, b1, kWenoEps);
   const double q0 = fma(1.5, v1, -0.5 * v0);  // (-v0 + 3 v1) / 2
   const double q1 = 0.5 * (v1 + v2);          // (v1 + v2) / 2
   const double t0 = (1.0 / 3.0) * s1, t1 = (2.0 / 3.0) * s0;
-  return fma(t0, q0, t1 * q1) / (t0 + t1);
+  n = fma(t0, q0, t1 * q1);
+  d = t0 + t1;
 }
 
-// Upwind value at the right face of v2 from (v0..v4).
-__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4) {
-  constexpr double c13 = 13.0 / 12.0;
+__device__ __forceinline__ double weno3(double v0, double v1, double v2) {
+  double n, d;
+  weno3_nd(v0, v1, v2, n, d);
+  return n / d;
+}
+
+// Upwind reconstruction at the right face of v2 from (v0..v4) as n / d. The
+// indicators are formed as 4 beta_k = 13/3 (second difference)^2 + (one-sided
+// difference)^2, with eps scaled by 16 to match, which leaves the weights
+// unchanged.
+__device__ __forceinline__ void weno5_nd(double v0, double v1, double v2, double v3, double v4, double &n,
+                                         double &d) {
+  constexpr double c13 = 13.0 / 3.0;
+  constexpr double eps16 = 16.0 * kWenoEps;
   const double a0 = v0 - 2.0 * v1 + v2, e0 = v0 - 4.0 * v1 + 3.0 * v2;
   const double a1 = v1 - 2.0 * v2 + v3, e1 = v1 - v3;
   const double a2 = v2 - 2.0 * v3 + v4, e2 = 3.0 * v2 - 4.0 * v3 + v4;
-  const double b0 = fma(c13 * a0, a0, 0.25 * e0 * e0);
-  const double b1 = fma(c13 * a1, a1, 0.25 * e1 * e1);
-  const double b2 = fma(c13 * a2, a2, 0.25 * e2 * e2);
-  const double s0 = fma(b0, b0, kWenoEps), s1 = fma(b1, b1, kWenoEps), s2 = fma(b2, b2, kWenoEps);
+  const double b0 = fma(c13 * a0, a0, e0 * e0);
+  const double b1 = fma(c13 * a1, a1, e1 * e1);
+  const double b2 = fma(c13 * a2, a2, e2 * e2);
+  const double s0 = fma(b0, b0, eps16), s1 = fma(b1, b1, eps16), s2 = fma(b2, b2, eps16);
   // candidate stencils (2 v0 - 7 v1 + 11 v2)/6, (-v1 + 5 v2 + 2 v3)/6, (2 v2 + 5 v3 - v4)/6
-  const double q0 = (1.0 / 6.0) * (2.0 * v0 - 7.0 * v1 + 11.0 * v2);
-  const double q1 = (1.0 / 6.0) * (-v1 + 5.0 * v2 + 2.0 * v3);
-  const double q2 = (1.0 / 6.0) * (2.0 * v2 + 5.0 * v3 - v4);
+  const double q0 = fma(1.0 / 3.0, v0, fma(-7.0 / 6.0, v1, (11.0 / 6.0) * v2));
+  const double q1 = fma(-1.0 / 6.0, v1, fma(5.0 / 6.0, v2, (1.0 / 3.0) * v3));
+  const double q2 = fma(1.0 / 3.0, v2, fma(5.0 / 6.0, v3, (-1.0 / 6.0) * v4));
   const double t0 = 0.1 * (s1 * s2), t1 = 0.6 * (s0 * s2), t2 = 0.3 * (s0 * s1);
-  return fma(t0, q0, fma(t1, q1, t2 * q2)) / (t0 + t1 + t2);
+  n = fma(t0, q0, fma(t1, q1, t2 * q2));
+  d = t0 + t1 + t2;
+}
+
+__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4) {
+  double n, d;
+  weno5_nd(v0, v1, v2, v3, v4, n, d);
+  return n / d;
 }
 
 template <int ORDER>
 struct Weno;
 
+// Face value = upwind reconstruction of f+ plus downwind reconstruction of
+// f-, combined over one division: (nu dd + nd du) / (du dd).
 template <>
 struct Weno<3> {
   static constexpr int R = 2;  // stencil radius = (order + 1) / 2
   // Face between cells R-1 and R of p[0 .. 2R-1] (upwind on fp, downwind on fm).
   template <class LP, class LM>
   __device__ __forceinline__ static double face(LP fp, LM fm) {
-    return weno3(fp(0), fp(1), fp(2)) + weno3(fm(3), fm(2), fm(1));
+    double nu, du, nd, dd;
+    weno3_nd(fp(0), fp(1), fp(2), nu, du);
+    weno3_nd(fm(3), fm(2), fm(1), nd, dd);
+    return fma(nu, dd, nd * du) / (du * dd);
   }
 };
 
@@ -84,7 +109,10 @@ struct Weno<5> {
   static constexpr int R = 3;
   template <class LP, class LM>
   __device__ __forceinline__ static double face(LP fp, LM fm) {
-    return weno5(fp(0), fp(1), fp(2), fp(3), fp(4)) + weno5(fm(5), fm(4), fm(3), fm(2), fm(1));
+    double nu, du, nd, dd;
+    weno5_nd(fp(0), fp(1), fp(2), fp(3), fp(4), nu, du);
+    weno5_nd(fm(5), fm(4), fm(3), fm(2), fm(1), nd, dd);
+    return fma(nu, dd, nd * du) / (du * dd);
   }
 };
 

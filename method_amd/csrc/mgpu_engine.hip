@@ -503,9 +503,8 @@ int mgpu_source(mgpu_engine *e, double *source) {
   return 0;
 }
 
-int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
-  if (!e) return fail("null engine");
-  if (zero_stats(e)) return -1;
+namespace {
+int launch_c2p(mgpu_engine *e) {
   if (e->cfg.model == MGPU_SRMHD && e->guess_mode == MGPU_GUESS_LEARNED)
     launch_c2p_learned_srmhd(e->geo, e->U, e->V, e->A, e->mlp, e->cfg.gamma, e->dstats, e->stream);
   else if (e->cfg.model == MGPU_SRMHD)
@@ -514,7 +513,46 @@ int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
   else
     launch_c2p_srrmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->cfg.sigma, e->dstats, e->stream);
   HIP_OK(hipGetLastError());
-  if (retry(e, e->U, e->V, e->A, e->dstats)) return -1;
+  return 0;
+}
+}  // namespace
+
+int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats) {
+  if (!e) return fail("null engine");
+  if (zero_stats(e)) return -1;
+  if (launch_c2p(e)) return -1;
+  if (e->guess_mode != MGPU_GUESS_LEARNED && retry(e, e->U, e->V, e->A, e->dstats)) return -1;
+  mgpu_stats local;
+  if (read_stats(e, &local)) return -1;
+  if (stats) *stats = local;
+  return local.failures > 0 ? (int)std::min<long long>(local.failures, 1 << 30) : 0;
+}
+
+int mgpu_timed_c2p(mgpu_engine *e, int reps, double *ms_per_call, mgpu_stats *stats) {
+  if (!e || reps <= 0) return fail("mgpu_timed_c2p: bad arguments");
+  if (ensure_staging(e)) return -1;
+  const size_t bytes = sizeof(double) * e->nprims * e->geo.vstride;
+  HIP_OK(hipMemcpyAsync(e->staging, e->V, bytes, hipMemcpyDeviceToDevice, e->stream));
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  double total = 0.0;
+  int rc = 0;
+  for (int r = 0; r < reps && rc == 0; ++r) {
+    HIP_OK(hipMemcpyAsync(e->V, e->staging, bytes, hipMemcpyDeviceToDevice, e->stream));
+    if (zero_stats(e)) rc = -1;
+    HIP_OK(hipEventRecord(a, e->stream));
+    if (rc == 0 && launch_c2p(e)) rc = -1;
+    HIP_OK(hipEventRecord(b, e->stream));
+    HIP_OK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
+    total += ms;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc) return rc;
+  if (ms_per_call) *ms_per_call = total / reps;
   mgpu_stats local;
   if (read_stats(e, &local)) return -1;
   if (stats) *stats = local;

@@ -91,6 +91,17 @@ struct SweepShape {
   static constexpr int EAX = 3 - DAX - CAX;
   static constexpr int kFaces = (TL + 1) * TC;
   static constexpr int kLds = 2 * M::NC * TLH * TC;
+  // LDS position of (variable slot v, cell l along the sweep, pencil c): the
+  // unit-stride index follows the threads' fast index, which is l when the
+  // sweep axis is the contiguous one (DFAST) and the pencil index otherwise.
+  __device__ static constexpr int fidx(int v, int l, int c) {
+    return DFAST ? (v * TC + c) * TLH + l : (v * TLH + l) * TC + c;
+  }
+  // Same for the face fluxes (TL + 1 faces per pencil).
+  __device__ static constexpr int gidx(int v, int lf, int c) {
+    return DFAST ? (v * TC + c) * (TL + 1) + lf : (v * (TL + 1) + lf) * TC + c;
+  }
+  static constexpr int kStrideL = DFAST ? 1 : TC;  // LDS step between neighbouring cells of a pencil
   static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
   static_assert(kFaces <= 2 * kThreads, "at most two faces per thread");
 };
@@ -216,8 +227,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      s_f[(v * TLH + l) * TC + cc] = fp[v];
-      s_f[((NV + v) * TLH + l) * TC + cc] = fm[v];
+      s_f[S::fidx(v, l, cc)] = fp[v];
+      s_f[S::fidx(NV + v, l, cc)] = fm[v];
     }
   }
   __syncthreads();
@@ -228,12 +239,14 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   for (int rep = 0; rep < 2; ++rep) {
     const int idx = tid + rep * kThreads;
     if (idx < S::kFaces) {
-      const int lf = idx / TC, cc = idx % TC;
+      const int lf = DFAST ? idx % (TL + 1) : idx / TC;
+      const int cc = DFAST ? idx / (TL + 1) : idx % TC;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const double *fpv = s_f + (v * TLH + lf) * TC + cc;
-        const double *fmv = s_f + ((NV + v) * TLH + lf) * TC + cc;
-        face[rep][v] = Weno<ORDER>::face([&](int m) { return fpv[m * TC]; }, [&](int m) { return fmv[m * TC]; });
+        const double *fpv = s_f + S::fidx(v, lf, cc);
+        const double *fmv = s_f + S::fidx(NV + v, lf, cc);
+        face[rep][v] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
+                                         [&](int m) { return fmv[m * S::kStrideL]; });
       }
     }
   }
@@ -242,6 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   for (int rep = 0; rep < 2; ++rep) {
     const int idx = tid + rep * kThreads;
     if (idx < S::kFaces) {
+      // idx enumerates (lf, cc) in gidx order for both layouts
 #pragma unroll
       for (int v = 0; v < NV; ++v) s_f[v * (TL + 1) * TC + idx] = face[rep][v];
     }
@@ -259,10 +273,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     const double inv_d = g.inv_d[DAX];
     double fn[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const double *fv = s_f + v * (TL + 1) * TC;
-      fn[v] = (fv[(l + 1) * TC + cc] - fv[l * TC + cc]) * inv_d;
-    }
+    for (int v = 0; v < NV; ++v) fn[v] = (s_f[S::gidx(v, l + 1, cc)] - s_f[S::gidx(v, l, cc)]) * inv_d;
     if (MODE == kSweepWrite) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];

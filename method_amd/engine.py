@@ -26,7 +26,7 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_set_c2p_guess", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
 ]
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
     fp = ctypes.POINTER(ctypes.c_float)
     L.mgpu_set_c2p_guess.argtypes = [E, ctypes.c_int, fp, fp]
     L.mgpu_sync.argtypes = [E]
+    L.mgpu_timed_c2p.argtypes = [E, ctypes.c_int, dp, sp]
     L.mgpu_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(E)]
     L.mgpu_mlp_destroy.argtypes = [E]
     L.mgpu_mlp_set_data.argtypes = [E, dp, dp, ctypes.c_longlong]
@@ -260,6 +261,14 @@ class Engine:
         st = MgpuStats()
         rc = self._check(self.L.mgpu_c2p(self._h, ctypes.byref(st)), "mgpu_c2p")
         return rc, st.as_dict()
+
+    def timed_c2p(self, reps: int = 5):
+        """(ms per cons2prim kernel call, counters of the last call), each call
+        starting from the prims resident on entry."""
+        ms = ctypes.c_double()
+        st = MgpuStats()
+        rc = self._check(self.L.mgpu_timed_c2p(self._h, reps, ctypes.byref(ms), ctypes.byref(st)), "mgpu_timed_c2p")
+        return ms.value, st.as_dict(), rc
 
     def set_c2p_guess(self, mode: int, params=None, ymean_ystd=None):
         """Starting guess of c2p(): GUESS_PLAIN (previous primitives, the
