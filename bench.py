@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-srrmhd", action="store_true", help="skip the SRRMHD SSP2 1024^2 leg")
     ap.add_argument("--srrmhd-steps", type=int, default=10)
     ap.add_argument("--mlp-steps", type=int, default=12000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-members", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -147,6 +150,55 @@ def srrmhd_leg(steps: int):
             "secant_steps_per_cell_solve": st["iterations"] / max(st["cells"], 1), "failures": st["failures"]}
 
 
+def batch_leg(steps: int, members: int):
+    """64 independent 2-D ideal SRMHD KH runs of uneven size, WENO5, RK2, per-member
+    dt: the batched engine (autotuned persistent stage kernels under a hipGraph)
+    against the same members launched one at a time on the single-instance engine."""
+    from method_amd import engine as eng
+    from method_amd import initdata as idt
+
+    ms_list = idt.kh_ensemble(members)
+    cells = sum(m[0] * m[1] for m in ms_list)
+    inst = [eng.Instance(m[0], m[1]) for m in ms_list]
+    with eng.Batch(inst, weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW)) as b:
+        for k, m in enumerate(ms_list):
+            b.set_prims(k, idt.kh_prims(idt.Grid(m[0], m[1], 0, -0.5, 0.5, -1.0, 1.0), seed=m[4], shear=m[2], bz=m[3]))
+        best, cands = b.autotune(trial_steps=3)
+        b.step(2)
+        b.sync()
+        w0 = time.perf_counter()
+        ev_ms = b.timed_steps(steps)
+        wall = time.perf_counter() - w0
+        _, st = b.step(1, stats=True)
+    batched = cells * steps / wall
+
+    # one at a time: each member on its own engine, its steps queued back to
+    # back and timed with events on that engine's stream
+    single_ms = 0.0
+    for m in ms_list:
+        nx, ny = m[0], m[1]
+        cfg = eng.Config(nx=nx, ny=ny, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, weno_order=5,
+                         bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), cfl=0.4)
+        with eng.Engine(cfg) as e:
+            e.set_prims(idt.kh_prims(idt.Grid(nx, ny, 0, -0.5, 0.5, -1.0, 1.0), seed=m[4], shear=m[2], bz=m[3]))
+            for _ in range(2):
+                e.update_time()
+            base = 0.4 * min(1.0 / nx, 2.0 / ny) / np.sqrt(2.0)
+            dts = [base * (0.1 if it < 15 else (0.25 if it < 30 else 1.0)) for it in range(2, 2 + steps)]
+            ms, _ = e.timed_steps(dts)
+            single_ms += ms
+    one_at_a_time = cells * steps / (single_ms * 1e-3)
+    return {"workload": f"{members} independent ideal SRMHD 2-D KH runs, nx, ny from {{128..768}}, WENO5, RK2, "
+                        "per-member dt, FP64",
+            "cells": cells, "steps": steps,
+            "zone_updates_per_s": batched, "ms_per_step": 1e3 * wall / steps, "event_ms_per_step": ev_ms / steps,
+            "hbm_frac_784B": batched * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+            "one_at_a_time_zone_updates_per_s": one_at_a_time, "one_at_a_time_ms_per_step": single_ms / steps,
+            "speedup_vs_one_at_a_time": batched / one_at_a_time,
+            "tuning": best, "candidates": len(cands),
+            "newton_steps_per_cell": st["iterations"] / max(st["cells"], 1), "failures": st["failures"]}
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -238,6 +290,9 @@ def main():
     if not args.no_srrmhd:
         print("bench: SRRMHD SSP2 1024^2", file=sys.stderr, flush=True)
         out["srrmhd_ssp2"] = srrmhd_leg(args.srrmhd_steps)
+    if not args.no_batch:
+        print("bench: batched ensemble", file=sys.stderr, flush=True)
+        out["batched_ensemble"] = batch_leg(args.batch_steps, args.batch_members)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)

@@ -118,6 +118,14 @@ int mgpu_source(mgpu_engine *e, double *source);
  * resident cons with the resident prims as the starting guess. */
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);
 
+/* CFL max-wavespeed reduction (no counterpart in the reference, whose dt and
+ * flux split take the light speed as the bound: simulation.cc:39-41,
+ * fluxVectorSplitting.cc:10). out[d] = the largest |characteristic speed|
+ * along axis d over interior cells of the resident state, 0 for axes the
+ * grid does not have. SRMHD: the fast magnetosonic bound from the resident
+ * prims and aux; SRRMHD: 1, the speed of its light modes. Synchronous. */
+int mgpu_max_wavespeed(mgpu_engine *e, double out[3]);
+
 /* Starting guess of mgpu_c2p's SRMHD Newton solve. The reference starts from
  * the previous primitives (srmhd.cc:438-442): MGPU_GUESS_PLAIN, the default.
  * MGPU_GUESS_LEARNED (no reference counterpart) evaluates an 8-64-64-2 GELU
@@ -155,6 +163,69 @@ int mgpu_mlp_get_params(mgpu_mlp *m, float *params, float *ymean_ystd);
 int mgpu_mlp_set_params(mgpu_mlp *m, const float *params, const float *ymean_ystd);
 /* Inference through the BF16 MFMA kernel: normalised outputs y (n x 2). */
 int mgpu_mlp_predict(mgpu_mlp *m, const double *cons8, long long n, float *y);
+
+/* ---- batched ensemble ----
+ * Many independent 2-D ideal SRMHD simulations (RK2, FVS with WENO3/5) of
+ * uneven grid size advanced together on one GPU. Each instance behaves as
+ * its own Simulation (simulation.cc:8-133): Simulation::set through
+ * mgpu_batch_set_prims, updateTime's dt rule (alpha = 1, slow start, clamp
+ * to its end_time) per instance, RK2::step (RK2.cc:93-100), and it stops
+ * stepping once t reaches end_time, as Simulation::evolve's loop does
+ * (simulation.cc:136-160). One persistent kernel per RK stage advances all
+ * instances; one step is captured in a hipGraph. Host arrays use the
+ * reference layout of each instance (padded, [var][i][j], Ng ghosts). */
+typedef struct mgpu_batch mgpu_batch;
+
+typedef struct {
+  int Ng;          /* ghost width of the host arrays */
+  double gamma;
+  int weno_order;  /* 3 or 5 */
+  int bc[2];       /* MGPU_BC_* for x and y, shared by every instance */
+  int quirks;      /* the reference's cons2prim retry quirk */
+  int device;
+} mgpu_batch_config;
+
+typedef struct {
+  int nx, ny;      /* interior cells */
+  double xmin, xmax, ymin, ymax;
+  double cfl;
+  double end_time; /* the instance stops stepping when t reaches it */
+} mgpu_instance;
+
+/* Tile shape (tile_x cells along x times tile_y along y), schedule (0 static,
+ * 1 dynamic work counters), workgroups per stage launch, and the measured
+ * time per step of the autotuner trial. */
+typedef struct {
+  int tile_x, tile_y, schedule, blocks;
+  double ms_per_step;
+} mgpu_batch_tuning;
+
+int mgpu_batch_create(const mgpu_batch_config *cfg, const mgpu_instance *inst, int ninst, mgpu_batch **out);
+int mgpu_batch_destroy(mgpu_batch *b);
+int mgpu_batch_sizes(const mgpu_batch *b, int *ninst, long long *cells, int *ntiles);
+/* Simulation::set for instance i (simulation.cc:80-101): prims in the padded
+ * reference layout [8][nx+2Ng][ny+2Ng]; resets the instance's t and iters. */
+int mgpu_batch_set_prims(mgpu_batch *b, int i, const double *prims);
+int mgpu_batch_upload_state(mgpu_batch *b, int i, const double *cons, const double *prims, const double *aux);
+/* Any of cons / prims / aux may be NULL. Ghost cells hold the Bcs values. */
+int mgpu_batch_download_state(mgpu_batch *b, int i, double *cons, double *prims, double *aux);
+/* nsteps updateTime calls of every instance still short of its end_time.
+ * With stats, the cons2prim counters of all instances and stages. */
+int mgpu_batch_step(mgpu_batch *b, int nsteps, mgpu_stats *stats);
+/* t, the dt of the last step taken and the iteration count of instance i. */
+int mgpu_batch_get_time(mgpu_batch *b, int i, double *t, double *dt, int *iters);
+/* 1 (default): steps replay the captured graph; 0: direct launches. */
+int mgpu_batch_use_graph(mgpu_batch *b, int on);
+int mgpu_batch_get_tuning(const mgpu_batch *b, mgpu_batch_tuning *t);
+int mgpu_batch_set_tuning(mgpu_batch *b, const mgpu_batch_tuning *t);
+/* Times every candidate tuning on trial_steps steps of this batch, keeps
+ * the fastest and restores the state and clocks as they were on entry. all
+ * (may be NULL) receives up to max_all candidates; *nall the count. */
+int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best, mgpu_batch_tuning *all,
+                        int max_all, int *nall);
+/* nsteps steps timed with HIP events on the batch stream. */
+int mgpu_batch_timed_steps(mgpu_batch *b, int nsteps, double *total_ms);
+int mgpu_batch_sync(mgpu_batch *b);
 
 /* Block until the engine's queued work is done. */
 int mgpu_sync(mgpu_engine *e);

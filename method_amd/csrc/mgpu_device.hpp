@@ -271,4 +271,30 @@ __device__ __forceinline__ void srmhd_plain_guess(double rho, double vx, double 
   Z = rho * h / (1.0 - v2);
 }
 
+// RK2 stage updates on one conserved value (RK2.cc:31-91), written with
+// explicit fused multiply-adds so that every kernel rounds them alike:
+// stage one U1 = U - dt F(U), stage two U = (U + U1 - dt F(U1)) / 2.
+__device__ __forceinline__ double rk2_stage1(double u, double f, double dt) { return fma(-dt, f, u); }
+__device__ __forceinline__ double rk2_stage2(double un, double u1, double f, double dt) {
+  return 0.5 * fma(-dt, f, un + u1);
+}
+
+// Largest |characteristic speed| along one axis of an ideal SRMHD state: the
+// fast magnetosonic speed taken as isotropic in the fluid frame,
+//   a2 = cs2 + va2 - cs2 va2,  cs2 = c^2,  va2 = b2 / (rho h + b2),
+// carried to the lab frame,
+//   lambda = [v_d (1 - a2) +- sqrt(a2 (1 - v2) (1 - v2 a2 - v_d^2 (1 - a2)))] / (1 - v2 a2)
+// (Gammie, McKinney & Toth 2003, ApJ 589, 444, sec. 3.3, in flat space). c, h,
+// b2 and v2 are the SRMHD aux entries 3, 0, 8 and 9.
+__device__ __host__ inline double srmhd_max_speed(double rho, double vd, double v2, double h, double c, double b2) {
+  const double cs2 = c * c;
+  const double va2 = b2 / (rho * h + b2);
+  const double a2 = cs2 + va2 - cs2 * va2;
+  const double den = 1.0 - v2 * a2;
+  const double disc = a2 * (1.0 - v2) * (den - vd * vd * (1.0 - a2));
+  const double sq = sqrt(disc > 0.0 ? disc : 0.0);
+  const double b = vd * (1.0 - a2);
+  return fmax(fabs(b + sq), fabs(b - sq)) / den;
+}
+
 }  // namespace mgpu

@@ -75,7 +75,8 @@ struct mgpu_engine {
   std::vector<std::string> slot_names;
   Marks *marks = nullptr;
   int guess_mode = MGPU_GUESS_PLAIN;  // starting guess of mgpu_c2p
-  MlpDev *mlp = nullptr;              // learned guess network (device)  // non-null while a timed pass records per-kernel events
+  MlpDev *mlp = nullptr;              // learned guess network (device)
+  unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
 };
 
 namespace {
@@ -369,6 +370,7 @@ int mgpu_destroy(mgpu_engine *e) {
   for (double *b : bufs)
     if (b) hipFree(b);
   if (e->dstats) hipFree(e->dstats);
+  if (e->dspeed) hipFree(e->dspeed);
   if (e->fail_count) hipFree(e->fail_count);
   if (e->fail_list) hipFree(e->fail_list);
   if (e->fail_flag) hipFree(e->fail_flag);
@@ -573,6 +575,23 @@ int mgpu_set_c2p_guess(mgpu_engine *e, int mode, const float *params, const floa
     HIP_OK(hipMemcpy(e->mlp, &pack, sizeof pack, hipMemcpyHostToDevice));
   }
   e->guess_mode = mode;
+  return 0;
+}
+
+int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
+  if (!e || !out) return fail("mgpu_max_wavespeed: null argument");
+  if (e->cfg.model != MGPU_SRMHD) {
+    for (int d = 0; d < 3; ++d) out[d] = d < e->geo.dims ? 1.0 : 0.0;
+    return 0;
+  }
+  if (!e->dspeed) HIP_OK(hipMalloc(&e->dspeed, 3 * sizeof(unsigned long long)));
+  HIP_OK(hipMemsetAsync(e->dspeed, 0, 3 * sizeof(unsigned long long), e->stream));
+  launch_max_wavespeed_srmhd(e->geo, e->V, e->A, e->dspeed, e->stream);
+  HIP_OK(hipGetLastError());
+  unsigned long long bits[3];
+  HIP_OK(hipMemcpyAsync(bits, e->dspeed, sizeof bits, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  for (int d = 0; d < 3; ++d) memcpy(&out[d], &bits[d], sizeof(double));
   return 0;
 }
 

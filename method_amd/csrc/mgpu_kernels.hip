@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace mgpu {
 
 namespace {
@@ -103,7 +105,6 @@ struct SweepShape {
   }
   static constexpr int kStrideL = DFAST ? 1 : TC;  // LDS step between neighbouring cells of a pencil
   static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
-  static_assert(kFaces <= 2 * kThreads, "at most two faces per thread");
 };
 
 // Counters of one cell's cons2prim, accumulated over a block.
@@ -234,31 +235,32 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   __syncthreads();
 
   // Phase 2: WENO face values, face lf being the left face of tile cell lf.
-  double face[2][NV];
+  // The (variable, face) pairs are dealt out one per thread per pass, so the
+  // faces beyond one per thread cost a fraction of a pass spread over the
+  // waves instead of a full pass of all variables on one wave.
+  constexpr int kItems = NV * S::kFaces;
+  constexpr int kPasses = (kItems + kThreads - 1) / kThreads;
+  double face[kPasses];
 #pragma unroll
-  for (int rep = 0; rep < 2; ++rep) {
-    const int idx = tid + rep * kThreads;
-    if (idx < S::kFaces) {
+  for (int k = 0; k < kPasses; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < kItems) {
+      const int v = item / S::kFaces, idx = item - v * S::kFaces;
       const int lf = DFAST ? idx % (TL + 1) : idx / TC;
       const int cc = DFAST ? idx / (TL + 1) : idx % TC;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const double *fpv = s_f + S::fidx(v, lf, cc);
-        const double *fmv = s_f + S::fidx(NV + v, lf, cc);
-        face[rep][v] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
-                                         [&](int m) { return fmv[m * S::kStrideL]; });
-      }
+      const double *fpv = s_f + S::fidx(v, lf, cc);
+      const double *fmv = s_f + S::fidx(NV + v, lf, cc);
+      face[k] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
+                                  [&](int m) { return fmv[m * S::kStrideL]; });
     }
   }
   __syncthreads();
+  // item = v * kFaces + idx, idx enumerating (lf, cc) in gidx order for both
+  // layouts, so the face of item goes to s_f[item]
 #pragma unroll
-  for (int rep = 0; rep < 2; ++rep) {
-    const int idx = tid + rep * kThreads;
-    if (idx < S::kFaces) {
-      // idx enumerates (lf, cc) in gidx order for both layouts
-#pragma unroll
-      for (int v = 0; v < NV; ++v) s_f[v * (TL + 1) * TC + idx] = face[rep][v];
-    }
+  for (int k = 0; k < kPasses; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < kItems) s_f[item] = face[k];
   }
   __syncthreads();
 
@@ -271,15 +273,17 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   if (live) {
     const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
     const double inv_d = g.inv_d[DAX];
+    // net flux of this axis, added to the earlier axes' as one fused
+    // multiply-add so every kernel that accumulates it rounds alike
     double fn[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) fn[v] = (s_f[S::gidx(v, l + 1, cc)] - s_f[S::gidx(v, l, cc)]) * inv_d;
     if (MODE == kSweepWrite) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
+      for (int v = 0; v < NV; ++v)
+        a.fnet[v * vs + q] = (s_f[S::gidx(v, l + 1, cc)] - s_f[S::gidx(v, l, cc)]) * inv_d;
     } else {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) fn[v] += a.fnet[v * vs + q];
+      for (int v = 0; v < NV; ++v)
+        fn[v] = fma(s_f[S::gidx(v, l + 1, cc)] - s_f[S::gidx(v, l, cc)], inv_d, a.fnet[v * vs + q]);
       if (MODE == kSweepAdd) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
@@ -287,10 +291,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
         double un[NV];
         if (MODE == kSweepStage1) {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) un[v] = a.U[v * vs + q] - a.dt * fn[v];
+          for (int v = 0; v < NV; ++v) un[v] = rk2_stage1(a.U[v * vs + q], fn[v], a.dt);
         } else {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) un[v] = 0.5 * (a.Un[v * vs + q] + a.U[v * vs + q] - a.dt * fn[v]);
+          for (int v = 0; v < NV; ++v) un[v] = rk2_stage2(a.Un[v * vs + q], a.U[v * vs + q], fn[v], a.dt);
         }
         finish_rk2(a, q, un, M(), cs);
       } else if (MODE == kSweepImex2) {
@@ -441,6 +445,35 @@ __global__ void k_clear_fail_flags(const int *fail_count, const long long *fail_
   const int nfail = *fail_count;
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nfail; f += gridDim.x * blockDim.x)
     fail_flag[fail_list[f]] = 0;
+}
+
+// Per-axis maximum of srmhd_max_speed over interior cells: grid-stride
+// per-thread maxima, a butterfly over the 64 lanes with __shfl_xor, one LDS
+// slot per wave, then one atomicMax per block on the bit pattern (for
+// non-negative doubles the unsigned order of the bits is the numeric order).
+__global__ __launch_bounds__(kThreads) void k_max_wavespeed_srmhd(Geo g, const double *V, const double *A,
+                                                                   unsigned long long *out) {
+  const long long vs = g.vstride;
+  double m[3] = {0.0, 0.0, 0.0};
+  for (long long q = (long long)blockIdx.x * kThreads + threadIdx.x; q < vs; q += (long long)gridDim.x * kThreads) {
+    const double rho = V[q], h = A[q], c = A[3 * vs + q], b2 = A[8 * vs + q], v2 = A[9 * vs + q];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d < g.dims) m[d] = fmax(m[d], srmhd_max_speed(rho, V[(1 + d) * vs + q], v2, h, c, b2));
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    for (int off = 32; off > 0; off >>= 1) m[d] = fmax(m[d], __shfl_xor(m[d], off));
+  __shared__ double s_m[kThreads / 64][3];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d) s_m[w][d] = m[d];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double r = 0.0;
+    for (int k = 0; k < kThreads / 64; ++k) r = fmax(r, s_m[k][threadIdx.x]);
+    atomicMax(out + threadIdx.x, (unsigned long long)__double_as_longlong(r));
+  }
 }
 
 __global__ void k_zero_int(int *p) {
@@ -719,6 +752,12 @@ void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A,
                      fail_flag, last_cell, st);
   hipLaunchKernelGGL(k_clear_fail_flags, dim3(64), dim3(kThreads), 0, s, fail_count, fail_list, fail_flag);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, const_cast<int *>(fail_count));
+}
+
+void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, unsigned long long *out,
+                                hipStream_t s) {
+  const long long blocks = std::min<long long>(blocks_for(g.vstride), 4096);
+  hipLaunchKernelGGL(k_max_wavespeed_srmhd, dim3((unsigned)blocks), dim3(kThreads), 0, s, g, V, A, out);
 }
 
 void launch_zero_int(int *p, hipStream_t s) { hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p); }

@@ -68,6 +68,10 @@ void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A,
                             const long long *fail_list, unsigned char *fail_flag, long long last_cell,
                             C2PCounters *st, hipStream_t s);
 void launch_zero_int(int *p, hipStream_t s);
+// Per-axis maximum fast magnetosonic speed over interior cells into out[3]
+// (bit patterns of non-negative doubles; out must be zero on entry).
+void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, unsigned long long *out,
+                                hipStream_t s);
 void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s);
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s);

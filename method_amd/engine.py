@@ -26,9 +26,13 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
+    "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_sizes", "mgpu_batch_set_prims", "mgpu_batch_upload_state",
+    "mgpu_batch_download_state", "mgpu_batch_step", "mgpu_batch_get_time", "mgpu_batch_use_graph",
+    "mgpu_batch_get_tuning", "mgpu_batch_set_tuning", "mgpu_batch_autotune", "mgpu_batch_timed_steps",
+    "mgpu_batch_sync",
 ]
 
 GUESS_PLAIN, GUESS_LEARNED, GUESS_COLD = 0, 1, 2
@@ -44,6 +48,25 @@ class MgpuConfig(ctypes.Structure):
         ("model", ctypes.c_int), ("weno_order", ctypes.c_int), ("integrator", ctypes.c_int),
         ("bc", ctypes.c_int * 3), ("quirks", ctypes.c_int), ("device", ctypes.c_int),
     ]
+
+
+class MgpuBatchConfig(ctypes.Structure):
+    _fields_ = [("Ng", ctypes.c_int), ("gamma", ctypes.c_double), ("weno_order", ctypes.c_int),
+                ("bc", ctypes.c_int * 2), ("quirks", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class MgpuInstance(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("end_time", ctypes.c_double)]
+
+
+class MgpuBatchTuning(ctypes.Structure):
+    _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("schedule", ctypes.c_int),
+                ("blocks", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class MgpuStats(ctypes.Structure):
@@ -88,6 +111,7 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_flux_vector.argtypes = [E, ctypes.c_int, dp]
     L.mgpu_source.argtypes = [E, dp]
     L.mgpu_c2p.argtypes = [E, sp]
+    L.mgpu_max_wavespeed.argtypes = [E, dp]
     fp = ctypes.POINTER(ctypes.c_float)
     L.mgpu_set_c2p_guess.argtypes = [E, ctypes.c_int, fp, fp]
     L.mgpu_sync.argtypes = [E]
@@ -103,6 +127,23 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_timed_steps.argtypes = [E, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mgpu_kernel_slot_name.argtypes = [E, ctypes.c_int]
     L.mgpu_kernel_slot_name.restype = ctypes.c_char_p
+    tp = ctypes.POINTER(MgpuBatchTuning)
+    L.mgpu_batch_create.argtypes = [ctypes.POINTER(MgpuBatchConfig), ctypes.POINTER(MgpuInstance), ctypes.c_int,
+                                    ctypes.POINTER(E)]
+    L.mgpu_batch_destroy.argtypes = [E]
+    L.mgpu_batch_sizes.argtypes = [E, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_int)]
+    L.mgpu_batch_set_prims.argtypes = [E, ctypes.c_int, dp]
+    L.mgpu_batch_upload_state.argtypes = [E, ctypes.c_int, dp, dp, dp]
+    L.mgpu_batch_download_state.argtypes = [E, ctypes.c_int, dp, dp, dp]
+    L.mgpu_batch_step.argtypes = [E, ctypes.c_int, sp]
+    L.mgpu_batch_get_time.argtypes = [E, ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_int)]
+    L.mgpu_batch_use_graph.argtypes = [E, ctypes.c_int]
+    L.mgpu_batch_get_tuning.argtypes = [E, tp]
+    L.mgpu_batch_set_tuning.argtypes = [E, tp]
+    L.mgpu_batch_autotune.argtypes = [E, ctypes.c_int, tp, tp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mgpu_batch_timed_steps.argtypes = [E, ctypes.c_int, dp]
+    L.mgpu_batch_sync.argtypes = [E]
     _lib = L
     return L
 
@@ -262,6 +303,12 @@ class Engine:
         rc = self._check(self.L.mgpu_c2p(self._h, ctypes.byref(st)), "mgpu_c2p")
         return rc, st.as_dict()
 
+    def max_wavespeed(self) -> np.ndarray:
+        """Per-axis largest |characteristic speed| over the interior cells."""
+        out = np.zeros(3)
+        self._check(self.L.mgpu_max_wavespeed(self._h, _dp(out)), "mgpu_max_wavespeed")
+        return out
+
     def timed_c2p(self, reps: int = 5):
         """(ms per cons2prim kernel call, counters of the last call), each call
         starting from the prims resident on entry."""
@@ -355,3 +402,128 @@ class Mlp:
         y = np.zeros((cons8.shape[0], 2), dtype=np.float32)
         self._check(self.L.mgpu_mlp_predict(self._h, _dp(cons8), cons8.shape[0], _fp(y)), "mgpu_mlp_predict")
         return y
+
+
+@dataclass
+class Instance:
+    """One member of a batched ensemble: a 2-D grid, its Courant number and
+    the time at which it stops stepping."""
+
+    nx: int
+    ny: int
+    xmin: float = -0.5
+    xmax: float = 0.5
+    ymin: float = -1.0
+    ymax: float = 1.0
+    cfl: float = 0.4
+    end_time: float = 1e10
+
+    def to_c(self) -> MgpuInstance:
+        c = MgpuInstance()
+        for f in ("nx", "ny", "xmin", "xmax", "ymin", "ymax", "cfl", "end_time"):
+            setattr(c, f, getattr(self, f))
+        return c
+
+
+class Batch:
+    """Independent 2-D ideal SRMHD simulations advanced together (C ABI mgpu_batch_*)."""
+
+    def __init__(self, instances, weno_order: int = 5, bc=(PERIODIC, OUTFLOW), gamma: float = 4.0 / 3.0,
+                 Ng: int = 4, quirks: int = 1, device: int = 0):
+        self.L = load_library()
+        self.instances = list(instances)
+        self.Ng = Ng
+        c = MgpuBatchConfig()
+        c.Ng, c.gamma, c.weno_order, c.quirks, c.device = Ng, gamma, weno_order, quirks, device
+        c.bc[0], c.bc[1] = int(bc[0]), int(bc[1])
+        arr = (MgpuInstance * len(self.instances))(*[i.to_c() for i in self.instances])
+        self._h = ctypes.c_void_p()
+        self._check(self.L.mgpu_batch_create(ctypes.byref(c), arr, len(self.instances), ctypes.byref(self._h)),
+                    "mgpu_batch_create")
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise EngineError(f"{what}: {self.L.mgpu_last_error().decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.L.mgpu_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sizes(self):
+        n, c, t = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+        self._check(self.L.mgpu_batch_sizes(self._h, ctypes.byref(n), ctypes.byref(c), ctypes.byref(t)), "sizes")
+        return n.value, c.value, t.value
+
+    def shape(self, i: int):
+        inst = self.instances[i]
+        return (inst.nx + 2 * self.Ng, inst.ny + 2 * self.Ng, 1)
+
+    def set_prims(self, i: int, prims: np.ndarray):
+        prims = np.ascontiguousarray(prims, dtype=np.float64)
+        assert prims.shape == (8,) + self.shape(i), prims.shape
+        self._check(self.L.mgpu_batch_set_prims(self._h, i, _dp(prims)), "mgpu_batch_set_prims")
+
+    def upload(self, i: int, cons, prims, aux):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (cons, prims, aux)]
+        self._check(self.L.mgpu_batch_upload_state(self._h, i, *[_dp(a) for a in arrs]), "mgpu_batch_upload_state")
+
+    def download(self, i: int):
+        sh = self.shape(i)
+        cons, prims, aux = np.zeros((9,) + sh), np.zeros((8,) + sh), np.zeros((13,) + sh)
+        self._check(self.L.mgpu_batch_download_state(self._h, i, _dp(cons), _dp(prims), _dp(aux)),
+                    "mgpu_batch_download_state")
+        return cons, prims, aux
+
+    def step(self, nsteps: int = 1, stats: bool = False):
+        st = MgpuStats() if stats else None
+        rc = self._check(self.L.mgpu_batch_step(self._h, nsteps, ctypes.byref(st) if st else None), "mgpu_batch_step")
+        return (rc, st.as_dict()) if stats else rc
+
+    def time(self, i: int):
+        t, dt, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(self.L.mgpu_batch_get_time(self._h, i, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(it)),
+                    "mgpu_batch_get_time")
+        return t.value, dt.value, it.value
+
+    def use_graph(self, on: bool):
+        self._check(self.L.mgpu_batch_use_graph(self._h, 1 if on else 0), "mgpu_batch_use_graph")
+
+    def tuning(self) -> dict:
+        t = MgpuBatchTuning()
+        self._check(self.L.mgpu_batch_get_tuning(self._h, ctypes.byref(t)), "mgpu_batch_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, tile_x: int, tile_y: int, schedule: int, blocks: int):
+        t = MgpuBatchTuning(tile_x, tile_y, schedule, blocks, 0.0)
+        self._check(self.L.mgpu_batch_set_tuning(self._h, ctypes.byref(t)), "mgpu_batch_set_tuning")
+
+    def autotune(self, trial_steps: int = 3):
+        """(chosen tuning, every candidate tried) -- the state is left as it was."""
+        best = MgpuBatchTuning()
+        cands = (MgpuBatchTuning * 256)()
+        n = ctypes.c_int()
+        self._check(self.L.mgpu_batch_autotune(self._h, trial_steps, ctypes.byref(best), cands, 256, ctypes.byref(n)),
+                    "mgpu_batch_autotune")
+        return best.as_dict(), [cands[k].as_dict() for k in range(min(n.value, 256))]
+
+    def timed_steps(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        self._check(self.L.mgpu_batch_timed_steps(self._h, nsteps, ctypes.byref(ms)), "mgpu_batch_timed_steps")
+        return ms.value
+
+    def sync(self):
+        self._check(self.L.mgpu_batch_sync(self._h), "mgpu_batch_sync")

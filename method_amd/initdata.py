@@ -158,3 +158,19 @@ def kh3d_grid(n: int = 256) -> Grid:
 def ot2d_grid(n: int = 1024) -> Grid:
     """2-D Orszag-Tang grid on [0,1]^2 (n x n interior cells, Ng=4)."""
     return Grid(n, n, 0, 0.0, 1.0, 0.0, 1.0)
+
+
+def kh_ensemble(n: int = 64, seed: int = 2024, sizes=(128, 256, 384, 512, 768)):
+    """Members of the batched Kelvin-Helmholtz ensemble: independent nx, ny drawn
+    uniformly from `sizes`, shear velocity U[0.2, 0.8], Bz U[0.01, 0.5] and a
+    per-member perturbation seed, all from one seeded generator. Returns a list
+    of (nx, ny, shear, bz, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nx = int(rng.choice(sizes))
+        ny = int(rng.choice(sizes))
+        shear = float(rng.uniform(0.2, 0.8))
+        bz = float(rng.uniform(0.01, 0.5))
+        out.append((nx, ny, shear, bz, int(seed * 1000 + k)))
+    return out

@@ -843,6 +843,39 @@ double oc_timestep(const oc_config *c, double t, int iters, double end_time) {
   return dt;
 }
 
+/* Largest |characteristic speed| along axis d of an ideal SRMHD state: the
+ * fast magnetosonic speed a (a^2 = cs^2 + va^2 - cs^2 va^2, va^2 = b^2/(rho h + b^2))
+ * taken as isotropic in the fluid frame and carried to the lab frame
+ * (Gammie, McKinney & Toth 2003, ApJ 589, 444, sec. 3.3, flat space). The
+ * reference has no such bound: its dt and flux split use the light speed
+ * (simulation.cc:39-41, fluxVectorSplitting.cc:10). */
+double oc_srmhd_max_speed(double rho, double vd, double v2, double h, double c, double b2) {
+  double cs2 = c * c;
+  double va2 = b2 / (rho * h + b2);
+  double a2 = cs2 + va2 - cs2 * va2;
+  double den = 1.0 - v2 * a2;
+  double disc = a2 * (1.0 - v2) * (den - vd * vd * (1.0 - a2));
+  double sq = sqrt(disc > 0.0 ? disc : 0.0);
+  double b = vd * (1.0 - a2);
+  double lp = fabs(b + sq), lm = fabs(b - sq);
+  return (lp > lm ? lp : lm) / den;
+}
+
+/* Maximum of oc_srmhd_max_speed over interior cells, per axis (0 for axes
+ * the grid does not have). aux entries: h = 0, c = 3, b^2 = 8, v^2 = 9. */
+void oc_max_wavespeed(const oc_config *c, const double *prims, const double *aux, double out[3]) {
+  geom g = mkgeom(c);
+  out[0] = out[1] = out[2] = 0.0;
+  for (int i = g.is; i < g.ie; i++)
+    for (int j = g.js; j < g.je; j++)
+      for (int k = g.ks; k < g.ke; k++)
+        for (int d = 0; d < g.dims; d++) {
+          double s = oc_srmhd_max_speed(prims[IX(0, i, j, k)], prims[IX(1 + d, i, j, k)], aux[IX(9, i, j, k)],
+                                        aux[IX(0, i, j, k)], aux[IX(3, i, j, k)], aux[IX(8, i, j, k)]);
+          if (s > out[d]) out[d] = s;
+        }
+}
+
 /* TimeIntegrator::finalise (timeInt.h:117-132): C2P, then BCs. */
 static long finalise(const oc_config *c, double *cons, double *prims, double *aux, oc_stats *st) {
   long fails = oc_c2p(c, cons, prims, aux, st);

@@ -79,6 +79,12 @@ int oc_srmhd_newton(double D, double Sx, double Sy, double Sz, double tau, doubl
 /* Simulation::updateTime's dt (simulation.cc:113-124). */
 double oc_timestep(const oc_config *c, double t, int iters, double end_time);
 
+/* Fast magnetosonic bound of one SRMHD state along an axis, and its maximum
+ * over the interior per axis (no counterpart in the reference, whose dt uses
+ * the light speed). */
+double oc_srmhd_max_speed(double rho, double vd, double v2, double h, double c, double b2);
+void oc_max_wavespeed(const oc_config *c, const double *prims, const double *aux, double out[3]);
+
 /* One RK2 step (RK2.cc:31-100) or one SSP2 IMEX step (SSP2.cc:53-173). */
 long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
 long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);

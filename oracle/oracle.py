@@ -72,6 +72,9 @@ def lib():
         L.oc_rk2_step.restype = ctypes.c_long
         L.oc_ssp2_step.argtypes = [cp, dp, dp, dp, ctypes.c_double, sp]
         L.oc_ssp2_step.restype = ctypes.c_long
+        L.oc_srmhd_max_speed.argtypes = [ctypes.c_double] * 6
+        L.oc_srmhd_max_speed.restype = ctypes.c_double
+        L.oc_max_wavespeed.argtypes = [cp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -179,6 +182,12 @@ class Oracle:
 
     def timestep(self, t: float, iters: int, end_time: float = 1e10) -> float:
         return self.L.oc_timestep(ctypes.byref(self.c), t, iters, end_time)
+
+    def max_wavespeed(self, prims, aux) -> np.ndarray:
+        """Per-axis maximum fast magnetosonic speed over interior cells (SRMHD)."""
+        out = np.zeros(3)
+        self.L.oc_max_wavespeed(ctypes.byref(self.c), _p(prims), _p(aux), _p(out))
+        return out
 
     def step(self, cons, prims, aux, dt: float, integrator: str = "rk2"):
         st = OcStats()

@@ -245,3 +245,26 @@ def test_full_size_periodic_shift_invariance():
     a, b = outs
     assert np.isfinite(a).all()
     assert np.array_equal(np.roll(np.roll(a, sx, axis=1), sz, axis=3), b)
+
+
+@pytest.mark.parametrize("grid", ["2d", "3d"])
+def test_max_wavespeed_matches_oracle(grid):
+    """CFL max-wavespeed reduction on the engine's own state after two steps,
+    against the oracle's reduction of the same downloaded prims and aux."""
+    if grid == "2d":
+        g, cfg = idt.kh2d_grid(64), KH2D
+    else:
+        g = idt.Grid(24, 20, 28, -0.5, 0.5, -1.0, 1.0, 0.0, 1.0)
+        cfg = dict(nx=24, ny=20, nz=28, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    say(f"gpu: max wavespeed {grid}")
+    with engine(weno=5, **cfg) as e:
+        e.set_prims(idt.kh_prims(g, shear=0.7, bz=0.3))
+        for _ in range(2):
+            e.update_time()
+        got = e.max_wavespeed()
+        c, p, a = e.download()
+    want = oracle_for(weno=5, **cfg).max_wavespeed(p, a)
+    say(f"  gpu {got}  oracle {want}")
+    dims = 2 if grid == "2d" else 3
+    assert np.all(got[:dims] > 0.0) and np.all(got < 1.0)
+    assert np.max(np.abs(got - want)) <= 1e-14

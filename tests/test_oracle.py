@@ -233,3 +233,79 @@ def test_ot2d_ssp2_steps_match_reference(golden):
         print(parity.report("oracle vs reference, OT 2-D 32^2 SRRMHD SSP2, 3 steps", rows))
         for n, dm, fm, rf, rg, pw in rows:
             assert rg <= 1e-9, n
+
+
+# ------------------------------------------------------ max wavespeed bound
+def _state(rng, gam=4.0 / 3.0):
+    """Random SRMHD state and the aux entries the bound reads (h, c, b^2, v^2)
+    from the primsToAll formulas (srmhd.cc:573-673)."""
+    rho = 10.0 ** rng.uniform(-2, 1)
+    p = rho * 10.0 ** rng.uniform(-2, 1)
+    d = rng.normal(size=3)
+    v = d / np.linalg.norm(d) * rng.uniform(0.0, 0.95)
+    B = rng.normal(size=3) * math.sqrt(p) * 10.0 ** rng.uniform(-1, 1)
+    W = 1.0 / math.sqrt(1.0 - v @ v)
+    h = 1.0 + p / rho * gam / (gam - 1.0)
+    c = math.sqrt(gam * p / (rho * h))
+    b0 = W * (B @ v)
+    b2 = (B @ B + b0 * b0) / (W * W)
+    return rho, v, h, c, b2
+
+
+def test_max_speed_special_cases():
+    L = orc.lib()
+    # hydro, motion along the axis: relativistic addition (v + cs) / (1 + v cs)
+    for v, cs in ((0.0, 0.3), (0.6, 0.5), (-0.9, 0.2), (0.3, 0.57)):
+        h = 2.0
+        got = L.oc_srmhd_max_speed(1.0, v, v * v, h, cs, 0.0)
+        assert abs(got - (abs(v) + cs) / (1.0 + abs(v) * cs)) < 1e-15
+    # fluid at rest: the fast speed a, a^2 = cs^2 + va^2 - cs^2 va^2
+    rho, h, cs, b2 = 0.7, 1.9, 0.4, 0.8
+    va2 = b2 / (rho * h + b2)
+    a = math.sqrt(cs * cs + va2 - cs * cs * va2)
+    assert abs(L.oc_srmhd_max_speed(rho, 0.0, 0.0, h, cs, b2) - a) < 1e-15
+
+
+def test_max_speed_is_the_boosted_envelope():
+    """Brute force: signals of the fluid-frame fast speed a in every direction,
+    carried to the lab frame by relativistic velocity addition
+    u = (u'/W + v + W/(W+1) (v.u') v) / (1 + v.u'); the largest |u_d| over the
+    directions is the bound (up to the sampling of directions)."""
+    L = orc.lib()
+    rng = np.random.default_rng(11)
+    n = rng.normal(size=(400000, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    for _ in range(12):
+        rho, v, h, c, b2 = _state(rng)
+        va2 = b2 / (rho * h + b2)
+        a = math.sqrt(c * c + va2 - c * c * va2)
+        W = 1.0 / math.sqrt(1.0 - v @ v)
+        up = a * n
+        vu = up @ v
+        u = (up / W + v[None, :] + (W / (W + 1.0)) * vu[:, None] * v[None, :]) / (1.0 + vu)[:, None]
+        for d in range(3):
+            brute = np.abs(u[:, d]).max()
+            got = L.oc_srmhd_max_speed(rho, v[d], v @ v, h, c, b2)
+            assert got < 1.0
+            assert brute <= got * (1.0 + 1e-12)
+            assert got - brute < 2e-4 * got, (d, got, brute)
+
+
+def test_max_wavespeed_grid_reduction():
+    """The grid maximum is the largest per-cell bound over interior cells and
+    axes the grid has; ghost cells are not read."""
+    g = orc.Case(model=orc.SRMHD, **KH2D)
+    o = orc.Oracle(g)
+    from method_amd import initdata as idt
+
+    p0 = idt.kh_prims(idt.kh2d_grid(64), shear=0.7, bz=0.3)
+    cons, prims, aux = o.set(p0)
+    prims_g = prims.copy()
+    prims_g[1, :4] = 0.99  # a ghost layer may hold anything
+    got = o.max_wavespeed(prims_g, aux)
+    L = orc.lib()
+    I = (slice(4, -4), slice(4, -4), 0)
+    want = [max(L.oc_srmhd_max_speed(r, vd, v2, h, c, b2) for r, vd, v2, h, c, b2 in
+                zip(prims[0][I].ravel(), prims[1 + d][I].ravel(), aux[9][I].ravel(), aux[0][I].ravel(),
+                    aux[3][I].ravel(), aux[8][I].ravel())) for d in range(2)]
+    assert got[0] == want[0] and got[1] == want[1] and got[2] == 0.0
