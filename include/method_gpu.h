@@ -227,6 +227,14 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best,
 int mgpu_batch_timed_steps(mgpu_batch *b, int nsteps, double *total_ms);
 int mgpu_batch_sync(mgpu_batch *b);
 
+/* Sweep plan of 3-D grids: 0 (default) runs the z and y terms of FVS::F
+ * (fluxVectorSplitting.cc:92-112) as two launches, 1 as one fused launch.
+ * The results are identical bit for bit; only the launch count, the HBM
+ * traffic of the net flux and the register pressure differ (on MI355X the
+ * fused launch holds more registers and is slower). 2-D grids are not
+ * affected. */
+int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz);
+
 /* Block until the engine's queued work is done. */
 int mgpu_sync(mgpu_engine *e);
 

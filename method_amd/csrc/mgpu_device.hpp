@@ -90,29 +90,51 @@ template <int ORDER>
 struct Weno;
 
 // Face value = upwind reconstruction of f+ plus downwind reconstruction of
-// f-, combined over one division: (nu dd + nd du) / (du dd).
+// f-, combined over one division: (nu dd + nd du) / (du dd). The two halves
+// are also available apart (up, down, combine) for kernels that stage f+ and
+// f- through LDS one after the other; the arithmetic is the same.
+__device__ __forceinline__ double weno_combine(double nu, double du, double nd, double dd) {
+  return fma(nu, dd, nd * du) / (du * dd);
+}
+
 template <>
 struct Weno<3> {
   static constexpr int R = 2;  // stencil radius = (order + 1) / 2
   // Face between cells R-1 and R of p[0 .. 2R-1] (upwind on fp, downwind on fm).
+  template <class LP>
+  __device__ __forceinline__ static void up(LP fp, double &n, double &d) {
+    weno3_nd(fp(0), fp(1), fp(2), n, d);
+  }
+  template <class LM>
+  __device__ __forceinline__ static void down(LM fm, double &n, double &d) {
+    weno3_nd(fm(3), fm(2), fm(1), n, d);
+  }
   template <class LP, class LM>
   __device__ __forceinline__ static double face(LP fp, LM fm) {
     double nu, du, nd, dd;
-    weno3_nd(fp(0), fp(1), fp(2), nu, du);
-    weno3_nd(fm(3), fm(2), fm(1), nd, dd);
-    return fma(nu, dd, nd * du) / (du * dd);
+    up(fp, nu, du);
+    down(fm, nd, dd);
+    return weno_combine(nu, du, nd, dd);
   }
 };
 
 template <>
 struct Weno<5> {
   static constexpr int R = 3;
+  template <class LP>
+  __device__ __forceinline__ static void up(LP fp, double &n, double &d) {
+    weno5_nd(fp(0), fp(1), fp(2), fp(3), fp(4), n, d);
+  }
+  template <class LM>
+  __device__ __forceinline__ static void down(LM fm, double &n, double &d) {
+    weno5_nd(fm(5), fm(4), fm(3), fm(2), fm(1), n, d);
+  }
   template <class LP, class LM>
   __device__ __forceinline__ static double face(LP fp, LM fm) {
     double nu, du, nd, dd;
-    weno5_nd(fp(0), fp(1), fp(2), fp(3), fp(4), nu, du);
-    weno5_nd(fm(5), fm(4), fm(3), fm(2), fm(1), nd, dd);
-    return fma(nu, dd, nd * du) / (du * dd);
+    up(fp, nu, du);
+    down(fm, nd, dd);
+    return weno_combine(nu, du, nd, dd);
   }
 };
 

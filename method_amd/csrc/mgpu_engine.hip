@@ -77,6 +77,7 @@ struct mgpu_engine {
   int guess_mode = MGPU_GUESS_PLAIN;  // starting guess of mgpu_c2p
   MlpDev *mlp = nullptr;              // learned guess network (device)
   unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
+  int fused_yz = 0;                      // 3-D: fused z+y sweep launch (off: slower here)
 };
 
 namespace {
@@ -151,10 +152,23 @@ SweepArgs sweep_args(mgpu_engine *e, const double *U, const double *V, C2PCounte
   return a;
 }
 
-// Sweeps in the order z, y, x; the x sweep applies `last_mode`.
+// Sweeps in the order z, y, x; the x sweep applies `last_mode`. On 3-D grids
+// the z and y sweeps can run as one fused launch instead; both plans give the
+// same bits.
 int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
   const int dims = e->geo.dims;
   const int order = e->cfg.weno_order, quirk = e->cfg.quirks;
+  if (dims == 3 && e->fused_yz) {
+    launch_sweep_yz(a, e->cfg.model, order, quirk, e->stream);
+    if (mark(e, "sweep_yz_write")) return -1;
+    const int mode = (last_mode == kSweepWrite || last_mode == kSweepAdd) ? kSweepAdd : last_mode;
+    launch_sweep(a, e->cfg.model, order, 0, mode, quirk, e->stream);
+    static const char *xnames[6] = {"sweep_x_write", "sweep_x_add", "sweep_x_stage1_c2p", "sweep_x_stage2_c2p",
+                                    "sweep_x_imex2", "sweep_x_final"};
+    if (mark(e, xnames[mode])) return -1;
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
   int first = 1;
   for (int dir = dims - 1; dir >= 0; --dir) {
     int mode = dir == 0 ? last_mode : (first ? kSweepWrite : kSweepAdd);
@@ -592,6 +606,12 @@ int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
   HIP_OK(hipMemcpyAsync(bits, e->dspeed, sizeof bits, hipMemcpyDeviceToHost, e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
   for (int d = 0; d < 3; ++d) memcpy(&out[d], &bits[d], sizeof(double));
+  return 0;
+}
+
+int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz) {
+  if (!e) return fail("null engine");
+  e->fused_yz = fused_yz ? 1 : 0;
   return 0;
 }
 

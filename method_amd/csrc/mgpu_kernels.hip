@@ -92,7 +92,10 @@ struct SweepShape {
   static constexpr int TLH = TL + 2 * R;      // with halo
   static constexpr int EAX = 3 - DAX - CAX;
   static constexpr int kFaces = (TL + 1) * TC;
-  static constexpr int kLds = 2 * M::NC * TLH * TC;
+  // SRMHD stages f+ and then f- through LDS (kSplit); the resistive model's
+  // 14 variables would need too many registers for that and keeps both.
+  static constexpr bool kSplit = M::NC <= 9;
+  static constexpr int kLds = (kSplit ? 1 : 2) * M::NC * TLH * TC;
   // LDS position of (variable slot v, cell l along the sweep, pencil c): the
   // unit-stride index follows the threads' fast index, which is l when the
   // sweep axis is the contiguous one (DFAST) and the pencil index otherwise.
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   using S = SweepShape<M, ORDER, DAX, CAX, DFAST>;
   constexpr int R = S::R, TL = S::TL, TC = S::TC, TLH = S::TLH, EAX = S::EAX;
   constexpr int NV = M::NC;
-  // f+ then f- for each variable: [2*NV][TLH][TC]; reused for face fluxes.
+  // f+ (then f-) of each variable [NV][TLH][TC]; reused for face fluxes.
   __shared__ double s_f[S::kLds];
 
   const Geo &g = a.geo;
@@ -199,70 +202,159 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   const long long vs = g.vstride;
   const int tid = threadIdx.x;
 
-  // Phase 1: split fluxes of every tile cell and its halo along DAX.
-  for (int idx = tid; idx < TLH * TC; idx += kThreads) {
-    const int l = DFAST ? idx % TLH : idx / TC;
-    const int cc = DFAST ? idx / TLH : idx % TC;
-    const int gc = c0 + cc;
-    double fp[NV], fm[NV];
-    if (gc < nc) {
-      // cells past nd + R - 1 only feed faces outside the grid; clamping them
-      // keeps the periodic image inside [0, nd) for any tile overhang
-      const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
-      const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
-      double u[NV], pl[M::NPL];
+  if constexpr (S::kSplit) {
+    // Phase 1: split fluxes of every tile cell and its halo along DAX. f+ goes
+    // to LDS now; each thread keeps its cells' f- in registers until the upwind
+    // half of every face is done, then f- takes f+'s place. LDS holds one of
+    // the two at a time, which halves it and lets more blocks share a CU.
+    constexpr int kCells = TLH * TC;
+    constexpr int kCellPasses = (kCells + kThreads - 1) / kThreads;
+    double fm_keep[kCellPasses][NV];
+  #pragma unroll
+    for (int cp = 0; cp < kCellPasses; ++cp) {
+      const int idx = tid + cp * kThreads;
+      if (idx < kCells) {
+        const int l = DFAST ? idx % TLH : idx / TC;
+        const int cc = DFAST ? idx / TLH : idx % TC;
+        const int gc = c0 + cc;
+        double fp[NV];
+        if (gc < nc) {
+          // cells past nd + R - 1 only feed faces outside the grid; clamping
+          // them keeps the periodic image inside [0, nd) for any tile overhang
+          const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
+          const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
+          double u[NV], pl[M::NPL];
+  #pragma unroll
+          for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
+  #pragma unroll
+          for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
+          double f[NV];
+          M::template flux<DAX, QUIRK>(u, pl, a, f);
+  #pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            fp[v] = 0.5 * (f[v] + u[v]);
+            fm_keep[cp][v] = 0.5 * (f[v] - u[v]);
+          }
+        } else {
+  #pragma unroll
+          for (int v = 0; v < NV; ++v) fp[v] = fm_keep[cp][v] = 0.0;
+        }
+  #pragma unroll
+        for (int v = 0; v < NV; ++v) s_f[S::fidx(v, l, cc)] = fp[v];
+      }
+    }
+    __syncthreads();
+
+    // Phase 2: WENO face values, face lf being the left face of tile cell lf.
+    // The (variable, face) pairs are dealt out one per thread per pass, so the
+    // faces beyond one per thread cost a fraction of a pass spread over the
+    // waves instead of a full pass of all variables on one wave. Upwind halves
+    // from f+ first, then downwind halves from f-.
+    constexpr int kItems = NV * S::kFaces;
+    constexpr int kPasses = (kItems + kThreads - 1) / kThreads;
+    double nu[kPasses], du[kPasses];
+  #pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const int item = tid + k * kThreads;
+      if (item < kItems) {
+        const int v = item / S::kFaces, idx = item - v * S::kFaces;
+        const int lf = DFAST ? idx % (TL + 1) : idx / TC;
+        const int cc = DFAST ? idx / (TL + 1) : idx % TC;
+        const double *fpv = s_f + S::fidx(v, lf, cc);
+        Weno<ORDER>::up([&](int m) { return fpv[m * S::kStrideL]; }, nu[k], du[k]);
+      }
+    }
+    __syncthreads();
+  #pragma unroll
+    for (int cp = 0; cp < kCellPasses; ++cp) {
+      const int idx = tid + cp * kThreads;
+      if (idx < kCells) {
+        const int l = DFAST ? idx % TLH : idx / TC;
+        const int cc = DFAST ? idx / TLH : idx % TC;
+  #pragma unroll
+        for (int v = 0; v < NV; ++v) s_f[S::fidx(v, l, cc)] = fm_keep[cp][v];
+      }
+    }
+    __syncthreads();
+  #pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const int item = tid + k * kThreads;
+      if (item < kItems) {
+        const int v = item / S::kFaces, idx = item - v * S::kFaces;
+        const int lf = DFAST ? idx % (TL + 1) : idx / TC;
+        const int cc = DFAST ? idx / (TL + 1) : idx % TC;
+        const double *fmv = s_f + S::fidx(v, lf, cc);
+        double nd, dd;
+        Weno<ORDER>::down([&](int m) { return fmv[m * S::kStrideL]; }, nd, dd);
+        nu[k] = weno_combine(nu[k], du[k], nd, dd);
+      }
+    }
+    __syncthreads();
+    // item = v * kFaces + idx, idx enumerating (lf, cc) in gidx order for both
+    // layouts, so the face of item goes to s_f[item]
+  #pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const int item = tid + k * kThreads;
+      if (item < kItems) s_f[item] = nu[k];
+    }
+    __syncthreads();
+  } else {
+    // Phase 1: f+ and f- of every tile cell and its halo side by side in LDS.
+    for (int idx = tid; idx < TLH * TC; idx += kThreads) {
+      const int l = DFAST ? idx % TLH : idx / TC;
+      const int cc = DFAST ? idx / TLH : idx % TC;
+      const int gc = c0 + cc;
+      double fp[NV], fm[NV];
+      if (gc < nc) {
+        const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
+        const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
+        double u[NV], pl[M::NPL];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
+        for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
 #pragma unroll
-      for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
-      double f[NV];
-      M::template flux<DAX, QUIRK>(u, pl, a, f);
+        for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
+        double f[NV];
+        M::template flux<DAX, QUIRK>(u, pl, a, f);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          fp[v] = 0.5 * (f[v] + u[v]);
+          fm[v] = 0.5 * (f[v] - u[v]);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
+      }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        fp[v] = 0.5 * (f[v] + u[v]);
-        fm[v] = 0.5 * (f[v] - u[v]);
+        s_f[S::fidx(v, l, cc)] = fp[v];
+        s_f[S::fidx(NV + v, l, cc)] = fm[v];
       }
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
     }
+    __syncthreads();
+    constexpr int kItems = NV * S::kFaces;
+    constexpr int kPasses = (kItems + kThreads - 1) / kThreads;
+    double face[kPasses];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      s_f[S::fidx(v, l, cc)] = fp[v];
-      s_f[S::fidx(NV + v, l, cc)] = fm[v];
+    for (int k = 0; k < kPasses; ++k) {
+      const int item = tid + k * kThreads;
+      if (item < kItems) {
+        const int v = item / S::kFaces, idx = item - v * S::kFaces;
+        const int lf = DFAST ? idx % (TL + 1) : idx / TC;
+        const int cc = DFAST ? idx / (TL + 1) : idx % TC;
+        const double *fpv = s_f + S::fidx(v, lf, cc);
+        const double *fmv = s_f + S::fidx(NV + v, lf, cc);
+        face[k] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
+                                    [&](int m) { return fmv[m * S::kStrideL]; });
+      }
     }
-  }
-  __syncthreads();
-
-  // Phase 2: WENO face values, face lf being the left face of tile cell lf.
-  // The (variable, face) pairs are dealt out one per thread per pass, so the
-  // faces beyond one per thread cost a fraction of a pass spread over the
-  // waves instead of a full pass of all variables on one wave.
-  constexpr int kItems = NV * S::kFaces;
-  constexpr int kPasses = (kItems + kThreads - 1) / kThreads;
-  double face[kPasses];
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kPasses; ++k) {
-    const int item = tid + k * kThreads;
-    if (item < kItems) {
-      const int v = item / S::kFaces, idx = item - v * S::kFaces;
-      const int lf = DFAST ? idx % (TL + 1) : idx / TC;
-      const int cc = DFAST ? idx / (TL + 1) : idx % TC;
-      const double *fpv = s_f + S::fidx(v, lf, cc);
-      const double *fmv = s_f + S::fidx(NV + v, lf, cc);
-      face[k] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
-                                  [&](int m) { return fmv[m * S::kStrideL]; });
+    for (int k = 0; k < kPasses; ++k) {
+      const int item = tid + k * kThreads;
+      if (item < kItems) s_f[item] = face[k];
     }
+    __syncthreads();
   }
-  __syncthreads();
-  // item = v * kFaces + idx, idx enumerating (lf, cc) in gidx order for both
-  // layouts, so the face of item goes to s_f[item]
-#pragma unroll
-  for (int k = 0; k < kPasses; ++k) {
-    const int item = tid + k * kThreads;
-    if (item < kItems) s_f[item] = face[k];
-  }
-  __syncthreads();
 
   // Phase 3: net flux of each tile cell, then the sweep's mode.
   const int l = DFAST ? tid % TL : tid / TC;
@@ -329,6 +421,162 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     }
   }
   if (MODE >= kSweepStage1) reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
+}
+
+// Fused z and y sweeps of a 3-D grid (FVS::F's z and y terms,
+// fluxVectorSplitting.cc:92-112). A TYT x TZT tile of one x plane
+// reconstructs its z faces, then its y faces, through LDS, and accumulates
+// the net flux in the order of the separate sweeps (z, then y as one fused
+// multiply-add), so the result is the same bit for bit; the z net flux is
+// re-read from L2 rather than making an HBM round trip.
+template <class M, int ORDER>
+struct YzShape {
+  static constexpr int TYT = 16, TZT = 16;  // rows along y, cells along z
+  static constexpr int R = Weno<ORDER>::R, NV = M::NC;
+  static constexpr int WZ = TZT + 2 * R, HY = TYT + 2 * R;
+  static constexpr int kLdsZ = 2 * NV * TYT * WZ, kLdsY = 2 * NV * HY * TZT;
+  static constexpr int kLds = kLdsZ > kLdsY ? kLdsZ : kLdsY;
+  static_assert(TYT * TZT == kThreads, "one cell per thread");
+};
+
+template <class M, int DIR, bool QUIRK>
+__device__ __forceinline__ void split_cell(const SweepArgs &a, long long q, double *fp, double *fm) {
+  const long long vs = a.geo.vstride;
+  double u[M::NC], pl[M::NPL], f[M::NC];
+#pragma unroll
+  for (int v = 0; v < M::NC; ++v) u[v] = a.U[v * vs + q];
+#pragma unroll
+  for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
+  M::template flux<DIR, QUIRK>(u, pl, a, f);
+#pragma unroll
+  for (int v = 0; v < M::NC; ++v) {
+    fp[v] = 0.5 * (f[v] + u[v]);
+    fm[v] = 0.5 * (f[v] - u[v]);
+  }
+}
+
+template <class M, int ORDER, bool QUIRK>
+__global__ __launch_bounds__(kThreads) void k_sweep_yz(SweepArgs a) {
+  using S = YzShape<M, ORDER>;
+  constexpr int TYT = S::TYT, TZT = S::TZT, R = S::R, NV = S::NV, WZ = S::WZ, HY = S::HY;
+  __shared__ double s_f[S::kLds];
+  const Geo &g = a.geo;
+  const int ny = g.n[1], nz = g.n[2];
+  const int tiles_z = (nz + TZT - 1) / TZT, tiles_y = (ny + TYT - 1) / TYT;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int tz = lb % tiles_z;
+  const int rest = lb / tiles_z;
+  const int ty = rest % tiles_y;
+  const int i = rest / tiles_y;
+  const int y0 = ty * TYT, z0 = tz * TZT;
+  const long long base = (long long)i * g.stride[0], sy = g.stride[1];
+  const long long vs = g.vstride;
+  const int tid = threadIdx.x;
+  const int r = tid / TZT, c = tid % TZT;  // the thread's cell: y0 + r, z0 + c
+
+  // ---- z faces: rows y0 .. y0+TYT-1, cells z0-R .. z0+TZT+R-1
+  for (int idx = tid; idx < TYT * WZ; idx += kThreads) {
+    const int rr = idx / WZ, cz = idx - rr * WZ;
+    const int gy = y0 + rr;
+    double fp[NV], fm[NV];
+    if (gy < ny) {
+      const int gz = remap(min(z0 + cz - R, nz - 1 + R), nz, g.periodic[2]);
+      split_cell<M, 2, QUIRK>(a, base + gy * sy + gz, fp, fm);
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      s_f[(v * TYT + rr) * WZ + cz] = fp[v];
+      s_f[((NV + v) * TYT + rr) * WZ + cz] = fm[v];
+    }
+  }
+  __syncthreads();
+  constexpr int kFacesZ = TYT * (TZT + 1), kItemsZ = NV * kFacesZ;
+  constexpr int kPassesZ = (kItemsZ + kThreads - 1) / kThreads;
+  double face_z[kPassesZ];
+#pragma unroll
+  for (int k = 0; k < kPassesZ; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < kItemsZ) {
+      const int v = item / kFacesZ, idx = item - v * kFacesZ;
+      const int rr = idx / (TZT + 1), lf = idx - rr * (TZT + 1);
+      const double *fpv = s_f + (v * TYT + rr) * WZ + lf;
+      const double *fmv = s_f + ((NV + v) * TYT + rr) * WZ + lf;
+      face_z[k] = Weno<ORDER>::face([&](int m) { return fpv[m]; }, [&](int m) { return fmv[m]; });
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the passes from piling up their LDS reads
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPassesZ; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < kItemsZ) s_f[item] = face_z[k];
+  }
+  __syncthreads();
+  // the z net flux goes to fnet now and is read back by the same thread
+  // below: the line is still in L2 then, so HBM sees one write of it, and no
+  // registers are held across the y phase
+  const bool live = y0 + r < ny && z0 + c < nz;
+  const long long q = base + (y0 + r) * sy + (z0 + c);
+  if (live) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double *gf = s_f + (v * TYT + r) * (TZT + 1) + c;
+      a.fnet[v * vs + q] = (gf[1] - gf[0]) * g.inv_d[2];
+    }
+  }
+  __syncthreads();
+
+  // ---- y faces: rows y0-R .. y0+TYT+R-1, cells z0 .. z0+TZT-1
+  for (int idx = tid; idx < HY * TZT; idx += kThreads) {
+    const int ry = idx / TZT, cc = idx - ry * TZT;
+    const int gz = z0 + cc;
+    double fp[NV], fm[NV];
+    if (gz < nz) {
+      const int gy = remap(min(y0 + ry - R, ny - 1 + R), ny, g.periodic[1]);
+      split_cell<M, 1, QUIRK>(a, base + gy * sy + gz, fp, fm);
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      s_f[(v * HY + ry) * TZT + cc] = fp[v];
+      s_f[((NV + v) * HY + ry) * TZT + cc] = fm[v];
+    }
+  }
+  __syncthreads();
+  constexpr int kFacesY = (TYT + 1) * TZT, kItemsY = NV * kFacesY;
+  constexpr int kPassesY = (kItemsY + kThreads - 1) / kThreads;
+  double face_y[kPassesY];
+#pragma unroll
+  for (int k = 0; k < kPassesY; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < kItemsY) {
+      const int v = item / kFacesY, idx = item - v * kFacesY;
+      const int lf = idx / TZT, cc = idx - lf * TZT;
+      const double *fpv = s_f + (v * HY + lf) * TZT + cc;
+      const double *fmv = s_f + ((NV + v) * HY + lf) * TZT + cc;
+      face_y[k] = Weno<ORDER>::face([&](int m) { return fpv[m * TZT]; }, [&](int m) { return fmv[m * TZT]; });
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the passes from piling up their LDS reads
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPassesY; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < kItemsY) s_f[item] = face_y[k];
+  }
+  __syncthreads();
+
+  if (!live) return;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double *gf = s_f + (v * (TYT + 1) + r) * TZT + c;
+    a.fnet[v * vs + q] = fma(gf[TZT] - gf[0], g.inv_d[1], a.fnet[v * vs + q]);
+  }
 }
 
 // SRMHD::primsToAll over interior cells (srmhd.cc:573-673).
@@ -719,7 +967,27 @@ void launch_sweep_m(const SweepArgs &a, int dir, int mode, int quirk, hipStream_
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
+template <class M, int ORDER>
+void launch_sweep_yz_m(const SweepArgs &a, int quirk, hipStream_t s) {
+  using S = YzShape<M, ORDER>;
+  const int tiles = ((a.geo.n[1] + S::TYT - 1) / S::TYT) * ((a.geo.n[2] + S::TZT - 1) / S::TZT) * a.geo.n[0];
+  if (quirk)
+    hipLaunchKernelGGL((k_sweep_yz<M, ORDER, true>), dim3(tiles), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep_yz<M, ORDER, false>), dim3(tiles), dim3(kThreads), 0, s, a);
+}
+
 }  // namespace
+
+void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStream_t s) {
+  if (model == 0) {
+    if (order == 3) launch_sweep_yz_m<SrmhdM, 3>(a, quirk, s);
+    else launch_sweep_yz_m<SrmhdM, 5>(a, quirk, s);
+  } else {
+    if (order == 3) launch_sweep_yz_m<SrrmhdM, 3>(a, quirk, s);
+    else launch_sweep_yz_m<SrrmhdM, 5>(a, quirk, s);
+  }
+}
 
 void launch_sweep(const SweepArgs &a, int model, int order, int dir, int mode, int quirk, hipStream_t s) {
   if (model == 0) {

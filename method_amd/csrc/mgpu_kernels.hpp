@@ -52,6 +52,8 @@ struct SweepArgs {
 };
 
 void launch_sweep(const SweepArgs &a, int model, int order, int dir, int mode, int quirk, hipStream_t s);
+// Fused z and y sweeps of a 3-D grid: writes their summed net flux to a.fnet.
+void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStream_t s);
 void launch_imex1_srrmhd(const SweepArgs &a, hipStream_t s);
 void launch_prims_to_all_srrmhd(const Geo &g, const double *V, double *U, double *A, double gam, double sigma,
                                 hipStream_t s);

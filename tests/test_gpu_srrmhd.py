@@ -149,3 +149,19 @@ def test_ot_1024_two_steps_stay_finite_and_symmetric():
     assert np.isfinite(ci).all()
     rho = ci[0]
     assert np.max(np.abs(rho - rho[::-1, ::-1])) <= 1e-10 * rho.max()
+
+
+def test_fused_yz_sweep_equals_separate_sweeps_bitwise():
+    g, p0 = ot3d_prims(20)
+    grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    say("gpu: SRRMHD fused yz vs separate sweeps")
+    out = []
+    for fused in (True, False):
+        with engine(weno=5, **grid) as e:
+            e.set_sweep_plan(fused)
+            e.set_prims(p0)
+            for _ in range(2):
+                assert e.update_time() == 0
+            out.append(e.download())
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
