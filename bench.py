@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-members", type=int, default=64)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-ensemble leg and print its JSON (for profiling its kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -63,8 +65,44 @@ def calibrate_copy():
                       "launches": 3}))
 
 
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "method_ref")
+
+
 def cpu_baseline(n: int, steps: int):
-    """The C oracle (serial restatement of the reference step) on the host."""
+    """The reference's own serial CPU build (oracle/_ref/method_ref, compiled from
+    the reference sources) when it is present, else the C oracle, on one host
+    core, over a bounded sample of the workload: KH 3-D n^3, WENO5, RK2."""
+    if os.path.exists(REF_BIN):
+        return reference_baseline(n, steps)
+    return port_baseline(n, steps)
+
+
+def reference_baseline(n: int, steps: int):
+    import subprocess
+    import tempfile
+
+    from method_amd import initdata as idt
+
+    g = idt.kh3d_grid(n)
+    prims0 = idt.kh_prims(g, seed=1234)
+    with tempfile.TemporaryDirectory() as td:
+        pin = os.path.join(td, "prims.bin")
+        np.ascontiguousarray(prims0).tofile(pin)
+        args = ["model=srmhd", f"nx={n}", f"ny={n}", f"nz={n}", "xmin=-0.5", "xmax=0.5", "ymin=-1.0", "ymax=1.0",
+                "zmin=0.0", "zmax=1.0", "Ng=4", f"gamma={4.0 / 3.0!r}", "cfl=0.4", "weno=5", "integrator=rk2",
+                "bcx=periodic", "bcy=outflow", "bcz=periodic", "mode=run", f"steps={steps}", f"prims_in={pin}",
+                f"out={os.path.join(td, 'out')}", "dump=-1"]
+        r = subprocess.run([REF_BIN] + args, capture_output=True, text=True, cwd=td)
+    if r.returncode != 0:
+        raise RuntimeError("reference CPU build failed: " + r.stderr[-2000:])
+    secs = float(r.stdout.split("step_seconds=")[1].split()[0])
+    return {"value": n ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"reference METHOD serial CPU build (Simulation::updateTime with RK2, FVS, WENO5, SRMHD "
+                      f"with cminpack hybrd C2P), KH 3-D {n}^3, {steps} steps, {secs:.1f} s inside updateTime "
+                      f"on 1 host core"}
+
+
+def port_baseline(n: int, steps: int):
     from method_amd import initdata as idt
     from oracle import oracle as orc
 
@@ -208,6 +246,9 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU halo exchange "
                                    "is a design note only)"}))
         sys.exit(2)
+    if args.batch_only:
+        print(json.dumps({"batched_ensemble": batch_leg(args.batch_steps, args.batch_members)}))
+        return
     from method_amd import engine as eng
     from method_amd import initdata as idt
 

@@ -36,6 +36,7 @@
 #include "weno.h"
 #include "cminpack.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -293,13 +294,16 @@ int main(int argc, char *argv[]) {
   int steps = argi("steps", 10);
   FILE *log = fopen((out + "_log.txt").c_str(), "w");
   if (dumps.count(0)) dump_state(out + "_step0.bin", &data);
+  double step_seconds = 0.0;  // time inside updateTime only (no dumps, no logging)
   for (int s = 1; s <= steps; s++) {
+    const auto t0 = std::chrono::steady_clock::now();
     sim.updateTime();
+    step_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     fprintf(log, "step %d t %.17g dt %.17g\n", s, data.t, data.dt);
     if (dumps.count(s)) dump_state(out + "_step" + std::to_string(s) + ".bin", &data);
   }
   fprintf(log, "c2p_calls %ld c2p_cells %ld c2p_fail %ld\n", g_c2p_calls, g_c2p_cells, g_c2p_fail);
   fclose(log);
-  printf("done steps=%d c2p_fail=%ld\n", steps, g_c2p_fail);
+  printf("done steps=%d c2p_fail=%ld step_seconds=%.6f\n", steps, g_c2p_fail, step_seconds);
   return 0;
 }
