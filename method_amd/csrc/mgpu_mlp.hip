@@ -35,6 +35,10 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// FP32 -> BF16 bits, round to nearest even: the hardware conversion on the
+// device, the same rounding in integer arithmetic on the host.
+__device__ __forceinline__ short bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+
 __host__ __device__ inline uint16_t f2bf(float x) {
   uint32_t u;
   memcpy(&u, &x, 4);
@@ -181,28 +185,24 @@ __device__ __forceinline__ void mlp_wave(const MlpDev &m, const float feat[8], f
     {
       const float *f = sfeat + (16 * t + lr) * 8 + 4 * lq;
       const bool in = lq < 2;
-      x = bf16x4{(short)(in ? f2bf(f[0]) : 0), (short)(in ? f2bf(f[1]) : 0), (short)(in ? f2bf(f[2]) : 0),
-                 (short)(in ? f2bf(f[3]) : 0)};
+      x = bf16x4{bf(in ? f[0] : 0.f), bf(in ? f[1] : 0.f), bf(in ? f[2] : 0.f), bf(in ? f[3] : 0.f)};
     }
     bf16x4 h1[4], h2[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      // the accumulator starts from the bias of its rows
+      floatx4 acc = {m.b1[16 * mt + 4 * lq], m.b1[16 * mt + 4 * lq + 1], m.b1[16 * mt + 4 * lq + 2],
+                     m.b1[16 * mt + 4 * lq + 3]};
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[mt], x, acc, 0, 0, 0);
-      short hv[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) hv[e] = (short)f2bf(gelu(acc[e] + m.b1[16 * mt + 4 * lq + e]));
-      h1[mt] = bf16x4{hv[0], hv[1], hv[2], hv[3]};
+      h1[mt] = bf16x4{bf(gelu(acc[0])), bf(gelu(acc[1])), bf(gelu(acc[2])), bf(gelu(acc[3]))};
     }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      floatx4 acc = {m.b2[16 * mt + 4 * lq], m.b2[16 * mt + 4 * lq + 1], m.b2[16 * mt + 4 * lq + 2],
+                     m.b2[16 * mt + 4 * lq + 3]};
 #pragma unroll
       for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[mt][kt], h1[kt], acc, 0, 0, 0);
-      short hv[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) hv[e] = (short)f2bf(gelu(acc[e] + m.b2[16 * mt + 4 * lq + e]));
-      h2[mt] = bf16x4{hv[0], hv[1], hv[2], hv[3]};
+      h2[mt] = bf16x4{bf(gelu(acc[0])), bf(gelu(acc[1])), bf(gelu(acc[2])), bf(gelu(acc[3]))};
     }
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

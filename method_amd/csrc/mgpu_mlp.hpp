@@ -55,18 +55,25 @@ __device__ __host__ inline void mlp_targets(double W, double p, double D, double
 }
 
 // tanh-form GELU written through the logistic function:
-// 0.5 x (1 + tanh(c (x + 0.044715 x^3))) = x sigmoid(2 c (x + 0.044715 x^3)).
+// 0.5 x (1 + tanh(c (x + 0.044715 x^3))) = x sigmoid(2 c (x + 0.044715 x^3)),
+// with the logistic's exp taken as a base-2 exponential (log2 e folded into
+// the constants) and its reciprocal as the hardware reciprocal: one exp2, one
+// rcp and five FP32 operations, for training and inference alike.
 constexpr float kGeluK = 1.5957691216057308f;  // 2 sqrt(2/pi)
 constexpr float kGeluA = 0.044715f;
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kGeluK2 = -kGeluK * kLog2e;           // exp(-u) = exp2(x (kGeluK2 + kGeluKA2 x^2))
+constexpr float kGeluKA2 = -kGeluK * kGeluA * kLog2e;
 
-__device__ __forceinline__ float gelu(float x) {
-  const float u = kGeluK * (x + kGeluA * x * x * x);
-  return x / (1.0f + __expf(-u));
+__device__ __forceinline__ float gelu_sigmoid(float x) {
+  const float t = x * fmaf(kGeluKA2, x * x, kGeluK2);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
 
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sigmoid(x); }
+
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float u = kGeluK * (x + kGeluA * x * x * x);
-  const float s = 1.0f / (1.0f + __expf(-u));
+  const float s = gelu_sigmoid(x);
   return s + x * s * (1.0f - s) * kGeluK * (1.0f + 3.0f * kGeluA * x * x);
 }
 
