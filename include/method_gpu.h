@@ -31,7 +31,9 @@ extern "C" {
 #define MGPU_ABI_VERSION 1
 
 enum { MGPU_SRMHD = 0, MGPU_SRRMHD = 1 };
-enum { MGPU_RK2 = 0, MGPU_SSP2 = 1 };
+/* MGPU_RKSPLIT: RK2 followed by the operator-split source update and
+ * another cons2prim (RKSplit::step, rkSplit.cc:31-53). */
+enum { MGPU_RK2 = 0, MGPU_SSP2 = 1, MGPU_RKSPLIT = 2 };
 enum { MGPU_BC_OUTFLOW = 0, MGPU_BC_PERIODIC = 1 };
 
 /* Simulation parameters: the constructor arguments of the reference Data
@@ -46,7 +48,7 @@ typedef struct {
   double gamma, sigma, cp, cfl;
   int model;       /* MGPU_SRMHD | MGPU_SRRMHD */
   int weno_order;  /* 3 | 5 */
-  int integrator;  /* MGPU_RK2 | MGPU_SSP2 */
+  int integrator;  /* MGPU_RK2 | MGPU_SSP2 | MGPU_RKSPLIT */
   int bc[3];       /* per axis: MGPU_BC_PERIODIC | MGPU_BC_OUTFLOW */
   int quirks;      /* 1: reproduce the reference's flux-index quirks */
   int device;      /* HIP device ordinal */

@@ -258,11 +258,27 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   return run_sweeps(e, c, kSweepImexFinal);
 }
 
+// RKSplit::step (rkSplit.cc:31-53): RK2::step, then cons += dt source and
+// finalise (cons2prim, with the SRMHD retry).
+int rksplit_step(mgpu_engine *e, double dt, C2PCounters *st) {
+  if (rk2_step(e, dt, st)) return -1;
+  if (e->cfg.model == MGPU_SRMHD)
+    launch_split_srmhd(e->geo, e->U, e->V, e->A, dt, e->cfg.gamma, e->cfg.cp, st, e->fail_count, e->fail_list,
+                       e->fail_flag, e->stream);
+  else
+    launch_split_srrmhd(e->geo, e->U, e->V, e->A, dt, e->cfg.gamma, e->cfg.sigma, e->cfg.cp, st, e->stream);
+  HIP_OK(hipGetLastError());
+  if (mark(e, "split_source_c2p")) return -1;
+  return retry(e, e->U, e->V, e->A, st);
+}
+
 int do_step(mgpu_engine *e, double dt, mgpu_stats *stats) {
   C2PCounters *st = stats ? e->dstats : nullptr;
   if (stats && zero_stats(e)) return -1;
   if (e->cfg.integrator == MGPU_SSP2) {
     if (ssp2_step(e, dt, st)) return -1;
+  } else if (e->cfg.integrator == MGPU_RKSPLIT) {
+    if (rksplit_step(e, dt, st)) return -1;
   } else if (rk2_step(e, dt, st)) {
     return -1;
   }
@@ -297,7 +313,7 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   *out = nullptr;
   const mgpu_config &c = *cfg;
   if (c.model != MGPU_SRMHD && c.model != MGPU_SRRMHD) return fail("mgpu_create: unknown model %d", c.model);
-  if (c.integrator != MGPU_RK2 && c.integrator != MGPU_SSP2)
+  if (c.integrator != MGPU_RK2 && c.integrator != MGPU_SSP2 && c.integrator != MGPU_RKSPLIT)
     return fail("mgpu_create: unknown integrator %d", c.integrator);
   if (c.model == MGPU_SRMHD && c.integrator == MGPU_SSP2)
     return fail("mgpu_create: SSP2 is provided with the SRRMHD model only");

@@ -797,6 +797,49 @@ __global__ __launch_bounds__(kThreads) void k_source_srmhd(Geo g, PadGeo pg, con
   dst[8 * pg.ntot + p] = -U[8 * g.vstride + q] / (cp * cp);
 }
 
+// RKSplit's source half-step and finalise (rkSplit.cc:36-53): cons += dt
+// source(cons) with SRMHD's only source, -phi / cp^2 on phi (srmhd.cc:233-250),
+// then cons2prim from the current primitives (srmhd.cc:390-562). The other
+// conserved variables are unchanged, so only phi is written back.
+__global__ __launch_bounds__(kThreads) void k_split_srmhd(Geo g, double *U, double *V, double *A, double dt,
+                                                           double gam, double cp, C2PCounters *st, int *fail_count,
+                                                           long long *fail_list, unsigned char *fail_flag) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  int cells = 0, iters = 0, fails = 0, maxit = 0;
+  if (q < g.vstride) {
+    const long long vs = g.vstride;
+    double u[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
+    const double src = -u[8] / (cp * cp);
+    u[8] = u[8] + dt * src;
+    U[8 * vs + q] = u[8];
+    double v2, Z;
+    srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
+    const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
+    const double Bsq = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+    const double Ssq = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
+    const int its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
+    double P[8], aux[13];
+    srmhd_recover(u, gam, v2, Z, P, aux);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
+#pragma unroll
+    for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+    cells = 1;
+    if (its < 0) {
+      fails = 1;
+      const int slot = atomicAdd(fail_count, 1);
+      fail_list[slot] = q;
+      fail_flag[q] = 1;
+    } else {
+      iters = its;
+      maxit = its;
+    }
+  }
+  reduce_stats(st, cells, iters, fails, maxit);
+}
+
 // ------------------------------------------------------------- SRRMHD
 
 // SSP2 stage one for every interior cell (SSP2.cc:72-105): solve
@@ -915,6 +958,45 @@ __global__ __launch_bounds__(kThreads) void k_source_srrmhd(Geo g, PadGeo pg, co
   dst[13 * pg.ntot + p] = 0.0;
 }
 
+// RKSplit's source half-step and finalise for the resistive model: the source
+// of SRRMHD::sourceTerm (srrmhd.cc:209-254) from the current state, cons +=
+// dt source, then the secant cons2prim from the current rho h W^2.
+__global__ __launch_bounds__(kThreads) void k_split_srrmhd(Geo g, double *U, double *V, double *A, double dt,
+                                                            double gam, double sigma, double cp, C2PCounters *st) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  int cells = 0, iters = 0, fails = 0, maxit = 0;
+  if (q < g.vstride) {
+    const long long vs = g.vstride;
+    double u[kSrrmhdCons], P[kSrrmhdPrims], aux[kSrrmhdAux];
+    for (int v = 0; v < kSrrmhdCons; ++v) u[v] = U[v * vs + q];
+    for (int v = 0; v < kSrrmhdPrims; ++v) P[v] = V[v * vs + q];
+    double uu[kSrrmhdCons];
+    srrmhd_prims_to_all_local(P, gam, uu, aux);
+    double J[3];
+    srrmhd_current(u, P, aux, sigma, J);
+    aux[4] = J[0];
+    aux[5] = J[1];
+    aux[6] = J[2];
+    double s5[5];
+    srrmhd_source5(u, aux, cp, s5);
+    for (int m = 0; m < 5; ++m) u[8 + m] = u[8 + m] + dt * s5[m];
+    const double Zg = srrmhd_guess_Z(P[0], P[1], P[2], P[3], P[4], gam);
+    double Pn[kSrrmhdPrims], An[kSrrmhdAux];
+    const int its = srrmhd_c2p_cell(u, gam, sigma, Zg, Pn, An);
+    for (int m = 0; m < 5; ++m) U[(8 + m) * vs + q] = u[8 + m];
+    for (int v = 0; v < kSrrmhdPrims; ++v) V[v * vs + q] = Pn[v];
+    for (int v = 0; v < kSrrmhdAux; ++v) A[v * vs + q] = An[v];
+    cells = 1;
+    if (its < 0) {
+      fails = 1;
+    } else {
+      iters = its;
+      maxit = its;
+    }
+  }
+  reduce_stats(st, cells, iters, fails, maxit);
+}
+
 template <int DIR, bool QUIRK>
 __global__ __launch_bounds__(kThreads) void k_flux_vector_srrmhd(Geo g, PadGeo pg, const double *U, const double *V,
                                                                   double *dst, double gam, double sigma) {
@@ -1026,6 +1108,19 @@ void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, 
                                 hipStream_t s) {
   const long long blocks = std::min<long long>(blocks_for(g.vstride), 4096);
   hipLaunchKernelGGL(k_max_wavespeed_srmhd, dim3((unsigned)blocks), dim3(kThreads), 0, s, g, V, A, out);
+}
+
+void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double cp,
+                        C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_split_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, dt, gam, cp, st,
+                     fail_count, fail_list, fail_flag);
+}
+
+void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double sigma,
+                         double cp, C2PCounters *st, hipStream_t s) {
+  hipLaunchKernelGGL(k_split_srrmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, dt, gam, sigma,
+                     cp, st);
 }
 
 void launch_zero_int(int *p, hipStream_t s) { hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p); }

@@ -70,6 +70,12 @@ void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A,
                             const long long *fail_list, unsigned char *fail_flag, long long last_cell,
                             C2PCounters *st, hipStream_t s);
 void launch_zero_int(int *p, hipStream_t s);
+// RKSplit's source half-step and cons2prim over interior cells.
+void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double cp,
+                        C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
+                        hipStream_t s);
+void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double sigma,
+                         double cp, C2PCounters *st, hipStream_t s);
 // Per-axis maximum fast magnetosonic speed over interior cells into out[3]
 // (bit patterns of non-negative doubles; out must be zero on entry).
 void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, unsigned long long *out,

@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libmethod_gpu.so")
 
 SRMHD, SRRMHD = 0, 1
-RK2, SSP2 = 0, 1
+RK2, SSP2, RKSPLIT = 0, 1, 2
 OUTFLOW, PERIODIC = 0, 1
 
 EXPORTED = [

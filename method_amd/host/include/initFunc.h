@@ -31,6 +31,16 @@ class KHShearLayers : public InitialFunc {
                 double rhoIn = 0.55, double rhoOut = 0.45, double amp = 0.01, double pressure = 1.0);
 };
 
+// The reference's single-fluid Kelvin-Helmholtz state (initFunc.cc:512-563):
+// shear layers at y = +-0.5 of width a = 0.01 with rho = 0.55 +- 0.45 tanh,
+// vx = +-0.5 tanh, a vy bump 0.1 * 0.5 sin(2 pi x) exp(-(y -+ 0.5)^2 / 0.1^2),
+// p = 1, and Bz = 0.1 when mag is non-zero (ideal E = -v x B for resistive
+// models). Requires gamma = 4/3 and the domain [-0.5, 0.5] x [-1, 1].
+class KHInstabilitySingleFluid : public InitialFunc {
+ public:
+  KHInstabilitySingleFluid(Data *data, int mag = 0);
+};
+
 // Uniform draw in [0,1) of the counter stream used by KHShearLayers.
 double initUniform01(uint64_t seed, uint64_t index, int stream);
 

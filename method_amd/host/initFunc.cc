@@ -54,3 +54,36 @@ KHShearLayers::KHShearLayers(Data *data, uint64_t seed, double shear, double bz,
         d->prims[ID(7, i, j, k)] = bz;
       }
 }
+
+KHInstabilitySingleFluid::KHInstabilitySingleFluid(Data *data, int mag) : InitialFunc(data) {
+  Data *d = data;
+  if (d->Nprims > 15)
+    throw std::invalid_argument(
+        "Trying to implement a single fluid initial state on incorrect model.\n\tModel has wrong number of primitive "
+        "variables to be single fluid model.");
+  if (d->gamma != 4.0 / 3.0) throw std::invalid_argument("Expected the index gamma = 4/3\n");
+  if (d->xmin != -0.5 || d->xmax != 0.5) throw std::invalid_argument("Domain has incorrect values. Expected x E [-0.5, 0.5]\n");
+  if (d->ymin != -1.0 || d->ymax != 1.0) throw std::invalid_argument("Domain has incorrect values. Expected y E [-1.0, 1.0]\n");
+  const double pi = 3.14159265358979323846;
+  const double sig = 0.1, vShear = 0.5, A0 = 0.1, a = 0.01, rho0 = 0.55, rho1 = 0.45, B0 = 0.1;
+  for (int i = 0; i < d->Nx; i++)
+    for (int j = 0; j < d->Ny; j++)
+      for (int k = 0; k < d->Nz; k++) {
+        double *P = d->prims;
+        P[ID(4, i, j, k)] = 1.0;
+        if (mag) P[ID(7, i, j, k)] = B0;
+        // the layer at y = +0.5 for y > 0, its mirror image below
+        const double s = d->y[j] > 0 ? 1.0 : -1.0;
+        const double yl = d->y[j] - s * 0.5;
+        const double t = std::tanh(yl / a);
+        P[ID(0, i, j, k)] = d->y[j] > 0 ? rho0 + rho1 * t : rho0 - rho1 * t;
+        P[ID(1, i, j, k)] = d->y[j] > 0 ? vShear * t : -vShear * t;
+        const double bump = A0 * vShear * std::sin(2 * pi * d->x[i]) * (std::exp(-std::pow(yl, 2) / (sig * sig)));
+        P[ID(2, i, j, k)] = d->y[j] > 0 ? bump : -bump;
+        if (d->Ncons > 9) {
+          P[ID(8, i, j, k)] = -(P[ID(2, i, j, k)] * P[ID(7, i, j, k)] - P[ID(3, i, j, k)] * P[ID(6, i, j, k)]);
+          P[ID(9, i, j, k)] = -(P[ID(3, i, j, k)] * P[ID(5, i, j, k)] - P[ID(1, i, j, k)] * P[ID(7, i, j, k)]);
+          P[ID(10, i, j, k)] = -(P[ID(1, i, j, k)] * P[ID(6, i, j, k)] - P[ID(2, i, j, k)] * P[ID(5, i, j, k)]);
+        }
+      }
+}

@@ -3,6 +3,7 @@
 #include <stdexcept>
 
 #include "RK2.h"
+#include "rkSplit.h"
 #include "gpu_link.h"
 
 using method_host::check;
@@ -45,3 +46,10 @@ void RK2::step(double *cons, double *prims, double *aux, double dt) {
   d->gpu.deviceCurrent = own;
   d->gpu.hostStale = false;
 }
+
+RKSplit::RKSplit(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : RK2(data, model, bcs, fluxMethod, ext) {
+  data->gpu.integrator = MGPU_RKSPLIT;
+}
+
+void RKSplit::setSource(double *cons, double *prims, double *aux) { model->sourceTerm(cons, prims, aux, data->source); }

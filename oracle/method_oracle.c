@@ -918,6 +918,24 @@ long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, d
   return fails;
 }
 
+/* RKSplit::step (rkSplit.cc:31-53): RK2::step, then cons += dt * source over
+ * the interior (source from Model::sourceTerm on the whole grid), then
+ * finalise. */
+long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st) {
+  long fails = oc_rk2_step(c, cons, prims, aux, dt, st);
+  geom g = mkgeom(c);
+  int nc = oc_ncons(c->model);
+  double *src = (double *)calloc(g.ntot * nc, sizeof(double));
+  oc_source(c, cons, prims, aux, src);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) cons[IX(v, i, j, k)] += dt * src[IX(v, i, j, k)];
+  free(src);
+  fails += finalise(c, cons, prims, aux, st);
+  return fails;
+}
+
 /* ----------------------------------------------------------- SSP2 IMEX */
 
 /* IMEX SSP2(222) constants as the reference's SSP2 object ends up holding them

@@ -88,6 +88,8 @@ void oc_max_wavespeed(const oc_config *c, const double *prims, const double *aux
 /* One RK2 step (RK2.cc:31-100) or one SSP2 IMEX step (SSP2.cc:53-173). */
 long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
 long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
+/* One RKSplit step (rkSplit.cc:31-53): RK2, split source, finalise. */
+long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,8 @@ def lib():
         L.oc_rk2_step.restype = ctypes.c_long
         L.oc_ssp2_step.argtypes = [cp, dp, dp, dp, ctypes.c_double, sp]
         L.oc_ssp2_step.restype = ctypes.c_long
+        L.oc_rksplit_step.argtypes = [cp, dp, dp, dp, ctypes.c_double, sp]
+        L.oc_rksplit_step.restype = ctypes.c_long
         L.oc_srmhd_max_speed.argtypes = [ctypes.c_double] * 6
         L.oc_srmhd_max_speed.restype = ctypes.c_double
         L.oc_max_wavespeed.argtypes = [cp, dp, dp, dp]
@@ -191,7 +193,7 @@ class Oracle:
 
     def step(self, cons, prims, aux, dt: float, integrator: str = "rk2"):
         st = OcStats()
-        fn = self.L.oc_rk2_step if integrator == "rk2" else self.L.oc_ssp2_step
+        fn = {"rk2": self.L.oc_rk2_step, "ssp2": self.L.oc_ssp2_step, "rksplit": self.L.oc_rksplit_step}[integrator]
         fails = fn(ctypes.byref(self.c), _p(cons), _p(prims), _p(aux), dt, ctypes.byref(st))
         return fails, st
 

@@ -14,6 +14,11 @@
 //                Model::getPrimitiveVars once (srmhd.cc:390-562,
 //                srrmhd.cc:314-356); dumps prims and aux.
 //
+// Options: integrator=rk2 | ssp2 | rksplit (RK2, SSP2, RKSplit); init=file
+// (prims_in=...) or init=kh (the reference's KHInstabilitySingleFluid with
+// mag=..., initFunc.cc:512-563); save=1 calls SerialSaveData::saveAll
+// (serialSaveData.cc:8-31) after the steps, writing under ./Data/Final.
+//
 // Boundary conditions: bcx/bcy/bcz = periodic | outflow. When the triple matches
 // one of the reference classes it is used as is (Periodic, Outflow, Flow,
 // boundaryConds.cc:5-145,449-730); other triples (periodic x and z with outflow
@@ -31,12 +36,15 @@
 #include "boundaryConds.h"
 #include "RK2.h"
 #include "SSP2.h"
+#include "rkSplit.h"
+#include "serialSaveData.h"
 #include "fluxVectorSplitting.h"
 #include "serialEnv.h"
 #include "weno.h"
 #include "cminpack.h"
 
 #include <chrono>
+#include <sys/stat.h>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -270,15 +278,22 @@ int main(int argc, char *argv[]) {
     bcs = new AxisBcs(&data, px, py, pz);
 
   Simulation sim(&data, &env);
-  FromFile init(&data, arg("prims_in", ""));
+  InitialFunc *init = nullptr;
+  if (arg("init", "file") == "kh")
+    init = new KHInstabilitySingleFluid(&data, argi("mag", 1));
+  else
+    init = new FromFile(&data, arg("prims_in", ""));
 
   TimeIntegrator *ti = nullptr;
-  if (arg("integrator", "rk2") == "rk2")
+  const std::string integ = arg("integrator", "rk2");
+  if (integ == "rk2")
     ti = new RK2(&data, model, bcs, &fvs);
+  else if (integ == "rksplit")
+    ti = new RKSplit(&data, model, bcs, &fvs);
   else
     ti = new SSP2(&data, model, bcs, &fvs);
 
-  sim.set(&init, model, ti, bcs, &fvs);
+  sim.set(init, model, ti, bcs, &fvs);
 
   if (mode == "fnet") {
     std::vector<double> fnet(ntot * data.Ncons, 0.0);
@@ -304,6 +319,13 @@ int main(int argc, char *argv[]) {
   }
   fprintf(log, "c2p_calls %ld c2p_cells %ld c2p_fail %ld\n", g_c2p_calls, g_c2p_cells, g_c2p_fail);
   fclose(log);
+  if (argi("save", 0)) {
+    const char *dirs[] = {"Data", "Data/Final", "Data/Final/Conserved", "Data/Final/Primitive",
+                          "Data/Final/Auxiliary", "Data/Final/Domain", "Data/Final/Constants"};
+    for (const char *dname : dirs) mkdir(dname, 0755);
+    SerialSaveData save(&data, &env);
+    save.saveAll();
+  }
   printf("done steps=%d c2p_fail=%ld step_seconds=%.6f\n", steps, g_c2p_fail, step_seconds);
   return 0;
 }

@@ -138,6 +138,43 @@ def c2p_vectors(name, n=2000, seed=7):
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), reference hybrd failures: {fail}")
 
 
+def kh_divb_prims(g: idt.Grid):
+    """KH shear layers plus Bx = 0.05 sin(2 pi x), whose discrete divergence
+    feeds phi, so RKSplit's phi source term is exercised."""
+    p = idt.kh_prims(g, seed=77, shear=0.5, bz=0.1)
+    x, _, _ = g.centres()
+    p[5] = 0.05 * np.sin(2.0 * np.pi * x)[:, None, None]
+    return p
+
+
+def kh_example_save(name, n=32, steps=5):
+    """The reference's own Kelvin-Helmholtz example set-up
+    (Examples/KelvinHelmholtz/SingleFluidIdealSerial/main.cc: KHInstabilitySingleFluid
+    with mag = 1, SRMHD, WENO3, FVS, Periodic, RKSplit, cfl 0.6, sigma 10, cp 1,
+    gamma 4/3) at n x n for `steps` updateTime calls, then SerialSaveData::saveAll.
+    Stores the state after set-up and the five text files saveAll writes
+    after set-up (text0_*) and after the steps (text_*)."""
+    g = idt.Grid(n, n, 0, -0.5, 0.5, -1.0, 1.0, 0.0, 1.0)
+    files = {}
+    base = _grid_args(g, 4.0 / 3.0, 0.6, 3, "rksplit", ("periodic", "periodic", "periodic"), "srmhd", sigma=10.0,
+                      cp=1.0)
+    for tag, nsteps in (("text0_", 0), ("text_", steps)):
+        with tempfile.TemporaryDirectory() as td:
+            out = os.path.join(td, "out")
+            _run(base + ["mode=run", "init=kh", "mag=1", "endTime=3.0", f"steps={nsteps}", f"out={out}", "dump=0",
+                         "save=1"], td)
+            if nsteps == 0:
+                c0, p0, a0 = _load_state(out + "_step0.bin", "srmhd", g.shape)
+            for key, rel in (("cons", "Conserved/cons.dat"), ("prims", "Primitive/prims.dat"),
+                             ("aux", "Auxiliary/aux.dat"), ("domain", "Domain/domain.dat"),
+                             ("constants", "Constants/constants.dat")):
+                with open(os.path.join(td, "Data", "Final", rel), "rb") as f:
+                    files[tag + key] = np.frombuffer(f.read(), dtype=np.uint8)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, prims0=p0, cons0=c0, aux0=a0, meta=np.array([n, steps]), **files)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
 def main():
     if not os.path.exists(REF_BIN):
         raise SystemExit(f"{REF_BIN} missing: build it with `make -C oracle/ref`")
@@ -155,6 +192,12 @@ def main():
     run_case("ot2d_32_weno3_ssp2", go, idt.ot_prims_resistive(go), "srrmhd", 3, "ssp2",
              ("periodic", "periodic", "periodic"), 3, [1, 3], cfl=0.5, sigma=1e3, cp=0.1, with_fnet=True)
     c2p_vectors("srmhd_c2p_vectors")
+    # RKSplit (rkSplit.cc:31-53): KH 48x48 with a divergent Bx so phi's source
+    # acts, cp 0.1, WENO3, periodic x / outflow y, 8 steps.
+    gs = idt.kh2d_grid(48)
+    run_case("kh2d_48_weno3_rksplit", gs, kh_divb_prims(gs), "srmhd", 3, "rksplit",
+             ("periodic", "outflow", "periodic"), 8, [8], cp=0.1)
+    kh_example_save("kh_example_32_rksplit_save")
 
 
 if __name__ == "__main__":

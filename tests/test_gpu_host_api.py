@@ -48,3 +48,37 @@ def test_host_initial_data_matches_numpy_generator(tmp_path):
     ref = idt.kh_prims(idt.kh2d_grid(64), seed=1234)
     inner = (slice(None), slice(4, -4), slice(4, -4), slice(None))
     assert np.max(np.abs(p[inner] - ref[inner])) <= 1e-15
+
+
+EXAMPLE = os.path.join(ROOT, "method_amd", "lib", "kh_example_main")
+
+
+def _parse_field(text: bytes, nvar: int, nx: int, ny: int):
+    lines = text.decode().split("\n")
+    vals = np.array([float(v) for line in lines[1:] for v in line.split()])
+    return lines[0], vals.reshape((nvar, nx, ny))
+
+
+def test_reference_kh_example_output_matches_reference(golden, tmp_path):
+    """The reference's KH example set-up (KHInstabilitySingleFluid, SRMHD, WENO3,
+    Periodic, RKSplit, SerialSaveData) run through the host API for 5
+    updateTime calls: the text files of saveAll against the reference build's.
+    Headers, domain and constants (t and dt included) are identical; every
+    field value is within 1e-9 of its field's scale."""
+    z = golden("kh_example_32_rksplit_save")
+    n, steps = (int(v) for v in z["meta"])
+    print(f"run: {EXAMPLE} {n} {steps}", flush=True)
+    r = subprocess.run([EXAMPLE, str(n), str(steps)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    final = tmp_path / "Data" / "Final"
+    for key, rel in (("domain", "Domain/domain.dat"), ("constants", "Constants/constants.dat")):
+        assert (final / rel).read_bytes() == bytes(z["text_" + key]), rel
+    for key, rel, nvar in (("cons", "Conserved/cons.dat", 9), ("prims", "Primitive/prims.dat", 8),
+                           ("aux", "Auxiliary/aux.dat", 13)):
+        h_got, got = _parse_field((final / rel).read_bytes(), nvar, n, n)
+        h_ref, ref = _parse_field(bytes(z["text_" + key]), nvar, n, n)
+        assert h_got == h_ref
+        for v in range(nvar):
+            scale = max(np.abs(ref[v]).max(), 1e-300)
+            err = np.abs(got[v] - ref[v]).max() / scale
+            assert err <= 1e-9 or (scale == 1e-300 and np.abs(got[v]).max() == 0.0), f"{key}[{v}]: {err:.3e}"

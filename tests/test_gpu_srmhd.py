@@ -291,3 +291,22 @@ def test_fused_yz_sweep_equals_separate_sweeps_bitwise(n, weno, bc):
             out.append((f0,) + e.download())
     for a, b, name in zip(out[0], out[1], ("fnet", "cons", "prims", "aux")):
         assert np.array_equal(a, b), f"{name}: max diff {np.max(np.abs(a - b)):.3e}"
+
+
+def test_rksplit_steps_match_reference(golden):
+    """RKSplit on the GPU (RK2, split phi source, cons2prim) against the
+    reference build: KH 48^2 with a divergent Bx, cp 0.1, 8 steps."""
+    z = golden("kh2d_48_weno3_rksplit")
+    grid = dict(nx=48, ny=48, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+    say("gpu: RKSplit KH 48^2 8 steps vs reference")
+    with eng.Engine(eng.Config(weno_order=3, integrator=eng.RKSPLIT, cp=0.1, **grid)) as e:
+        e.set_prims(z["prims0"])
+        for s in range(8):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+            assert e.time()[1] == z["dts"][s]
+        c, p, a = e.download()
+    for got, ref, names in ((c, z["cons8"], parity.SRMHD_CONS), (p, z["prims8"], parity.SRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        say(parity.report("gpu vs reference, RKSplit KH 48^2, 8 steps", rows))
+        check_rows(rows, 1e-9, use_group=True)

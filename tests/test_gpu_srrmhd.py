@@ -165,3 +165,25 @@ def test_fused_yz_sweep_equals_separate_sweeps_bitwise():
             out.append(e.download())
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_rksplit_matches_oracle():
+    """RKSplit with the resistive model (explicit split source, sigma 10)."""
+    g = idt.ot2d_grid(24)
+    p0 = idt.ot_prims_resistive(g)
+    grid = dict(nx=24, ny=24, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0)
+    say("gpu: SRRMHD RKSplit OT 24^2 vs oracle")
+    o = orc.Oracle(orc.Case(model=orc.SRRMHD, weno_order=3, periodic=(1, 1, 1), quirks=1, sigma=10.0, cp=0.1,
+                            cfl=0.5, **grid))
+    oc, op, oa, _, _ = o.run(p0, 3, integrator="rksplit")
+    with eng.Engine(eng.Config(model=eng.SRRMHD, integrator=eng.RKSPLIT, weno_order=3, bc=(eng.PERIODIC,) * 3,
+                               quirks=1, sigma=10.0, cp=0.1, cfl=0.5, **grid)) as e:
+        e.set_prims(p0)
+        for _ in range(3):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+        c, p, a = e.download()
+    for got, ref, names in ((c, oc, parity.SRRMHD_CONS), (p, op, parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        say(parity.report("gpu vs oracle SRRMHD RKSplit", rows))
+        check_rows(rows, 1e-9)

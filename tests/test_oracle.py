@@ -309,3 +309,19 @@ def test_max_wavespeed_grid_reduction():
                 zip(prims[0][I].ravel(), prims[1 + d][I].ravel(), aux[9][I].ravel(), aux[0][I].ravel(),
                     aux[3][I].ravel(), aux[8][I].ravel())) for d in range(2)]
     assert got[0] == want[0] and got[1] == want[1] and got[2] == 0.0
+
+
+def test_rksplit_steps_match_reference(golden):
+    """RKSplit (rkSplit.cc:31-53) on KH 48^2 with a divergent Bx, cp 0.1:
+    8 steps against the reference build; the split phi source must act (the
+    RK2 run differs in phi far beyond the tolerance)."""
+    z = golden("kh2d_48_weno3_rksplit")
+    grid = dict(nx=48, ny=48, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+    o = orc.Oracle(orc.Case(weno_order=3, periodic=(1, 0, 1), cp=0.1, **grid))
+    oc, op, oa, dts, _ = o.run(z["prims0"], 8, integrator="rksplit")
+    assert np.array_equal(np.array(dts), z["dts"])
+    for got, ref, names in ((oc, z["cons8"], parity.SRMHD_CONS), (op, z["prims8"], parity.SRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        assert max(r[4] for r in rows) <= 1e-9
+    rc, _, _, _, _ = orc.Oracle(orc.Case(weno_order=3, periodic=(1, 0, 1), cp=0.1, **grid)).run(z["prims0"], 8)
+    assert np.abs(rc[8] - z["cons8"][8]).max() > 1e-5
