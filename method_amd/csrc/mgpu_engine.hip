@@ -169,6 +169,10 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
     HIP_OK(hipGetLastError());
     return 0;
   }
+  if (dims == 1 && last_mode != kSweepWrite && last_mode != kSweepAdd) {
+    // the x sweep is the only one: the earlier axes' net flux it adds to is zero
+    HIP_OK(hipMemsetAsync(e->fnet, 0, sizeof(double) * e->ncons * e->geo.vstride, e->stream));
+  }
   int first = 1;
   for (int dir = dims - 1; dir >= 0; --dir) {
     int mode = dir == 0 ? last_mode : (first ? kSweepWrite : kSweepAdd);
@@ -322,12 +326,15 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   if (c.Ng < shift + 1)  // WenoBase::checkSufficientGhostZones (weno.cc:13-20)
     return fail("You must increase number of boundary cells. (order %d needs Ng >= %d, have %d)", c.weno_order,
                 shift + 1, c.Ng);
-  if (c.nx < 1 || c.ny < 1 || c.nz < 0) return fail("mgpu_create: need nx >= 1, ny >= 1 (2-D or 3-D), nz >= 0");
+  if (c.nx < 1 || c.ny < 0 || c.nz < 0) return fail("mgpu_create: need nx >= 1, ny >= 0, nz >= 0");
   if (c.sigma < 0.0) return fail("Conductivity must be non-negative, sigma >= 0.");  // simData.cc:100-102
   for (int ax = 0; ax < 3; ++ax)
     if (c.bc[ax] != MGPU_BC_PERIODIC && c.bc[ax] != MGPU_BC_OUTFLOW) return fail("mgpu_create: bad bc on axis %d", ax);
+  // 1-D grids have ny = nz = 0, 2-D grids nz = 0 (Data::initData, simData.cc:74-97)
+  if (c.ny == 0 && c.nz > 0) return fail("mgpu_create: a grid with nz > 0 needs ny > 0");
+  const int n1 = c.ny > 0 ? c.ny : 1;
   const int n2 = c.nz > 0 ? c.nz : 1;
-  if (c.nx < c.Ng || c.ny < c.Ng || (c.nz > 0 && c.nz < c.Ng))
+  if (c.nx < c.Ng || (c.ny > 0 && c.ny < c.Ng) || (c.nz > 0 && c.nz < c.Ng))
     return fail("mgpu_create: every active axis needs at least Ng interior cells");
   HIP_OK(hipSetDevice(c.device));
 
@@ -339,27 +346,27 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   e->naux = res ? kSrrmhdAux : kSrmhdAux;
   Geo &g = e->geo;
   g.n[0] = c.nx;
-  g.n[1] = c.ny;
+  g.n[1] = n1;
   g.n[2] = n2;
   g.stride[2] = 1;
   g.stride[1] = n2;
-  g.stride[0] = (long long)c.ny * n2;
-  g.vstride = (long long)c.nx * c.ny * n2;
-  g.dims = c.nz > 0 ? 3 : 2;
+  g.stride[0] = (long long)n1 * n2;
+  g.vstride = (long long)c.nx * n1 * n2;
+  g.dims = c.nz > 0 ? 3 : (c.ny > 0 ? 2 : 1);
   for (int ax = 0; ax < 3; ++ax) g.periodic[ax] = c.bc[ax] == MGPU_BC_PERIODIC;
   // Data::initData / Simulation::Simulation (simData.cc:74-79, simulation.cc:34-36)
   e->dx = (c.xmax - c.xmin) / c.nx;
-  e->dy = (c.ymax - c.ymin) / c.ny;
+  e->dy = c.ny > 0 ? (c.ymax - c.ymin) / c.ny : INFINITY;
   e->dz = c.nz > 0 ? (c.zmax - c.zmin) / c.nz : INFINITY;
   g.inv_d[0] = 1.0 / e->dx;
-  g.inv_d[1] = 1.0 / e->dy;
+  g.inv_d[1] = c.ny > 0 ? 1.0 / e->dy : 0.0;
   g.inv_d[2] = c.nz > 0 ? 1.0 / e->dz : 0.0;
   PadGeo &p = e->pad;
   p.N[0] = c.nx + 2 * c.Ng;
-  p.N[1] = c.ny + 2 * c.Ng;
+  p.N[1] = c.ny > 0 ? c.ny + 2 * c.Ng : 1;
   p.N[2] = c.nz > 0 ? c.nz + 2 * c.Ng : 1;
   p.off[0] = c.Ng;
-  p.off[1] = c.Ng;
+  p.off[1] = c.ny > 0 ? c.Ng : 0;
   p.off[2] = c.nz > 0 ? c.Ng : 0;
   p.ntot = (long long)p.N[0] * p.N[1] * p.N[2];
 

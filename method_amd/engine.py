@@ -200,11 +200,12 @@ class Config:
 
     @property
     def shape(self):
-        return (self.nx + 2 * self.Ng, self.ny + 2 * self.Ng, self.nz + 2 * self.Ng if self.nz > 0 else 1)
+        return (self.nx + 2 * self.Ng, self.ny + 2 * self.Ng if self.ny > 0 else 1,
+                self.nz + 2 * self.Ng if self.nz > 0 else 1)
 
     @property
     def interior_cells(self) -> int:
-        return self.nx * self.ny * max(self.nz, 1)
+        return self.nx * max(self.ny, 1) * max(self.nz, 1)
 
 
 class Engine:

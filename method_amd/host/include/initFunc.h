@@ -41,6 +41,32 @@ class KHInstabilitySingleFluid : public InitialFunc {
   KHInstabilitySingleFluid(Data *data, int mag = 0);
 };
 
+// Orszag-Tang vortex (initFunc.cc:229-271) on [0,1]^2: rho = 25/(36 pi),
+// p = 5/(12 pi), v = 0.5 (-sin 2 pi y, sin 2 pi x), B = (-sin 2 pi y, sin 4 pi x)
+// / sqrt(4 pi); in 3-D the reference's form with a z modulation.
+class OTVortexSingleFluid : public InitialFunc {
+ public:
+  OTVortexSingleFluid(Data *data);
+};
+
+// Brio-Wu shock tube along axis dir (initFunc.cc:373-510): left state
+// rho = p = 1, right state rho = 0.125, p = 0.1, with the transverse field
+// +-0.5 (By for dir 0, Bz for dir 1, Bx for dir 2).
+class BrioWuSingleFluid : public InitialFunc {
+ public:
+  BrioWuSingleFluid(Data *data, int dir = 0);
+};
+
+// Kelvin-Helmholtz with two interfaces at y = 0.25 and 0.75 displaced by ten
+// random Fourier modes (initFunc.cc:612-680): rho = 1 / 0.1 and vx = +-0.5
+// outside / between them, p = 1, Bz = 0.1 when mag is non-zero. The mode
+// amplitudes and phases are drawn with std::mt19937(seed) and
+// std::uniform_real_distribution<>(0, 1) in the reference's order.
+class KHRandomInstabilitySingleFluid : public InitialFunc {
+ public:
+  KHRandomInstabilitySingleFluid(Data *data, int mag = 0, int seed = 10);
+};
+
 // Uniform draw in [0,1) of the counter stream used by KHShearLayers.
 double initUniform01(uint64_t seed, uint64_t index, int stream);
 

@@ -1,0 +1,75 @@
+// Resistive SRMHD model of the host API (srrmhd.cc:24-581): the full-grid
+// methods upload the given host arrays, run the engine's kernels and copy back.
+#include "srrmhd.h"
+
+#include <cstdio>
+#include <stdexcept>
+
+#include "gpu_link.h"
+
+using method_host::check;
+using method_host::engine;
+
+SRRMHD::SRRMHD() : Model() {
+  Ncons = 14;
+  Nprims = 11;
+  Naux = 17;
+}
+
+SRRMHD::SRRMHD(Data *data) : Model(data) {
+  Ncons = data->Ncons = 14;
+  Nprims = data->Nprims = 11;
+  Naux = data->Naux = 17;
+  data->gpu.model = MGPU_SRRMHD;
+  const char *c[] = {"D", "Sx", "Sy", "Sz", "tau", "Bx", "By", "Bz", "Ex", "Ey", "Ez", "psi", "phi", "qch"};
+  const char *p[] = {"rho", "vx", "vy", "vz", "p", "Bx", "By", "Bz", "Ex", "Ey", "Ez"};
+  const char *a[] = {"h",   "W",       "e",  "c",     "Jx",    "Jy",    "Jz",     "Bsq",   "Esq",
+                     "vsq", "rhohWsq", "vE", "Sbarx", "Sbary", "Sbarz", "Sbarsq", "tauBar"};
+  for (auto s : c) data->consLabels.push_back(s);
+  for (auto s : p) data->primsLabels.push_back(s);
+  for (auto s : a) data->auxLabels.push_back(s);
+}
+
+void SRRMHD::fluxVector(double *cons, double *prims, double *aux, double *f, const int dir) {
+  (void)aux;
+  mgpu_engine *e = engine(data);
+  check(mgpu_upload_state(e, cons, prims, nullptr), "SRRMHD::fluxVector");
+  check(mgpu_flux_vector(e, dir, f), "SRRMHD::fluxVector");
+  data->gpu.deviceCurrent = false;
+}
+
+void SRRMHD::sourceTermSingleCell(double *, double *, double *, double *, int, int, int) {
+  throw std::logic_error("SRRMHD::sourceTermSingleCell: single-cell model calls run inside the engine's kernels");
+}
+
+// The stiff source needs the current J, formed from cons, prims and sigma.
+void SRRMHD::sourceTerm(double *cons, double *prims, double *aux, double *source) {
+  (void)aux;
+  mgpu_engine *e = engine(data);
+  check(mgpu_upload_state(e, cons, prims, nullptr), "SRRMHD::sourceTerm");
+  check(mgpu_source(e, source), "SRRMHD::sourceTerm");
+  data->gpu.deviceCurrent = false;
+}
+
+void SRRMHD::getPrimitiveVarsSingleCell(double *, double *, double *, int, int, int) {
+  throw std::logic_error(
+      "SRRMHD::getPrimitiveVarsSingleCell: single-cell model calls run inside the engine's kernels");
+}
+
+void SRRMHD::getPrimitiveVars(double *cons, double *prims, double *aux) {
+  mgpu_engine *e = engine(data);
+  check(mgpu_upload_state(e, cons, prims, aux), "SRRMHD::getPrimitiveVars");
+  mgpu_stats st;
+  const int rc = check(mgpu_c2p(e, &st), "SRRMHD::getPrimitiveVars");
+  if (rc > 0) printf("C2P did not converge for %d cells\n", rc);
+  check(mgpu_download_state(e, nullptr, prims, aux), "SRRMHD::getPrimitiveVars");
+  data->gpu.deviceCurrent = false;
+}
+
+void SRRMHD::primsToAll(double *cons, double *prims, double *aux) {
+  mgpu_engine *e = engine(data);
+  check(mgpu_upload_state(e, nullptr, prims, nullptr), "SRRMHD::primsToAll");
+  check(mgpu_prims_to_all(e), "SRRMHD::primsToAll");
+  check(mgpu_download_state(e, cons, nullptr, aux), "SRRMHD::primsToAll");
+  data->gpu.deviceCurrent = false;
+}

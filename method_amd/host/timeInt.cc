@@ -4,6 +4,7 @@
 
 #include "RK2.h"
 #include "rkSplit.h"
+#include "SSP2.h"
 #include "gpu_link.h"
 
 using method_host::check;
@@ -25,26 +26,41 @@ RK2::RK2(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtens
   data->gpu.integrator = MGPU_RK2;
 }
 
-// Both stages run inside the engine on the device-resident state. The host
-// arrays are uploaded first unless the device already holds them, and copied
-// back afterwards unless the caller (Simulation::evolve) defers that.
-void RK2::step(double *cons, double *prims, double *aux, double dt) {
-  Data *d = data;
+namespace {
+
+// One engine step of whichever integrator the engine was created with, on the
+// device-resident state. The host arrays are uploaded first unless the device
+// already holds them, and copied back afterwards unless the caller
+// (Simulation::evolve) defers that.
+void engine_step(TimeIntegrator *ti, double *cons, double *prims, double *aux, double dt, const char *what) {
+  Data *d = ti->data;
   if (dt <= 0) dt = d->dt;
   mgpu_engine *e = engine(d);
   const bool own = cons == d->cons && prims == d->prims && aux == d->aux;
-  if (!(own && d->gpu.deviceCurrent)) check(mgpu_upload_state(e, cons, prims, aux), "RK2::step");
+  if (!(own && d->gpu.deviceCurrent)) check(mgpu_upload_state(e, cons, prims, aux), what);
   mgpu_stats st;
-  const int rc = check(mgpu_step(e, dt, &st), "RK2::step");
+  const int rc = check(mgpu_step(e, dt, &st), what);
   if (rc > 0) printf("Smart guessing did not work for %d cells\n", rc);
   if (own && !d->gpu.mirrorEachStep) {
     d->gpu.deviceCurrent = true;
     d->gpu.hostStale = true;
     return;
   }
-  check(mgpu_download_state(e, cons, prims, aux), "RK2::step");
+  check(mgpu_download_state(e, cons, prims, aux), what);
   d->gpu.deviceCurrent = own;
   d->gpu.hostStale = false;
+}
+
+}  // namespace
+
+void RK2::step(double *cons, double *prims, double *aux, double dt) { engine_step(this, cons, prims, aux, dt, "RK2::step"); }
+
+SSP2::SSP2(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod) : TimeIntegrator(data, model, bcs, fluxMethod) {
+  data->gpu.integrator = MGPU_SSP2;
+}
+
+void SSP2::step(double *cons, double *prims, double *aux, double dt) {
+  engine_step(this, cons, prims, aux, dt, "SSP2::step");
 }
 
 RKSplit::RKSplit(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)

@@ -15,8 +15,10 @@
 //                srrmhd.cc:314-356); dumps prims and aux.
 //
 // Options: integrator=rk2 | ssp2 | rksplit (RK2, SSP2, RKSplit); init=file
-// (prims_in=...) or init=kh (the reference's KHInstabilitySingleFluid with
-// mag=..., initFunc.cc:512-563); save=1 calls SerialSaveData::saveAll
+// (prims_in=...) or one of the reference's initial states: kh
+// (KHInstabilitySingleFluid, mag=...), khrandom (KHRandomInstabilitySingleFluid,
+// mag=..., seed=...), ot (OTVortexSingleFluid), briowu (BrioWuSingleFluid,
+// dir=...) (initFunc.cc); save=1 calls SerialSaveData::saveAll
 // (serialSaveData.cc:8-31) after the steps, writing under ./Data/Final.
 //
 // Boundary conditions: bcx/bcy/bcz = periodic | outflow. When the triple matches
@@ -279,8 +281,15 @@ int main(int argc, char *argv[]) {
 
   Simulation sim(&data, &env);
   InitialFunc *init = nullptr;
-  if (arg("init", "file") == "kh")
+  const std::string init_name = arg("init", "file");
+  if (init_name == "kh")
     init = new KHInstabilitySingleFluid(&data, argi("mag", 1));
+  else if (init_name == "khrandom")
+    init = new KHRandomInstabilitySingleFluid(&data, argi("mag", 1), argi("seed", 10));
+  else if (init_name == "ot")
+    init = new OTVortexSingleFluid(&data);
+  else if (init_name == "briowu")
+    init = new BrioWuSingleFluid(&data, argi("dir", 0));
   else
     init = new FromFile(&data, arg("prims_in", ""));
 

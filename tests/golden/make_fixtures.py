@@ -175,6 +175,50 @@ def kh_example_save(name, n=32, steps=5):
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
 
 
+def init_case(name, g, init_args, model, integrator, bc, steps, gamma, cfl, sigma=0.0, cp=0.1, weno=3):
+    """A reference example's set-up: one of the reference's own initial states
+    (init=...), run for `steps` updateTime calls. Stores the state after
+    Simulation::set (step 0) and after the steps."""
+    shape = g.shape
+    with tempfile.TemporaryDirectory() as td:
+        base = _grid_args(g, gamma, cfl, weno, integrator, bc, model, sigma, cp)
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", f"steps={steps}", f"out={out}", f"dump=0,{steps}"] + init_args, td)
+        log = open(out + "_log.txt").read().split("\n")
+        dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+        fail = [int(l.split()[-1]) for l in log if l.startswith("c2p_calls")]
+        rec = {"dts": np.array(dts), "meta": np.array([g.nx, g.ny, g.nz, g.Ng, weno, steps]),
+               "bounds": np.array([g.xmin, g.xmax, g.ymin, g.ymax, g.zmin, g.zmax]),
+               "params": np.array([gamma, cfl, sigma, cp]),
+               "periodic": np.array([1 if b == "periodic" else 0 for b in bc]), "c2p_fail": np.array(fail)}
+        for st in (0, steps):
+            c, p, a = _load_state(f"{out}_step{st}.bin", model, shape)
+            rec[f"cons{st}"], rec[f"prims{st}"], rec[f"aux{st}"] = c, p, a
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **rec)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), reference C2P failures: {fail}")
+
+
+def reference_examples():
+    """The reference's serial examples (Examples/*/main.cc) at their own
+    parameters, for a fixed number of steps."""
+    g1 = idt.Grid(100, 0, 0, 0.0, 1.0, -1.0, 1.0, 0.0, 1.0)
+    out3 = ("outflow", "outflow", "outflow")
+    per3 = ("periodic", "periodic", "periodic")
+    # BrioWuShockTube/SingleFluidIdealSerial: SRMHD, gamma 2, cfl 0.4, Outflow, RKSplit
+    init_case("briowu_100_rksplit", g1, ["init=briowu", "endTime=0.4"], "srmhd", "rksplit", out3, 60, 2.0, 0.4)
+    # BrioWuShockTube/SingleFluidResistiveSerial: SRRMHD, gamma 2, sigma 10, Outflow, SSP2
+    init_case("briowu_100_ssp2", g1, ["init=briowu", "endTime=0.4"], "srrmhd", "ssp2", out3, 60, 2.0, 0.4,
+              sigma=10.0)
+    # OrszagTangVortex/SingleFluidIdealSerial at 64^2: gamma 5/3, cfl 0.6, sigma 10, cp 1, Periodic, RKSplit
+    g2 = idt.Grid(64, 64, 0, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    init_case("ot_64_rksplit", g2, ["init=ot", "endTime=1.0"], "srmhd", "rksplit", per3, 10, 5.0 / 3.0, 0.6,
+              sigma=10.0, cp=1.0)
+    # KelvinHelmholtz/SingleFluidIdealRandomSerial at 64^2, seed 10
+    init_case("khrandom_64_rksplit", g2, ["init=khrandom", "mag=1", "seed=10", "endTime=3.0"], "srmhd", "rksplit",
+              per3, 10, 4.0 / 3.0, 0.6, sigma=10.0, cp=1.0)
+
+
 def main():
     if not os.path.exists(REF_BIN):
         raise SystemExit(f"{REF_BIN} missing: build it with `make -C oracle/ref`")
@@ -198,6 +242,7 @@ def main():
     run_case("kh2d_48_weno3_rksplit", gs, kh_divb_prims(gs), "srmhd", 3, "rksplit",
              ("periodic", "outflow", "periodic"), 8, [8], cp=0.1)
     kh_example_save("kh_example_32_rksplit_save")
+    reference_examples()
 
 
 if __name__ == "__main__":

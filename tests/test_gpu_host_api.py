@@ -82,3 +82,25 @@ def test_reference_kh_example_output_matches_reference(golden, tmp_path):
             scale = max(np.abs(ref[v]).max(), 1e-300)
             err = np.abs(got[v] - ref[v]).max() / scale
             assert err <= 1e-9 or (scale == 1e-300 and np.abs(got[v]).max() == 0.0), f"{key}[{v}]: {err:.3e}"
+
+
+def test_host_api_srrmhd_ssp2_matches_reference(golden, tmp_path):
+    """SRRMHD + SSP2 through the host API (OT 32^2, sigma 1e3, 3 steps) against
+    the reference build."""
+    z = golden("ot2d_32_weno3_ssp2")
+    exe = os.path.join(ROOT, "method_amd", "lib", "ot2d_ssp2_main")
+    pin = tmp_path / "prims0.bin"
+    np.ascontiguousarray(z["prims0"]).tofile(pin)
+    out = tmp_path / "state.bin"
+    print(f"run: {exe} {out} 3 {pin}", flush=True)
+    r = subprocess.run([exe, str(out), "3", str(pin)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shape = (40, 40, 1)
+    n = 40 * 40
+    a = np.fromfile(out)
+    c, p = a[:14 * n].reshape((14,) + shape), a[14 * n:25 * n].reshape((11,) + shape)
+    for got, ref, names in ((c, z["cons3"], parity.SRRMHD_CONS), (p, z["prims3"], parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        print(parity.report("host API SRRMHD SSP2 vs reference, OT 32^2, 3 steps", rows), flush=True)
+        for n_, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-9, f"{n_}: {rg:.3e}"

@@ -325,3 +325,34 @@ def test_rksplit_steps_match_reference(golden):
         assert max(r[4] for r in rows) <= 1e-9
     rc, _, _, _, _ = orc.Oracle(orc.Case(weno_order=3, periodic=(1, 0, 1), cp=0.1, **grid)).run(z["prims0"], 8)
     assert np.abs(rc[8] - z["cons8"][8]).max() > 1e-5
+
+
+EXAMPLE_FIXTURES = [("briowu_100_rksplit", orc.SRMHD, "rksplit"), ("briowu_100_ssp2", orc.SRRMHD, "ssp2"),
+                    ("ot_64_rksplit", orc.SRMHD, "rksplit"), ("khrandom_64_rksplit", orc.SRMHD, "rksplit")]
+
+
+def example_case(z, model):
+    nx, ny, nz, Ng, weno, steps = (int(v) for v in z["meta"])
+    b = z["bounds"]
+    gam, cfl, sig, cp = (float(v) for v in z["params"])
+    grid = dict(nx=nx, ny=ny, nz=nz, xmin=b[0], xmax=b[1], ymin=b[2], ymax=b[3], zmin=b[4], zmax=b[5])
+    per = tuple(int(v) for v in z["periodic"])
+    return orc.Case(model=model, weno_order=weno, periodic=per, gamma=gam, cfl=cfl, sigma=sig, cp=cp, **grid), steps
+
+
+@pytest.mark.parametrize("name, model, integrator", EXAMPLE_FIXTURES)
+def test_reference_examples_match_reference(golden, name, model, integrator):
+    """The reference's serial examples (1-D Brio-Wu ideal and resistive,
+    Orszag-Tang, random KH) at their own parameters, from the reference's
+    post-set state, against the reference build."""
+    z = golden(name)
+    case, steps = example_case(z, model)
+    o = orc.Oracle(case)
+    oc, op, oa, dts, st = o.run(z["prims0"], steps, integrator=integrator)
+    assert np.array_equal(np.array(dts), z["dts"]) and st.failures == 0
+    dims = 1 if case.ny == 0 else 2
+    cn = parity.SRMHD_CONS if model == orc.SRMHD else parity.SRRMHD_CONS
+    pn = parity.SRMHD_PRIMS if model == orc.SRMHD else parity.SRRMHD_PRIMS
+    for got, ref, names in ((oc, z[f"cons{steps}"], cn), (op, z[f"prims{steps}"], pn)):
+        rows = parity.field_errors(parity.interior(got, 4, dims), parity.interior(ref, 4, dims), names)
+        assert max(r[4] for r in rows) <= 1e-9
