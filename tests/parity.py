@@ -50,3 +50,28 @@ def report(title, rows):
     for n, dm, fm, rf, rg, pw in rows:
         lines.append(f"{n:>6} {dm:11.3e} {fm:11.3e} {rf:11.3e} {rg:11.3e} {pw:11.3e}")
     return "\n".join(lines)
+
+
+# A 16 x 16 KH state whose stored primitives at three interior cells are
+# replaced by a guess of |v| = 1, (vx, vy) = (1, 0): the starting root guess
+# Z = rho h / (1 - v^2) (srmhd.cc:438-442) is infinite, the Newton solve fails,
+# and those cells take the neighbour-mean retry of SRMHD::getPrimitiveVars
+# (srmhd.cc:465-507). Indices are into the padded
+# (Ng = 4) arrays; no two bad cells are face neighbours and none is the last
+# interior cell.
+C2P_RETRY_GRID = dict(nx=16, ny=16, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+C2P_RETRY_CELLS = ((6, 7), (9, 12), (13, 5))
+
+
+def c2p_retry_case(oracle):
+    """(cons, true prims, true aux, guess prims, guess aux) for the retry test,
+    built with the oracle's Simulation::set."""
+    from method_amd import initdata as idt
+
+    p0 = idt.kh_prims(idt.kh2d_grid(16), seed=5, shear=0.7, bz=0.5)
+    cons, prims, aux = oracle.set(p0)
+    gp, ga = prims.copy(), aux.copy()
+    for i, j in C2P_RETRY_CELLS:
+        gp[1, i, j, 0] = 1.0
+        gp[2, i, j, 0] = 0.0
+    return cons, prims, aux, gp, ga

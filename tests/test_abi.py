@@ -38,7 +38,8 @@ def _make(**kw):
     (dict(weno_order=5, Ng=3), "You must increase number of boundary cells."),  # weno.cc:13-20
     (dict(sigma=-1.0), "Conductivity must be non-negative"),                   # simData.cc:100-102
     (dict(weno_order=7), "weno_order must be 3 or 5"),
-    (dict(ny=0), "need nx >= 1, ny >= 1"),
+    (dict(nx=0), "need nx >= 1, ny >= 0, nz >= 0"),
+    (dict(ny=0, nz=8), "a grid with nz > 0 needs ny > 0"),
 ])
 def test_create_rejects_bad_configuration(kw, msg):
     with pytest.raises(eng.EngineError, match=re.escape(msg)):

@@ -310,3 +310,31 @@ def test_rksplit_steps_match_reference(golden):
         rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
         say(parity.report("gpu vs reference, RKSplit KH 48^2, 8 steps", rows))
         check_rows(rows, 1e-9, use_group=True)
+
+
+@pytest.mark.parametrize("quirks", [0, 1])
+def test_c2p_retry_matches_oracle(quirks):
+    """Three cells whose Newton solve fails take the neighbour-mean retry
+    (srmhd.cc:465-507): the failure count, and with the quirk on the reference's
+    use of the last cell's arguments, must match the oracle. The retry starts
+    from the neighbours' roots summed in another order than the oracle's, so the
+    agreement is to the Newton tolerance, not bitwise."""
+    say(f"gpu: c2p retry quirks={quirks}")
+    o = oracle_for(weno=3, quirks=quirks, **parity.C2P_RETRY_GRID)
+    cons, prims, aux, gp, ga = parity.c2p_retry_case(o)
+    op, oa = gp.copy(), ga.copy()
+    fails, _ = o.c2p(cons, op, oa)
+    with engine(weno=3, quirks=quirks, **parity.C2P_RETRY_GRID) as e:
+        e.upload(cons=cons, prims=gp, aux=ga)
+        rc, st = e.c2p()
+        _, p, a = e.download()
+    say(f"  failures {st['failures']} retry failures {st['fallbacks']}")
+    assert fails == len(parity.C2P_RETRY_CELLS)
+    assert rc == fails and st["failures"] == fails and st["fallbacks"] == 0
+    for got, ref, names in ((p, op, parity.SRMHD_PRIMS), (a, oa, parity.SRMHD_AUX)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        say(parity.report(f"c2p retry quirks={quirks}", rows))
+        check_rows(rows, 1e-12)
+    if not quirks:
+        sl = (slice(None), slice(4, -4), slice(4, -4), 0)
+        assert np.allclose(p[sl], prims[sl], rtol=1e-12, atol=1e-14)

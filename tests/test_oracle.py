@@ -356,3 +356,34 @@ def test_reference_examples_match_reference(golden, name, model, integrator):
     for got, ref, names in ((oc, z[f"cons{steps}"], cn), (op, z[f"prims{steps}"], pn)):
         rows = parity.field_errors(parity.interior(got, 4, dims), parity.interior(ref, 4, dims), names)
         assert max(r[4] for r in rows) <= 1e-9
+
+
+@pytest.mark.parametrize("quirks", [0, 1])
+def test_c2p_retry_from_neighbour_means(quirks):
+    """Cells whose Newton solve fails are solved again from the mean root of
+    their converged face neighbours (srmhd.cc:465-507). Without the quirk the
+    retry recovers the state that made the conserved fields; with it the retry
+    solves the last interior cell's equations (the reference passes that
+    cell's arguments), so a retried cell takes the last cell's root: its
+    Lorentz factor equals the last cell's."""
+    o = orc.Oracle(orc.Case(weno_order=3, periodic=(1, 0, 1), quirks=quirks, **parity.C2P_RETRY_GRID))
+    cons, prims, aux, gp, ga = parity.c2p_retry_case(o)
+    fails, st = o.c2p(cons, gp, ga)
+    assert fails == len(parity.C2P_RETRY_CELLS) and st.failures == fails
+    bad = tuple(np.array(parity.C2P_RETRY_CELLS).T)
+    if not quirks:
+        sl = (slice(None), slice(4, -4), slice(4, -4), 0)
+        scale = np.abs(prims[sl]).max(axis=(1, 2), keepdims=True)
+        assert (np.abs(gp[sl] - prims[sl]) / np.maximum(scale, 1e-300)).max() < 1e-12
+        assert np.allclose(ga[sl], aux[sl], rtol=1e-12, atol=1e-14)
+    else:
+        W_last = aux[1, 4 + 15, 4 + 15, 0]
+        assert np.allclose(ga[1][bad + (0,)], W_last, rtol=1e-12, atol=0)
+        # away from the bad cells the recovery is untouched
+        mask = np.ones((16, 16), bool)
+        for i, j in parity.C2P_RETRY_CELLS:
+            mask[i - 4, j - 4] = False
+        got, want = gp[:, 4:-4, 4:-4, 0][:, mask], prims[:, 4:-4, 4:-4, 0][:, mask]
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-14)
+        # and the bad cells are not recovered (the quirk is visible)
+        assert np.abs(gp[1][bad + (0,)] - prims[1][bad + (0,)]).max() > 1e-3
