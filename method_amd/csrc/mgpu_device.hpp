@@ -31,59 +31,66 @@ __device__ __forceinline__ int remap(int i, int n, int periodic) {
 // Jiang-Shu WENO with nonlinear weights alpha_k = d_k / (eps + beta_k^2),
 // eps = 1e-16, as in the reference (wenoUpwinds.cc:5-84). The normalised
 // weights are formed with one division: sum_k d_k q_k / s_k over sum_k d_k / s_k
-// is multiplied through by s_0 s_1 (s_2), which is the same quantity up to
-// rounding. beta_k are the Jiang-Shu smoothness indicators in factored form.
+// is multiplied through by s_0 s_1 (s_2) and divided by d_0, which is the same
+// quantity up to rounding. The indicators and candidate values are written in
+// first differences of the stencil so that they share their terms.
 
 constexpr double kWenoEps = 1e-16;
 
 // Upwind reconstruction at the right face of v1 from (v0, v1, v2), as a
 // numerator/denominator pair (value = n / d); the downwind value at the left
-// face of v1 is the same function of (v2, v1, v0).
+// face of v1 is the same function of (v2, v1, v0). Linear weights (1/3, 2/3)
+// over 1/3 give t0 = s1, t1 = 2 s0; candidates q0 = v1 + (v1 - v0)/2 and
+// q1 = (v1 + v2)/2, so t1 q1 = s0 (v1 + v2).
 __device__ __forceinline__ void weno3_nd(double v0, double v1, double v2, double &n, double &d) {
   const double a0 = v1 - v0, a1 = v2 - v1;
   const double b0 = a0 * a0, b1 = a1 * a1;
   const double s0 = fma(b0, b0, kWenoEps), s1 = fma(b1, b1, kWenoEps);
-  const double q0 = fma(1.5, v1, -0.5 * v0);  // (-v0 + 3 v1) / 2
-  const double q1 = 0.5 * (v1 + v2);          // (v1 + v2) / 2
-  const double t0 = (1.0 / 3.0) * s1, t1 = (2.0 / 3.0) * s0;
-  n = fma(t0, q0, t1 * q1);
-  d = t0 + t1;
+  n = fma(s1, fma(0.5, a0, v1), s0 * (v1 + v2));
+  d = fma(2.0, s0, s1);
 }
 
-__device__ __forceinline__ double weno3(double v0, double v1, double v2) {
-  double n, d;
-  weno3_nd(v0, v1, v2, n, d);
-  return n / d;
-}
-
-// Upwind reconstruction at the right face of v2 from (v0..v4) as n / d. The
-// indicators are formed as 4 beta_k = 13/3 (second difference)^2 + (one-sided
-// difference)^2, with eps scaled by 16 to match, which leaves the weights
-// unchanged.
+// Upwind reconstruction at the right face of v2 from (v0..v4) as n / d. With
+// the differences d01 = v0 - v1, d12, d23, d34, the second differences are
+// a_k = d(k)(k+1) - d(k+1)(k+2) and the one-sided ones
+//   e0 = v0 - 4 v1 + 3 v2 = d01 - 3 d12,  e1 = v1 - v3 = d12 + d23,
+//   e2 = 3 v2 - 4 v3 + v4 = 3 d23 - d34,
+// so 4 beta_k = 13/3 a_k^2 + e_k^2 (eps scaled by 16 to match, which leaves
+// the weights unchanged). Linear weights (0.1, 0.6, 0.3) over 0.1 are
+// (1, 6, 3); the candidates times those weights are
+//   q0     = (2 v0 - 7 v1 + 11 v2)/6 = a0/3 + v2 - d12/2,
+//   6 q1   = -v1 + 5 v2 + 2 v3      = 6 v2 - d12 - 2 d23,
+//   3 q2   = (2 v2 + 5 v3 - v4)/2   = 3 v3 + d23 + d34/2.
 __device__ __forceinline__ void weno5_nd(double v0, double v1, double v2, double v3, double v4, double &n,
                                          double &d) {
   constexpr double c13 = 13.0 / 3.0;
   constexpr double eps16 = 16.0 * kWenoEps;
-  const double a0 = v0 - 2.0 * v1 + v2, e0 = v0 - 4.0 * v1 + 3.0 * v2;
-  const double a1 = v1 - 2.0 * v2 + v3, e1 = v1 - v3;
-  const double a2 = v2 - 2.0 * v3 + v4, e2 = 3.0 * v2 - 4.0 * v3 + v4;
+  const double d01 = v0 - v1, d12 = v1 - v2, d23 = v2 - v3, d34 = v3 - v4;
+  const double a0 = d01 - d12, a1 = d12 - d23, a2 = d23 - d34;
+  const double e0 = fma(-3.0, d12, d01), e1 = d12 + d23, e2 = fma(3.0, d23, -d34);
   const double b0 = fma(c13 * a0, a0, e0 * e0);
   const double b1 = fma(c13 * a1, a1, e1 * e1);
   const double b2 = fma(c13 * a2, a2, e2 * e2);
   const double s0 = fma(b0, b0, eps16), s1 = fma(b1, b1, eps16), s2 = fma(b2, b2, eps16);
-  // candidate stencils (2 v0 - 7 v1 + 11 v2)/6, (-v1 + 5 v2 + 2 v3)/6, (2 v2 + 5 v3 - v4)/6
-  const double q0 = fma(1.0 / 3.0, v0, fma(-7.0 / 6.0, v1, (11.0 / 6.0) * v2));
-  const double q1 = fma(-1.0 / 6.0, v1, fma(5.0 / 6.0, v2, (1.0 / 3.0) * v3));
-  const double q2 = fma(1.0 / 3.0, v2, fma(5.0 / 6.0, v3, (-1.0 / 6.0) * v4));
-  const double t0 = 0.1 * (s1 * s2), t1 = 0.6 * (s0 * s2), t2 = 0.3 * (s0 * s1);
-  n = fma(t0, q0, fma(t1, q1, t2 * q2));
-  d = t0 + t1 + t2;
+  const double q0 = fma(1.0 / 3.0, a0, fma(-0.5, d12, v2));
+  const double q1 = fma(-2.0, d23, fma(6.0, v2, -d12));
+  const double q2 = fma(3.0, v3, fma(0.5, d34, d23));
+  const double p0 = s1 * s2, p1 = s0 * s2, p2 = s0 * s1;
+  n = fma(p0, q0, fma(p1, q1, p2 * q2));
+  d = fma(6.0, p1, fma(3.0, p2, p0));
 }
 
-__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4) {
-  double n, d;
-  weno5_nd(v0, v1, v2, v3, v4, n, d);
-  return n / d;
+// a / b for a positive normal b whose reciprocal is normal too (the WENO
+// denominators, products of eps-bounded positive sums): the hardware
+// reciprocal estimate refined by two Newton-Raphson steps, then one residual
+// correction of the quotient. It skips the operand scaling and special-case
+// fix-up of the general IEEE division and can differ from it in the last bit.
+__device__ __forceinline__ double div_pos(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
 }
 
 template <int ORDER>
@@ -94,7 +101,7 @@ struct Weno;
 // are also available apart (up, down, combine) for kernels that stage f+ and
 // f- through LDS one after the other; the arithmetic is the same.
 __device__ __forceinline__ double weno_combine(double nu, double du, double nd, double dd) {
-  return fma(nu, dd, nd * du) / (du * dd);
+  return div_pos(fma(nu, dd, nd * du), du * dd);
 }
 
 template <>

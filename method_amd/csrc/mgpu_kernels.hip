@@ -108,6 +108,16 @@ struct SweepShape {
   }
   static constexpr int kStrideL = DFAST ? 1 : TC;  // LDS step between neighbouring cells of a pencil
   static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
+  // fidx(v, lf, cc) of work item v * kFaces + idx, idx enumerating (lf, cc) in
+  // gidx order: for the strided layout idx = lf * TC + cc already is the
+  // offset within the variable; for the contiguous one each pencil skips the
+  // 2R - 1 halo slots past its TL + 1 faces.
+  __device__ static unsigned item_fidx(unsigned item) {
+    const unsigned v = item / kFaces;
+    unsigned off = item + v * (unsigned)(TLH * TC - kFaces);
+    if (DFAST) off += (item - v * kFaces) / (TL + 1) * (unsigned)(TLH - TL - 1);
+    return off;
+  }
 };
 
 // Counters of one cell's cons2prim, accumulated over a block.
@@ -200,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   const int d0 = td * TL, c0 = tc * TC;
   const long long base_e = (long long)e * g.stride[EAX];
   const long long vs = g.vstride;
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
 
   if constexpr (S::kSplit) {
     // Phase 1: split fluxes of every tile cell and its halo along DAX. f+ goes
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     double fm_keep[kCellPasses][NV];
   #pragma unroll
     for (int cp = 0; cp < kCellPasses; ++cp) {
-      const int idx = tid + cp * kThreads;
+      const unsigned idx = tid + cp * kThreads;
       if (idx < kCells) {
         const int l = DFAST ? idx % TLH : idx / TC;
         const int cc = DFAST ? idx / TLH : idx % TC;
@@ -255,19 +265,16 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     double nu[kPasses], du[kPasses];
   #pragma unroll
     for (int k = 0; k < kPasses; ++k) {
-      const int item = tid + k * kThreads;
+      const unsigned item = tid + k * kThreads;
       if (item < kItems) {
-        const int v = item / S::kFaces, idx = item - v * S::kFaces;
-        const int lf = DFAST ? idx % (TL + 1) : idx / TC;
-        const int cc = DFAST ? idx / (TL + 1) : idx % TC;
-        const double *fpv = s_f + S::fidx(v, lf, cc);
+        const double *fpv = s_f + S::item_fidx(item);
         Weno<ORDER>::up([&](int m) { return fpv[m * S::kStrideL]; }, nu[k], du[k]);
       }
     }
     __syncthreads();
   #pragma unroll
     for (int cp = 0; cp < kCellPasses; ++cp) {
-      const int idx = tid + cp * kThreads;
+      const unsigned idx = tid + cp * kThreads;
       if (idx < kCells) {
         const int l = DFAST ? idx % TLH : idx / TC;
         const int cc = DFAST ? idx / TLH : idx % TC;
@@ -278,12 +285,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     __syncthreads();
   #pragma unroll
     for (int k = 0; k < kPasses; ++k) {
-      const int item = tid + k * kThreads;
+      const unsigned item = tid + k * kThreads;
       if (item < kItems) {
-        const int v = item / S::kFaces, idx = item - v * S::kFaces;
-        const int lf = DFAST ? idx % (TL + 1) : idx / TC;
-        const int cc = DFAST ? idx / (TL + 1) : idx % TC;
-        const double *fmv = s_f + S::fidx(v, lf, cc);
+        const double *fmv = s_f + S::item_fidx(item);
         double nd, dd;
         Weno<ORDER>::down([&](int m) { return fmv[m * S::kStrideL]; }, nd, dd);
         nu[k] = weno_combine(nu[k], du[k], nd, dd);
@@ -294,13 +298,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     // layouts, so the face of item goes to s_f[item]
   #pragma unroll
     for (int k = 0; k < kPasses; ++k) {
-      const int item = tid + k * kThreads;
+      const unsigned item = tid + k * kThreads;
       if (item < kItems) s_f[item] = nu[k];
     }
     __syncthreads();
   } else {
     // Phase 1: f+ and f- of every tile cell and its halo side by side in LDS.
-    for (int idx = tid; idx < TLH * TC; idx += kThreads) {
+    for (unsigned idx = tid; idx < TLH * TC; idx += kThreads) {
       const int l = DFAST ? idx % TLH : idx / TC;
       const int cc = DFAST ? idx / TLH : idx % TC;
       const int gc = c0 + cc;
@@ -336,13 +340,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     double face[kPasses];
 #pragma unroll
     for (int k = 0; k < kPasses; ++k) {
-      const int item = tid + k * kThreads;
+      const unsigned item = tid + k * kThreads;
       if (item < kItems) {
-        const int v = item / S::kFaces, idx = item - v * S::kFaces;
-        const int lf = DFAST ? idx % (TL + 1) : idx / TC;
-        const int cc = DFAST ? idx / (TL + 1) : idx % TC;
-        const double *fpv = s_f + S::fidx(v, lf, cc);
-        const double *fmv = s_f + S::fidx(NV + v, lf, cc);
+        const double *fpv = s_f + S::item_fidx(item);
+        const double *fmv = fpv + NV * TLH * TC;
         face[k] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
                                     [&](int m) { return fmv[m * S::kStrideL]; });
       }
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPasses; ++k) {
-      const int item = tid + k * kThreads;
+      const unsigned item = tid + k * kThreads;
       if (item < kItems) s_f[item] = face[k];
     }
     __syncthreads();
