@@ -32,8 +32,12 @@ extern "C" {
 
 enum { MGPU_SRMHD = 0, MGPU_SRRMHD = 1 };
 /* MGPU_RKSPLIT: RK2 followed by the operator-split source update and
- * another cons2prim (RKSplit::step, rkSplit.cc:31-53). */
-enum { MGPU_RK2 = 0, MGPU_SSP2 = 1, MGPU_RKSPLIT = 2 };
+ * another cons2prim (RKSplit::step, rkSplit.cc:31-53). MGPU_RK2B, MGPU_RK3,
+ * MGPU_RK4 and MGPU_RK4_10: the RKPlus family with rhs = source - F
+ * (RK2B/RK3/RK4/RK4_10::step, RKPlus.cc:56-726); for RK4_10 quirks = 1 keeps
+ * the reference's reset of the stage primitives at its second preparation
+ * (RKPlus.cc:612-640), quirks = 0 gives the new stage state its own. */
+enum { MGPU_RK2 = 0, MGPU_SSP2 = 1, MGPU_RKSPLIT = 2, MGPU_RK2B = 3, MGPU_RK3 = 4, MGPU_RK4 = 5, MGPU_RK4_10 = 6 };
 enum { MGPU_BC_OUTFLOW = 0, MGPU_BC_PERIODIC = 1 };
 
 /* Simulation parameters: the constructor arguments of the reference Data
@@ -48,9 +52,9 @@ typedef struct {
   double gamma, sigma, cp, cfl;
   int model;       /* MGPU_SRMHD | MGPU_SRRMHD */
   int weno_order;  /* 3 | 5 */
-  int integrator;  /* MGPU_RK2 | MGPU_SSP2 | MGPU_RKSPLIT */
+  int integrator;  /* MGPU_RK2 | MGPU_SSP2 | MGPU_RKSPLIT | MGPU_RK2B | MGPU_RK3 | MGPU_RK4 | MGPU_RK4_10 */
   int bc[3];       /* per axis: MGPU_BC_PERIODIC | MGPU_BC_OUTFLOW */
-  int quirks;      /* 1: reproduce the reference's flux-index quirks */
+  int quirks;      /* 1: reproduce the reference's quirks (flux indices, C2P retry, RK4_10 reset) */
   int device;      /* HIP device ordinal */
 } mgpu_config;
 
@@ -72,7 +76,7 @@ const char *mgpu_last_error(void);
 
 /* Allocates device state for one simulation. Replaces the allocations of
  * Simulation::Simulation (Project/CPU/Src/simulation.cc:22-30) and the
- * integrator's work arrays (RK2.cc:5-18, SSP2.cc:13-34). */
+ * integrator's work arrays (RK2.cc:5-18, SSP2.cc:13-34, RKPlus.cc:56-586). */
 int mgpu_create(const mgpu_config *cfg, mgpu_engine **out);
 int mgpu_destroy(mgpu_engine *e);
 
@@ -94,7 +98,8 @@ int mgpu_download_state(mgpu_engine *e, double *cons, double *prims, double *aux
 int mgpu_prims_to_all(mgpu_engine *e);
 
 /* One full time step of the configured integrator on the resident state with
- * step size dt: TimeIntegrator::step (RK2.cc:93-100, SSP2.cc:53-173), with
+ * step size dt: TimeIntegrator::step (RK2.cc:93-100, SSP2.cc:53-173,
+ * rkSplit.cc:31-53, RKPlus.cc:127-726), with
  * FVS::F, Model::getPrimitiveVars and Bcs::apply inside. stats may be NULL. */
 int mgpu_step(mgpu_engine *e, double dt, mgpu_stats *stats);
 

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,10 @@ struct mgpu_engine {
   hipStream_t stream = nullptr;
   double *U = nullptr, *V = nullptr, *A = nullptr, *U1 = nullptr, *V1 = nullptr, *fnet = nullptr;
   double *U2 = nullptr, *V2 = nullptr, *F1 = nullptr, *S1 = nullptr, *S2 = nullptr;  // SSP2 only
+  // RKPlus stage states (aux only for the resistive model, whose source reads
+  // the current from aux) and RK4's kept stage-four rhs
+  double *rkU[4] = {}, *rkV[4] = {}, *rkA[4] = {}, *rkR = nullptr;
+  double *rkF = nullptr;  // RK4_10: stored flux vectors [dir][var][cell]
   double *staging = nullptr;
   size_t staging_elems = 0;
   C2PCounters *dstats = nullptr;
@@ -276,6 +281,194 @@ int rksplit_step(mgpu_engine *e, double dt, C2PCounters *st) {
   return retry(e, e->U, e->V, e->A, st);
 }
 
+// ---------------------------------------------------------------- RKPlus
+
+RkStageArgs rk_args(mgpu_engine *e, double dt, C2PCounters *st) {
+  RkStageArgs a;
+  memset(&a, 0, sizeof a);
+  a.geo = e->geo;
+  a.dt = dt;
+  a.gamma = e->cfg.gamma;
+  a.sigma = e->cfg.sigma;
+  a.cp = e->cfg.cp;
+  a.stats = st;
+  a.fail_count = e->fail_count;
+  a.fail_list = e->fail_list;
+  a.fail_flag = e->fail_flag;
+  return a;
+}
+
+// RKPlus::rhs of the state (Ub, Vb, Ab) (RKPlus.cc:15-50): its net flux into
+// e->fnet by the three sweeps; the stage kernel adds the source.
+int rk_rhs(mgpu_engine *e, RkStageArgs &a, const double *Ub, const double *Vb, const double *Ab) {
+  SweepArgs s = sweep_args(e, Ub, Vb, nullptr);
+  if (run_sweeps(e, s, kSweepAdd)) return -1;
+  a.Ub = Ub;
+  a.Ab = Ab;
+  a.fnet = e->fnet;
+  return 0;
+}
+
+// The same from flux vectors formed from the stored arrays as the reference
+// forms them, for a state whose cons and prims/aux disagree (RK4_10 after its
+// second preparation).
+int rk_rhs_stored(mgpu_engine *e, RkStageArgs &a, const double *Ub, const double *Vb, const double *Ab) {
+  launch_flux_stored(e->geo, e->cfg.model, Ub, Vb, Ab, e->rkF, e->cfg.quirks, e->stream);
+  HIP_OK(hipGetLastError());
+  if (mark(e, "flux_stored")) return -1;
+  SweepArgs s = sweep_args(e, Ub, Vb, nullptr);
+  s.Fin = e->rkF;
+  for (int dir = e->geo.dims - 1; dir >= 0; --dir) {
+    launch_sweep(s, e->cfg.model + 2, e->cfg.weno_order, dir, dir == e->geo.dims - 1 ? kSweepWrite : kSweepAdd, 0,
+                 e->stream);
+    HIP_OK(hipGetLastError());
+    if (mark(e, "sweep_stored")) return -1;
+  }
+  a.Ub = Ub;
+  a.Ab = Ab;
+  a.fnet = e->fnet;
+  return 0;
+}
+
+// Stage combination and finalise of its state (TimeIntegrator::finalise,
+// timeInt.h:117-132): cons2prim from Vg, the SRMHD retry.
+int rk_finish(mgpu_engine *e, const RkStageArgs &a, C2PCounters *st) {
+  launch_rkplus_stage(a, e->cfg.model, e->stream);
+  HIP_OK(hipGetLastError());
+  if (mark(e, "rkplus_stage_c2p")) return -1;
+  return retry(e, a.Uout, a.Vout, a.Aout, st);
+}
+
+void rk_terms(RkStageArgs &a, std::initializer_list<const double *> x, std::initializer_list<double> w) {
+  a.nx = 0;
+  auto wi = w.begin();
+  for (const double *p : x) {
+    a.X[a.nx] = p;
+    a.w[a.nx++] = *wi++;
+  }
+}
+
+void rk_out(RkStageArgs &a, double *U, double *V, double *A, const double *Vg) {
+  a.Uout = U;
+  a.Vout = V;
+  a.Aout = A;
+  a.Vg = Vg;
+}
+
+// RK2B, RK3, RK4 and RK4_10::step (RKPlus.cc:127-726). A stage's cons2prim
+// starts from the previous stage's prims, the last stage's from the step's.
+int rkplus_step(mgpu_engine *e, double dt, C2PCounters *st) {
+  double *U = e->U, *V = e->V, *A = e->A;
+  double **Us = e->rkU, **Vs = e->rkV, **As = e->rkA;
+  switch (e->cfg.integrator) {
+    case MGPU_RK2B: {
+      RkStageArgs a = rk_args(e, dt, st);
+      if (rk_rhs(e, a, U, V, A)) return -1;
+      rk_terms(a, {U}, {1.0});
+      a.wr = 1.0;
+      rk_out(a, Us[0], Vs[0], As[0], V);
+      if (rk_finish(e, a, st)) return -1;
+      RkStageArgs b = rk_args(e, dt, st);
+      if (rk_rhs(e, b, Us[0], Vs[0], As[0])) return -1;
+      rk_terms(b, {U, Us[0]}, {1.0, 1.0});
+      b.half_sum = 1;
+      rk_out(b, U, V, A, V);
+      return rk_finish(e, b, st);
+    }
+    case MGPU_RK3: {
+      RkStageArgs a = rk_args(e, dt, st);
+      if (rk_rhs(e, a, U, V, A)) return -1;
+      rk_terms(a, {U}, {1.0});
+      a.wr = 1.0;
+      rk_out(a, Us[0], Vs[0], As[0], V);
+      if (rk_finish(e, a, st)) return -1;
+      RkStageArgs b = rk_args(e, dt, st);
+      if (rk_rhs(e, b, Us[0], Vs[0], As[0])) return -1;
+      rk_terms(b, {U, Us[0]}, {3.0 / 4.0, 1.0 / 4.0});
+      b.wr = 1.0 / 4.0;
+      rk_out(b, Us[1], Vs[1], As[1], Vs[0]);
+      if (rk_finish(e, b, st)) return -1;
+      RkStageArgs c = rk_args(e, dt, st);
+      if (rk_rhs(e, c, Us[1], Vs[1], As[1])) return -1;
+      rk_terms(c, {U, Us[1]}, {1.0 / 3.0, 2.0 / 3.0});
+      c.wr = 2.0 / 3.0;
+      rk_out(c, U, V, A, V);
+      return rk_finish(e, c, st);
+    }
+    case MGPU_RK4: {
+      // RK4::stage1..stage5 (RKPlus.cc:330-470)
+      static const double wn[4] = {1.0, 0.444370493651235, 0.620101851488403, 0.178079954393132};
+      static const double wp[4] = {0.0, 0.555629506348765, 0.379898148511597, 0.821920045606868};
+      static const double wr[4] = {0.391752226571890, 0.368410593050371, 0.251891774271694, 0.544974750228521};
+      for (int k = 0; k < 4; ++k) {
+        const double *Ub = k ? Us[k - 1] : U, *Vb = k ? Vs[k - 1] : V, *Ab = k ? As[k - 1] : A;
+        RkStageArgs a = rk_args(e, dt, st);
+        if (rk_rhs(e, a, Ub, Vb, Ab)) return -1;
+        if (k)
+          rk_terms(a, {U, Us[k - 1]}, {wn[k], wp[k]});
+        else
+          rk_terms(a, {U}, {wn[0]});
+        a.wr = wr[k];
+        if (k == 3) a.Rout = e->rkR;
+        rk_out(a, Us[k], Vs[k], As[k], Vb);
+        if (rk_finish(e, a, st)) return -1;
+      }
+      RkStageArgs a = rk_args(e, dt, st);
+      if (rk_rhs(e, a, Us[3], Vs[3], As[3])) return -1;
+      rk_terms(a, {Us[1], Us[2], Us[3]}, {0.517231671970585, 0.096059710526147, 0.386708617503269});
+      a.R = e->rkR;
+      a.wR = 0.063692468666290;
+      a.wr = 0.226007483236906;
+      rk_out(a, U, V, A, V);
+      return rk_finish(e, a, st);
+    }
+    default: {
+      // RK4_10::prepare1, stageRepeat x 5, prepare2, stageRepeat x 4,
+      // stageFinal (RKPlus.cc:586-726)
+      const size_t cb = sizeof(double) * e->ncons * e->geo.vstride;
+      const size_t pb = sizeof(double) * e->nprims * e->geo.vstride;
+      const size_t ab = sizeof(double) * e->naux * e->geo.vstride;
+      HIP_OK(hipMemcpyAsync(Us[0], U, cb, hipMemcpyDeviceToDevice, e->stream));
+      HIP_OK(hipMemcpyAsync(Us[1], U, cb, hipMemcpyDeviceToDevice, e->stream));
+      HIP_OK(hipMemcpyAsync(Vs[0], V, pb, hipMemcpyDeviceToDevice, e->stream));
+      if (As[0]) HIP_OK(hipMemcpyAsync(As[0], A, ab, hipMemcpyDeviceToDevice, e->stream));
+      for (int s = 0; s < 9; ++s) {
+        if (s == 5) {
+          HIP_OK(hipMemcpyAsync(Vs[0], V, pb, hipMemcpyDeviceToDevice, e->stream));
+          if (As[0]) HIP_OK(hipMemcpyAsync(As[0], A, ab, hipMemcpyDeviceToDevice, e->stream));
+          launch_rk410_prepare2(e->geo, e->ncons, Us[0], Us[1], e->stream);
+          HIP_OK(hipGetLastError());
+          if (!e->cfg.quirks) {
+            // the new stage state gets its own primitives before its rhs
+            RkStageArgs c = rk_args(e, dt, st);
+            rk_terms(c, {Us[0]}, {1.0});
+            rk_out(c, Us[0], Vs[0], As[0], Vs[0]);
+            if (rk_finish(e, c, st)) return -1;
+          }
+        }
+        RkStageArgs a = rk_args(e, dt, st);
+        // with the quirk the first stage after prepare2 pairs new cons with
+        // the step's prims and aux: its fluxes come from those stored arrays
+        if (s == 5 && e->cfg.quirks) {
+          if (rk_rhs_stored(e, a, Us[0], Vs[0], As[0])) return -1;
+        } else if (rk_rhs(e, a, Us[0], Vs[0], As[0])) {
+          return -1;
+        }
+        rk_terms(a, {Us[0]}, {1.0});
+        a.wr = 1.0 / 6.0;
+        rk_out(a, Us[0], Vs[0], As[0], Vs[0]);
+        if (rk_finish(e, a, st)) return -1;
+      }
+      RkStageArgs a = rk_args(e, dt, st);
+      if (rk_rhs(e, a, Us[0], Vs[0], As[0])) return -1;
+      rk_terms(a, {Us[1], Us[0]}, {1.0, 3.0 / 5.0});
+      a.wr = 1.0 / 10.0;
+      rk_out(a, U, V, A, V);
+      return rk_finish(e, a, st);
+    }
+  }
+}
+
 int do_step(mgpu_engine *e, double dt, mgpu_stats *stats) {
   C2PCounters *st = stats ? e->dstats : nullptr;
   if (stats && zero_stats(e)) return -1;
@@ -283,6 +476,8 @@ int do_step(mgpu_engine *e, double dt, mgpu_stats *stats) {
     if (ssp2_step(e, dt, st)) return -1;
   } else if (e->cfg.integrator == MGPU_RKSPLIT) {
     if (rksplit_step(e, dt, st)) return -1;
+  } else if (e->cfg.integrator >= MGPU_RK2B) {
+    if (rkplus_step(e, dt, st)) return -1;
   } else if (rk2_step(e, dt, st)) {
     return -1;
   }
@@ -317,7 +512,7 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   *out = nullptr;
   const mgpu_config &c = *cfg;
   if (c.model != MGPU_SRMHD && c.model != MGPU_SRRMHD) return fail("mgpu_create: unknown model %d", c.model);
-  if (c.integrator != MGPU_RK2 && c.integrator != MGPU_SSP2 && c.integrator != MGPU_RKSPLIT)
+  if (c.integrator < MGPU_RK2 || c.integrator > MGPU_RK4_10)
     return fail("mgpu_create: unknown integrator %d", c.integrator);
   if (c.model == MGPU_SRMHD && c.integrator == MGPU_SSP2)
     return fail("mgpu_create: SSP2 is provided with the SRRMHD model only");
@@ -390,6 +585,27 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
       return cleanup(fail("hipMalloc of %zu bytes failed", b.elems * sizeof(double)));
     if (hipMemset(*b.ptr, 0, b.elems * sizeof(double)) != hipSuccess) return cleanup(fail("hipMemset failed"));
   }
+  if (c.integrator >= MGPU_RK2B) {
+    // stage states of RK2B, RK3, RK4, RK4_10 (RKPlus.cc:56-586)
+    static const int kStages[4] = {1, 2, 4, 2};
+    const int ns = kStages[c.integrator - MGPU_RK2B];
+    for (int k = 0; k < ns; ++k) {
+      struct {
+        double **ptr;
+        size_t elems;
+      } sb[] = {{&e->rkU[k], n * e->ncons}, {&e->rkV[k], n * e->nprims}, {&e->rkA[k], res ? n * e->naux : 0}};
+      for (auto &b : sb) {
+        if (!b.elems) continue;
+        if (hipMalloc(b.ptr, b.elems * sizeof(double)) != hipSuccess)
+          return cleanup(fail("hipMalloc of %zu bytes failed", b.elems * sizeof(double)));
+        if (hipMemset(*b.ptr, 0, b.elems * sizeof(double)) != hipSuccess) return cleanup(fail("hipMemset failed"));
+      }
+    }
+    if (c.integrator == MGPU_RK4 && hipMalloc(&e->rkR, n * e->ncons * sizeof(double)) != hipSuccess)
+      return cleanup(fail("hipMalloc of the RK4 rhs store failed"));
+    if (c.integrator == MGPU_RK4_10 && hipMalloc(&e->rkF, n * e->ncons * g.dims * sizeof(double)) != hipSuccess)
+      return cleanup(fail("hipMalloc of the RK4_10 flux store failed"));
+  }
   if (hipMalloc(&e->dstats, sizeof(C2PCounters) * kStatBuckets) != hipSuccess || hipMalloc(&e->fail_count, sizeof(int)) != hipSuccess ||
       hipMalloc(&e->fail_list, n * sizeof(long long)) != hipSuccess || hipMalloc(&e->fail_flag, n) != hipSuccess)
     return cleanup(fail("hipMalloc of C2P work arrays failed"));
@@ -406,6 +622,13 @@ int mgpu_destroy(mgpu_engine *e) {
   double *bufs[] = {e->U, e->V, e->A, e->U1, e->V1, e->fnet, e->staging, e->U2, e->V2, e->F1, e->S1, e->S2};
   for (double *b : bufs)
     if (b) hipFree(b);
+  for (int k = 0; k < 4; ++k) {
+    if (e->rkU[k]) hipFree(e->rkU[k]);
+    if (e->rkV[k]) hipFree(e->rkV[k]);
+    if (e->rkA[k]) hipFree(e->rkA[k]);
+  }
+  if (e->rkR) hipFree(e->rkR);
+  if (e->rkF) hipFree(e->rkF);
   if (e->dstats) hipFree(e->dstats);
   if (e->dspeed) hipFree(e->dspeed);
   if (e->fail_count) hipFree(e->fail_count);

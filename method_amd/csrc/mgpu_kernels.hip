@@ -69,8 +69,10 @@ __device__ __forceinline__ void reduce_stats(C2PCounters *st, int cells, int ite
 struct SrmhdM {
   static constexpr int NC = kSrmhdCons, NP = kSrmhdPrims, NA = kSrmhdAux;
   static constexpr int P0 = 1, NPL = 4;  // vx, vy, vz, p
+  static constexpr bool kStored = false;
   template <int DIR, bool Q>
-  __device__ __forceinline__ static void flux(const double *u, const double *pl, const SweepArgs &, double *f) {
+  __device__ __forceinline__ static void flux(const double *u, const double *pl, const SweepArgs &, long long,
+                                              double *f) {
     srmhd_flux<DIR, Q>(u, pl[0], pl[1], pl[2], pl[3], f);
   }
 };
@@ -78,11 +80,75 @@ struct SrmhdM {
 struct SrrmhdM {
   static constexpr int NC = kSrrmhdCons, NP = kSrrmhdPrims, NA = kSrrmhdAux;
   static constexpr int P0 = 0, NPL = 5;  // rho, vx, vy, vz, p
+  static constexpr bool kStored = false;
   template <int DIR, bool Q>
-  __device__ __forceinline__ static void flux(const double *u, const double *pl, const SweepArgs &a, double *f) {
+  __device__ __forceinline__ static void flux(const double *u, const double *pl, const SweepArgs &a, long long,
+                                              double *f) {
     srrmhd_flux<DIR, Q>(u, pl[0], pl[1], pl[2], pl[3], pl[4], a.gamma, a.sigma, f);
   }
 };
+
+// Model M with the flux vectors read from a.Fin (written by k_flux_stored).
+template <class M>
+struct StoredM {
+  static constexpr int NC = M::NC, NP = M::NP, NA = M::NA;
+  static constexpr int P0 = 0, NPL = 1;
+  static constexpr bool kStored = true;
+  template <int DIR, bool Q>
+  __device__ __forceinline__ static void flux(const double *, const double *, const SweepArgs &a, long long q,
+                                              double *f) {
+    const long long vs = a.geo.vstride;
+    const double *F = a.Fin + (long long)DIR * NC * vs;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) f[v] = F[v * vs + q];
+  }
+};
+
+// SRMHD::fluxVector from stored arrays (srmhd.cc:72-206): B, W and b^mu come
+// from the prims (the aux the reference reads is their function), D, S, tau
+// and phi from the cons.
+template <int DIR, bool QUIRK>
+__device__ __forceinline__ void srmhd_flux_stored(const double u[9], const double P[8], double f[9]) {
+  double uu[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) uu[v] = u[v];
+  uu[5] = P[5];
+  uu[6] = P[6];
+  uu[7] = P[7];
+  srmhd_flux<DIR, QUIRK>(uu, P[1], P[2], P[3], P[4], f);
+}
+
+template <int DIR, bool QUIRK>
+__device__ __forceinline__ void flux_stored_cell(int model, const double *U, const double *V, const double *A,
+                                                 long long vs, long long q, double *F) {
+  double u[kSrrmhdCons], P[kSrrmhdPrims], aux[kSrrmhdAux], f[kSrrmhdCons];
+  if (model == 0) {
+    for (int v = 0; v < kSrmhdCons; ++v) u[v] = U[v * vs + q];
+    for (int v = 0; v < kSrmhdPrims; ++v) P[v] = V[v * vs + q];
+    srmhd_flux_stored<DIR, QUIRK>(u, P, f);
+    for (int v = 0; v < kSrmhdCons; ++v) F[((long long)DIR * kSrmhdCons + v) * vs + q] = f[v];
+  } else {
+    for (int v = 0; v < kSrrmhdCons; ++v) u[v] = U[v * vs + q];
+    for (int v = 0; v < kSrrmhdPrims; ++v) P[v] = V[v * vs + q];
+    for (int v = 0; v < kSrrmhdAux; ++v) aux[v] = A[v * vs + q];
+    srrmhd_flux_stored<DIR, QUIRK>(u, P, aux, f);
+    for (int v = 0; v < kSrrmhdCons; ++v) F[((long long)DIR * kSrrmhdCons + v) * vs + q] = f[v];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_flux_stored(Geo g, int model, const double *U, const double *V,
+                                                          const double *A, double *F, int quirk) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  flux_stored_cell<0, false>(model, U, V, A, g.vstride, q, F);
+  if (g.dims > 1) flux_stored_cell<1, false>(model, U, V, A, g.vstride, q, F);
+  if (g.dims > 2) {
+    if (quirk)
+      flux_stored_cell<2, true>(model, U, V, A, g.vstride, q, F);
+    else
+      flux_stored_cell<2, false>(model, U, V, A, g.vstride, q, F);
+  }
+}
 
 template <class M, int ORDER, int DAX, int CAX, bool DFAST>
 struct SweepShape {
@@ -189,6 +255,10 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
   c2p_store_srrmhd(a, q, un, A, cs);
 }
 
+// Stored-flux sweeps only accumulate the net flux (write and add modes).
+template <class M>
+__device__ __forceinline__ void finish_rk2(const SweepArgs &, long long, const double *, StoredM<M>, CellC2P &) {}
+
 // One directional sweep of the flux divergence of model M.
 template <class M, int ORDER, int DAX, int CAX, bool DFAST, int MODE, bool QUIRK>
 __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
@@ -239,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   #pragma unroll
           for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
           double f[NV];
-          M::template flux<DAX, QUIRK>(u, pl, a, f);
+          M::template flux<DAX, QUIRK>(u, pl, a, q, f);
   #pragma unroll
           for (int v = 0; v < NV; ++v) {
             fp[v] = 0.5 * (f[v] + u[v]);
@@ -318,7 +388,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
 #pragma unroll
         for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
         double f[NV];
-        M::template flux<DAX, QUIRK>(u, pl, a, f);
+        M::template flux<DAX, QUIRK>(u, pl, a, q, f);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           fp[v] = 0.5 * (f[v] + u[v]);
@@ -448,7 +518,7 @@ __device__ __forceinline__ void split_cell(const SweepArgs &a, long long q, doub
   for (int v = 0; v < M::NC; ++v) u[v] = a.U[v * vs + q];
 #pragma unroll
   for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
-  M::template flux<DIR, QUIRK>(u, pl, a, f);
+  M::template flux<DIR, QUIRK>(u, pl, a, q, f);
 #pragma unroll
   for (int v = 0; v < M::NC; ++v) {
     fp[v] = 0.5 * (f[v] + u[v]);
@@ -926,6 +996,106 @@ __global__ __launch_bounds__(kThreads) void k_c2p_srrmhd(Geo g, const double *U,
   reduce_stats(st, cs.cells, cs.iters, cs.fails, cs.maxit);
 }
 
+// --------------------------------------------------------------- RKPlus
+
+template <class M>
+__global__ __launch_bounds__(kThreads) void k_rkplus_stage(RkStageArgs a) {
+  constexpr int NC = M::NC;
+  constexpr bool kRes = NC == kSrrmhdCons;
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  CellC2P cs;
+  if (q < a.geo.vstride) {
+    const long long vs = a.geo.vstride;
+    double r[NC];
+#pragma unroll
+    for (int v = 0; v < NC; ++v) r[v] = 0.0;
+    if (a.fnet) {
+      // Model::sourceTerm of the state whose rhs this is (srmhd.cc:233-250,
+      // srrmhd.cc:194-207), minus its net flux
+      if constexpr (kRes) {
+        double u[kSrrmhdCons], A[kSrrmhdAux], s5[5];
+#pragma unroll
+        for (int v = 0; v < kSrrmhdCons; ++v) u[v] = a.Ub[v * vs + q];
+#pragma unroll
+        for (int v = 4; v < 7; ++v) A[v] = a.Ab[v * vs + q];
+        srrmhd_source5(u, A, a.cp, s5);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) r[8 + m] = s5[m];
+      } else {
+        r[8] = -a.Ub[8 * vs + q] / (a.cp * a.cp);
+      }
+#pragma unroll
+      for (int v = 0; v < NC; ++v) r[v] = r[v] - a.fnet[v * vs + q];
+    }
+    double un[NC];
+    if (a.half_sum) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) un[v] = 0.5 * (a.X[0][v * vs + q] + a.X[1][v * vs + q] + a.dt * r[v]);
+    } else {
+      const double cr = a.wr * a.dt, cR = a.wR * a.dt;
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        double acc = a.w[0] * a.X[0][v * vs + q];
+        for (int m = 1; m < a.nx; ++m) acc += a.w[m] * a.X[m][v * vs + q];
+        if (a.R) acc += cR * a.R[v * vs + q];
+        acc += cr * r[v];
+        un[v] = acc;
+      }
+    }
+    if (a.Rout) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) a.Rout[v * vs + q] = r[v];
+    }
+    if constexpr (kRes) {
+      const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                                       a.Vg[4 * vs + q], a.gamma);
+      double P[kSrrmhdPrims], A[kSrrmhdAux];
+      cs.add(srrmhd_c2p_cell(un, a.gamma, a.sigma, Zg, P, A));
+#pragma unroll
+      for (int v = 0; v < kSrrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+#pragma unroll
+      for (int v = 0; v < kSrrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+      if (a.Aout) {
+#pragma unroll
+        for (int v = 0; v < kSrrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+      }
+    } else {
+      double v2, Z;
+      srmhd_plain_guess(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q], a.Vg[4 * vs + q],
+                        a.gamma, v2, Z);
+      const double BS = un[5] * un[1] + un[6] * un[2] + un[7] * un[3];
+      const double Bsq = un[5] * un[5] + un[6] * un[6] + un[7] * un[7];
+      const double Ssq = un[1] * un[1] + un[2] * un[2] + un[3] * un[3];
+      const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
+      double P[kSrmhdPrims], A[kSrmhdAux];
+      srmhd_recover(un, a.gamma, v2, Z, P, A);
+#pragma unroll
+      for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+#pragma unroll
+      for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+      if (a.Aout) {
+#pragma unroll
+        for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+      }
+      cs.add(its);
+      if (its < 0) {
+        const int slot = atomicAdd(a.fail_count, 1);
+        a.fail_list[slot] = q;
+        a.fail_flag[q] = 1;
+      }
+    }
+  }
+  reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rk410_prepare2(long long n, double *U1, double *U2) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= n) return;
+  const double u2 = 1.0 / 25.0 * U2[q] + 9.0 / 25.0 * U1[q];
+  U2[q] = u2;
+  U1[q] = 15.0 * u2 - 5.0 * U1[q];
+}
+
 __device__ __forceinline__ long long padded_image(const Geo &g, const PadGeo &pg, long long p) {
   const int I = (int)(p / ((long long)pg.N[1] * pg.N[2]));
   const int J = (int)((p / pg.N[2]) % pg.N[1]);
@@ -1014,7 +1184,10 @@ template <class M, int ORDER, int DAX, int CAX, bool DFAST, int MODE>
 void launch_sweep_t(const SweepArgs &a, int quirk, hipStream_t s) {
   using S = SweepShape<M, ORDER, DAX, CAX, DFAST>;
   const int tiles = ((a.geo.n[DAX] + S::TL - 1) / S::TL) * ((a.geo.n[CAX] + S::TC - 1) / S::TC) * a.geo.n[S::EAX];
-  if (quirk && DAX == 2)
+  // stored fluxes already carry the z quirk (k_flux_stored)
+  if (M::kStored)
+    hipLaunchKernelGGL((k_sweep<M, ORDER, DAX, CAX, DFAST, MODE, false>), dim3(tiles), dim3(kThreads), 0, s, a);
+  else if (quirk && DAX == 2)
     hipLaunchKernelGGL((k_sweep<M, ORDER, DAX, CAX, DFAST, MODE, true>), dim3(tiles), dim3(kThreads), 0, s, a);
   else
     hipLaunchKernelGGL((k_sweep<M, ORDER, DAX, CAX, DFAST, MODE, false>), dim3(tiles), dim3(kThreads), 0, s, a);
@@ -1034,6 +1207,11 @@ void launch_sweep_o(const SweepArgs &a, int dir, int quirk, hipStream_t s) {
 
 template <class M, int ORDER>
 void launch_sweep_m(const SweepArgs &a, int dir, int mode, int quirk, hipStream_t s) {
+  if constexpr (M::kStored) {
+    if (mode == kSweepWrite) launch_sweep_o<M, ORDER, kSweepWrite>(a, dir, 0, s);
+    else if (mode == kSweepAdd) launch_sweep_o<M, ORDER, kSweepAdd>(a, dir, 0, s);
+    return;
+  }
   switch (mode) {
     case kSweepWrite: launch_sweep_o<M, ORDER, kSweepWrite>(a, dir, quirk, s); break;
     case kSweepAdd: launch_sweep_o<M, ORDER, kSweepAdd>(a, dir, quirk, s); break;
@@ -1073,7 +1251,17 @@ void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStr
 }
 
 void launch_sweep(const SweepArgs &a, int model, int order, int dir, int mode, int quirk, hipStream_t s) {
-  if (model == 0) {
+  if (model == 2) {
+    if (order == 3)
+      launch_sweep_m<StoredM<SrmhdM>, 3>(a, dir, mode, quirk, s);
+    else
+      launch_sweep_m<StoredM<SrmhdM>, 5>(a, dir, mode, quirk, s);
+  } else if (model == 3) {
+    if (order == 3)
+      launch_sweep_m<StoredM<SrrmhdM>, 3>(a, dir, mode, quirk, s);
+    else
+      launch_sweep_m<StoredM<SrrmhdM>, 5>(a, dir, mode, quirk, s);
+  } else if (model == 0) {
     if (order == 3)
       launch_sweep_m<SrmhdM, 3>(a, dir, mode, quirk, s);
     else
@@ -1122,6 +1310,24 @@ void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double d
                          double cp, C2PCounters *st, hipStream_t s) {
   hipLaunchKernelGGL(k_split_srrmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, dt, gam, sigma,
                      cp, st);
+}
+
+void launch_rkplus_stage(const RkStageArgs &a, int model, hipStream_t s) {
+  const dim3 grid(blocks_for(a.geo.vstride)), block(kThreads);
+  if (model == 0)
+    hipLaunchKernelGGL(k_rkplus_stage<SrmhdM>, grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_rkplus_stage<SrrmhdM>, grid, block, 0, s, a);
+}
+
+void launch_flux_stored(const Geo &g, int model, const double *U, const double *V, const double *A, double *F,
+                        int quirk, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_stored, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, model, U, V, A, F, quirk);
+}
+
+void launch_rk410_prepare2(const Geo &g, int ncons, double *U1, double *U2, hipStream_t s) {
+  const long long n = (long long)ncons * g.vstride;
+  hipLaunchKernelGGL(k_rk410_prepare2, dim3(blocks_for(n)), dim3(kThreads), 0, s, n, U1, U2);
 }
 
 void launch_zero_int(int *p, hipStream_t s) { hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p); }

@@ -49,9 +49,19 @@ struct SweepArgs {
   int *fail_count;
   long long *fail_list;
   unsigned char *fail_flag;
+  const double *Fin;  // stored flux vectors [dir][var][cell] (model ids 2, 3)
 };
 
+// model: 0 SRMHD, 1 SRRMHD, or 2 / 3 for the same models with the flux vector
+// of every cell read from a.Fin instead of formed from (U, V) (write and add
+// modes only; the per-cell split f+- still uses U).
 void launch_sweep(const SweepArgs &a, int model, int order, int dir, int mode, int quirk, hipStream_t s);
+// Flux vectors of every interior cell and active direction into
+// F[dir][var][cell], formed as Model::fluxVector forms them from the stored
+// cons, prims and aux (srmhd.cc:72-206, srrmhd.cc:65-192) even when those
+// arrays do not belong to one state.
+void launch_flux_stored(const Geo &g, int model, const double *U, const double *V, const double *A, double *F,
+                        int quirk, hipStream_t s);
 // Fused z and y sweeps of a 3-D grid: writes their summed net flux to a.fnet.
 void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStream_t s);
 void launch_imex1_srrmhd(const SweepArgs &a, hipStream_t s);
@@ -80,6 +90,35 @@ void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double d
 // (bit patterns of non-negative doubles; out must be zero on entry).
 void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, unsigned long long *out,
                                 hipStream_t s);
+// One RKPlus stage over interior cells (RKPlus.cc:56-726): with
+// rhs = source(Ub, Ab) - fnet (RKPlus::rhs, RKPlus.cc:15-50; no rhs when fnet
+// is null), the stage state is
+//   half_sum: 0.5 (X0 + X1 + dt rhs)                  (RK2B's corrector)
+//   else:     sum_m w[m] Xm + (wR dt) R + (wr dt) rhs
+// optionally keeping rhs in Rout, then cons2prim from the primitives Vg into
+// (Uout, Vout, Aout); Aout may be null. Uout may alias X0 and Vout Vg.
+struct RkStageArgs {
+  Geo geo;
+  const double *X[4];
+  double w[4];
+  int nx, half_sum;
+  const double *Ub, *Ab, *fnet;
+  double wr;
+  const double *R;
+  double wR;
+  double *Rout;
+  const double *Vg;
+  double *Uout, *Vout, *Aout;
+  double dt, gamma, sigma, cp;
+  C2PCounters *stats;
+  int *fail_count;
+  long long *fail_list;
+  unsigned char *fail_flag;
+};
+void launch_rkplus_stage(const RkStageArgs &a, int model, hipStream_t s);
+// RK4_10::prepare2's cons update (RKPlus.cc:628-633) over interior cells:
+// U2 = U2/25 + 9 U1/25, then U1 = 15 U2 - 5 U1.
+void launch_rk410_prepare2(const Geo &g, int ncons, double *U1, double *U2, hipStream_t s);
 void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s);
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s);

@@ -11,27 +11,16 @@ namespace mgpu {
 
 constexpr int kSrrmhdCons = 14, kSrrmhdPrims = 11, kSrrmhdAux = 17;
 
-// Flux vector in direction DIR from the cell's cons and (rho, vx, vy, vz, p)
-// (SRRMHD::fluxVector, srrmhd.cc:65-192). rho h W^2, B^2, E^2 and J are formed
-// from the primitives as SRRMHD::primsToAll / getPrimitiveVarsSingleCell do
-// (srrmhd.cc:303-311,380-408). QUIRK: in z, the Sy flux takes the Sz formula
-// and the Sz flux is the y-direction Sz flux left in the reference's shared
-// flux buffer (srrmhd.cc:163-171 after the y call).
+// Flux vector in direction DIR (SRRMHD::fluxVector, srrmhd.cc:65-192) from the
+// cell's cons (D, S, E, B, psi, phi), velocity and pressure, Z = rho h W^2,
+// hs = (B^2 + E^2) / 2 and the current J along DIR. QUIRK: in z, the Sy flux
+// takes the Sz formula and the Sz flux is the y-direction Sz flux left in the
+// reference's shared flux buffer (srrmhd.cc:163-171 after the y call).
 template <int DIR, bool QUIRK>
-__device__ __forceinline__ void srrmhd_flux(const double u[14], double rho, double vx, double vy, double vz, double p,
-                                            double gam, double sigma, double f[14]) {
+__device__ __forceinline__ void srrmhd_flux_core(const double u[14], double vx, double vy, double vz, double p,
+                                                 double Z, double hs, double J, double f[14]) {
   const double Bx = u[5], By = u[6], Bz = u[7], Ex = u[8], Ey = u[9], Ez = u[10];
-  const double v2 = vx * vx + vy * vy + vz * vz;
-  const double W = 1.0 / sqrt(1.0 - v2);
-  const double e = p / (rho * (gam - 1.0));
-  const double h = 1.0 + e * gam;
-  const double Z = rho * h * W * W;
-  const double hs = 0.5 * (Bx * Bx + By * By + Bz * Bz + Ex * Ex + Ey * Ey + Ez * Ez);
-  const double vE = vx * Ex + vy * Ey + vz * Ez;
-  const double q = u[13];
-  const double ws = W * sigma;
   if (DIR == 0) {
-    const double Jx = q * vx + ws * (Ex + (vy * Bz - vz * By) - vE * vx);
     f[0] = u[0] * vx;
     f[1] = Z * vx * vx + p - Ex * Ex - Bx * Bx + hs;
     f[2] = Z * vx * vy - Ex * Ey - Bx * By;
@@ -45,9 +34,8 @@ __device__ __forceinline__ void srrmhd_flux(const double u[14], double rho, doub
     f[10] = -By;
     f[11] = Ex;
     f[12] = Bx;
-    f[13] = Jx;
+    f[13] = J;
   } else if (DIR == 1) {
-    const double Jy = q * vy + ws * (Ey + (vz * Bx - vx * Bz) - vE * vy);
     f[0] = u[0] * vy;
     f[1] = Z * vx * vy - Ey * Ex - By * Bx;
     f[2] = Z * vy * vy + p - Ey * Ey - By * By + hs;
@@ -61,9 +49,8 @@ __device__ __forceinline__ void srrmhd_flux(const double u[14], double rho, doub
     f[10] = Bx;
     f[11] = Ey;
     f[12] = By;
-    f[13] = Jy;
+    f[13] = J;
   } else {
-    const double Jz = q * vz + ws * (Ez + (vx * By - vy * Bx) - vE * vz);
     f[0] = u[0] * vz;
     f[1] = Z * vx * vz - Ez * Ex - Bz * Bx;
     const double szz = Z * vz * vz + p - Ez * Ez - Bz * Bz + hs;
@@ -79,8 +66,40 @@ __device__ __forceinline__ void srrmhd_flux(const double u[14], double rho, doub
     f[10] = u[11];
     f[11] = Ez;
     f[12] = Bz;
-    f[13] = Jz;
+    f[13] = J;
   }
+}
+
+
+// The flux with Z, B^2, E^2 and J formed from the primitives as
+// SRRMHD::primsToAll / getPrimitiveVarsSingleCell do (srrmhd.cc:303-311,380-408).
+template <int DIR, bool QUIRK>
+__device__ __forceinline__ void srrmhd_flux(const double u[14], double rho, double vx, double vy, double vz, double p,
+                                            double gam, double sigma, double f[14]) {
+  const double Bx = u[5], By = u[6], Bz = u[7], Ex = u[8], Ey = u[9], Ez = u[10];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double W = 1.0 / sqrt(1.0 - v2);
+  const double e = p / (rho * (gam - 1.0));
+  const double h = 1.0 + e * gam;
+  const double Z = rho * h * W * W;
+  const double hs = 0.5 * (Bx * Bx + By * By + Bz * Bz + Ex * Ex + Ey * Ey + Ez * Ez);
+  const double vE = vx * Ex + vy * Ey + vz * Ez;
+  const double q = u[13];
+  const double ws = W * sigma;
+  const double J = DIR == 0   ? q * vx + ws * (Ex + (vy * Bz - vz * By) - vE * vx)
+                   : DIR == 1 ? q * vy + ws * (Ey + (vz * Bx - vx * Bz) - vE * vy)
+                              : q * vz + ws * (Ez + (vx * By - vy * Bx) - vE * vz);
+  srrmhd_flux_core<DIR, QUIRK>(u, vx, vy, vz, p, Z, hs, J, f);
+}
+
+// The flux exactly as SRRMHD::fluxVector forms it from stored arrays: Z, B^2,
+// E^2 and J taken from aux (rhohWsq, Bsq, Esq, J; srrmhd.cc:78-99), E and B from
+// the cons, velocity and pressure from the prims. Equal to srrmhd_flux when the
+// three arrays belong to one state.
+template <int DIR, bool QUIRK>
+__device__ __forceinline__ void srrmhd_flux_stored(const double u[14], const double P[11], const double A[17],
+                                                   double f[14]) {
+  srrmhd_flux_core<DIR, QUIRK>(u, P[1], P[2], P[3], P[4], A[10], 0.5 * (A[7] + A[8]), A[4 + DIR], f);
 }
 
 // Residual of the recovery equation in Z = rho h W^2 (srrmhd.cc:512-537).

@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmethod_gpu.so")
 
 SRMHD, SRRMHD = 0, 1
 RK2, SSP2, RKSPLIT = 0, 1, 2
+RK2B, RK3, RK4, RK4_10 = 3, 4, 5, 6  # RKPlus family (RKPlus.h:35-515)
 OUTFLOW, PERIODIC = 0, 1
 
 EXPORTED = [

@@ -936,6 +936,175 @@ long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *au
   return fails;
 }
 
+/* ------------------------------------------------------- RKPlus family */
+
+typedef struct {
+  double *cons, *prims, *aux;
+} ostate;
+
+static ostate state_new(const oc_config *c) {
+  geom g = mkgeom(c);
+  ostate s;
+  s.cons = (double *)calloc(g.ntot * oc_ncons(c->model), sizeof(double));
+  s.prims = (double *)calloc(g.ntot * oc_nprims(c->model), sizeof(double));
+  s.aux = (double *)calloc(g.ntot * oc_naux(c->model), sizeof(double));
+  return s;
+}
+
+static void state_free(ostate *s) {
+  free(s->cons);
+  free(s->prims);
+  free(s->aux);
+}
+
+/* the guess arrays of a stage: prims and aux of another state, every cell */
+static void copy_guess(const oc_config *c, ostate *dst, const double *prims, const double *aux) {
+  geom g = mkgeom(c);
+  memcpy(dst->prims, prims, sizeof(double) * g.ntot * oc_nprims(c->model));
+  memcpy(dst->aux, aux, sizeof(double) * g.ntot * oc_naux(c->model));
+}
+
+/* RKPlus::rhs (RKPlus.cc:15-50) without a model extension: source - F over
+ * the interior. */
+static void rkplus_rhs(const oc_config *c, const double *cons, const double *prims, const double *aux,
+                       double *rhs) {
+  geom g = mkgeom(c);
+  int nc = oc_ncons(c->model);
+  double *flux = (double *)calloc(g.ntot * nc, sizeof(double));
+  double *src = (double *)calloc(g.ntot * nc, sizeof(double));
+  oc_fvs_F(c, cons, prims, aux, flux);
+  oc_source(c, cons, prims, aux, src);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) rhs[IX(v, i, j, k)] = src[IX(v, i, j, k)] - flux[IX(v, i, j, k)];
+  free(flux);
+  free(src);
+}
+
+/* out = sum_m w[m] x[m] over the interior, n terms (x may alias out). */
+static void lincomb(const oc_config *c, double *out, int n, const double *const *x, const double *w) {
+  geom g = mkgeom(c);
+  int nc = oc_ncons(c->model);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) {
+          long q = IX(v, i, j, k);
+          double acc = w[0] * x[0][q];
+          for (int m = 1; m < n; m++) acc += w[m] * x[m][q];
+          out[q] = acc;
+        }
+}
+
+/* One step of RK2B, RK3, RK4 or RK4_10 (RKPlus.cc:56-726). Each stage forms
+ * its state from earlier states and dt * rhs, then finalises it; the guess of
+ * a stage's cons2prim is the previous stage's prims (the final stage's are
+ * the step's own). RK4_10's second preparation (RKPlus.cc:612-640) resets the
+ * stage prims and aux to the step's own while replacing the cons, so the next
+ * rhs reads those prims with the new cons, as the reference does; with quirks
+ * off the new stage state is finalised (cons2prim from those prims) first. */
+long oc_rkplus_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, int method,
+                    oc_stats *st) {
+  geom g = mkgeom(c);
+  int nc = oc_ncons(c->model);
+  size_t cb = sizeof(double) * g.ntot * nc;
+  long fails = 0;
+  double *r1 = (double *)calloc(g.ntot * nc, sizeof(double));
+  double *r2 = (double *)calloc(g.ntot * nc, sizeof(double));
+  ostate u1 = state_new(c), u2 = state_new(c), u3 = state_new(c), u4 = state_new(c);
+  if (method == OC_RK2B) {
+    /* RK2B::stage1/stage2/step (RKPlus.cc:83-136) */
+    copy_guess(c, &u1, prims, aux);
+    rkplus_rhs(c, cons, prims, aux, r1);
+    { const double *x[2] = {cons, r1}; double w[2] = {1.0, dt}; lincomb(c, u1.cons, 2, x, w); }
+    fails += finalise(c, u1.cons, u1.prims, u1.aux, st);
+    rkplus_rhs(c, u1.cons, u1.prims, u1.aux, r2);
+    for (int v = 0; v < nc; v++)
+      for (int i = g.is; i < g.ie; i++)
+        for (int j = g.js; j < g.je; j++)
+          for (int k = g.ks; k < g.ke; k++) {
+            long q = IX(v, i, j, k);
+            cons[q] = 0.5 * (cons[q] + u1.cons[q] + dt * r2[q]);
+          }
+    fails += finalise(c, cons, prims, aux, st);
+  } else if (method == OC_RK3) {
+    /* RK3::stage1/2/3/step (RKPlus.cc:177-274) */
+    copy_guess(c, &u1, prims, aux);
+    rkplus_rhs(c, cons, prims, aux, r1);
+    { const double *x[2] = {cons, r1}; double w[2] = {1.0, dt}; lincomb(c, u1.cons, 2, x, w); }
+    fails += finalise(c, u1.cons, u1.prims, u1.aux, st);
+    copy_guess(c, &u2, u1.prims, u1.aux);
+    rkplus_rhs(c, u1.cons, u1.prims, u1.aux, r1);
+    { const double *x[3] = {cons, u1.cons, r1}; double w[3] = {0.75, 0.25, 0.25 * dt}; lincomb(c, u2.cons, 3, x, w); }
+    fails += finalise(c, u2.cons, u2.prims, u2.aux, st);
+    rkplus_rhs(c, u2.cons, u2.prims, u2.aux, r1);
+    { const double *x[3] = {cons, u2.cons, r1}; double w[3] = {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt};
+      lincomb(c, cons, 3, x, w); }
+    fails += finalise(c, cons, prims, aux, st);
+  } else if (method == OC_RK4) {
+    /* RK4::stage1..5/step (RKPlus.cc:330-470): the five-stage fourth-order
+     * strong-stability-preserving scheme with the reference's coefficients */
+    copy_guess(c, &u1, prims, aux);
+    rkplus_rhs(c, cons, prims, aux, r1);
+    { const double *x[2] = {cons, r1}; double w[2] = {1.0, 0.391752226571890 * dt}; lincomb(c, u1.cons, 2, x, w); }
+    fails += finalise(c, u1.cons, u1.prims, u1.aux, st);
+    copy_guess(c, &u2, u1.prims, u1.aux);
+    rkplus_rhs(c, u1.cons, u1.prims, u1.aux, r1);
+    { const double *x[3] = {cons, u1.cons, r1};
+      double w[3] = {0.444370493651235, 0.555629506348765, 0.368410593050371 * dt}; lincomb(c, u2.cons, 3, x, w); }
+    fails += finalise(c, u2.cons, u2.prims, u2.aux, st);
+    copy_guess(c, &u3, u2.prims, u2.aux);
+    rkplus_rhs(c, u2.cons, u2.prims, u2.aux, r1);
+    { const double *x[3] = {cons, u2.cons, r1};
+      double w[3] = {0.620101851488403, 0.379898148511597, 0.251891774271694 * dt}; lincomb(c, u3.cons, 3, x, w); }
+    fails += finalise(c, u3.cons, u3.prims, u3.aux, st);
+    copy_guess(c, &u4, u3.prims, u3.aux);
+    rkplus_rhs(c, u3.cons, u3.prims, u3.aux, r1); /* rhs4, kept for stage 5 */
+    { const double *x[3] = {cons, u3.cons, r1};
+      double w[3] = {0.178079954393132, 0.821920045606868, 0.544974750228521 * dt}; lincomb(c, u4.cons, 3, x, w); }
+    fails += finalise(c, u4.cons, u4.prims, u4.aux, st);
+    rkplus_rhs(c, u4.cons, u4.prims, u4.aux, r2);
+    { const double *x[5] = {u2.cons, u3.cons, u4.cons, r1, r2};
+      double w[5] = {0.517231671970585, 0.096059710526147, 0.386708617503269, 0.063692468666290 * dt,
+                     0.226007483236906 * dt};
+      lincomb(c, cons, 5, x, w); }
+    fails += finalise(c, cons, prims, aux, st);
+  } else {
+    /* RK4_10::prepare1/prepare2/stageRepeat/stageFinal/step (RKPlus.cc:586-726):
+     * ten stages of dt/6, the fourth-order ten-stage strong-stability-
+     * preserving scheme */
+    memcpy(u1.cons, cons, cb);
+    memcpy(u2.cons, cons, cb);
+    copy_guess(c, &u1, prims, aux);
+    for (int s = 0; s < 9; s++) {
+      if (s == 5) {
+        copy_guess(c, &u1, prims, aux);
+        for (long n = 0; n < g.ntot * nc; n++) {
+          u2.cons[n] = 1.0 / 25.0 * u2.cons[n] + 9.0 / 25.0 * u1.cons[n];
+          u1.cons[n] = 15.0 * u2.cons[n] - 5.0 * u1.cons[n];
+        }
+        /* without the quirk the new stage state gets its own primitives */
+        if (!c->quirks) fails += finalise(c, u1.cons, u1.prims, u1.aux, st);
+      }
+      rkplus_rhs(c, u1.cons, u1.prims, u1.aux, r1);
+      { const double *x[2] = {u1.cons, r1}; double w[2] = {1.0, 1.0 / 6.0 * dt}; lincomb(c, u1.cons, 2, x, w); }
+      fails += finalise(c, u1.cons, u1.prims, u1.aux, st);
+    }
+    rkplus_rhs(c, u1.cons, u1.prims, u1.aux, r1);
+    { const double *x[3] = {u2.cons, u1.cons, r1}; double w[3] = {1.0, 3.0 / 5.0, 1.0 / 10.0 * dt};
+      lincomb(c, cons, 3, x, w); }
+    fails += finalise(c, cons, prims, aux, st);
+  }
+  state_free(&u1);
+  state_free(&u2);
+  state_free(&u3);
+  state_free(&u4);
+  free(r1);
+  free(r2);
+  return fails;
+}
+
 /* ----------------------------------------------------------- SSP2 IMEX */
 
 /* IMEX SSP2(222) constants as the reference's SSP2 object ends up holding them

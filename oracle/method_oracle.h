@@ -90,6 +90,10 @@ long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, d
 long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
 /* One RKSplit step (rkSplit.cc:31-53): RK2, split source, finalise. */
 long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
+/* RKPlus family (RKPlus.cc): method OC_RK2B, OC_RK3, OC_RK4 or OC_RK4_10. */
+enum { OC_RK2B = 0, OC_RK3 = 1, OC_RK4 = 2, OC_RK4_10 = 3 };
+long oc_rkplus_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, int method,
+                    oc_stats *st);
 
 #ifdef __cplusplus
 }

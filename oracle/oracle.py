@@ -20,6 +20,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 SRMHD = 0
 SRRMHD = 1
+# RKPlus family (RKPlus.h:35-515): method ids of oc_rkplus_step
+RKPLUS = {"rk2b": 0, "rk3": 1, "rk4": 2, "rk4_10": 3}
 
 
 class OcConfig(ctypes.Structure):
@@ -74,6 +76,8 @@ def lib():
         L.oc_ssp2_step.restype = ctypes.c_long
         L.oc_rksplit_step.argtypes = [cp, dp, dp, dp, ctypes.c_double, sp]
         L.oc_rksplit_step.restype = ctypes.c_long
+        L.oc_rkplus_step.argtypes = [cp, dp, dp, dp, ctypes.c_double, ctypes.c_int, sp]
+        L.oc_rkplus_step.restype = ctypes.c_long
         L.oc_srmhd_max_speed.argtypes = [ctypes.c_double] * 6
         L.oc_srmhd_max_speed.restype = ctypes.c_double
         L.oc_max_wavespeed.argtypes = [cp, dp, dp, dp]
@@ -193,6 +197,10 @@ class Oracle:
 
     def step(self, cons, prims, aux, dt: float, integrator: str = "rk2"):
         st = OcStats()
+        if integrator in RKPLUS:
+            fails = self.L.oc_rkplus_step(ctypes.byref(self.c), _p(cons), _p(prims), _p(aux), dt,
+                                          RKPLUS[integrator], ctypes.byref(st))
+            return fails, st
         fn = {"rk2": self.L.oc_rk2_step, "ssp2": self.L.oc_ssp2_step, "rksplit": self.L.oc_rksplit_step}[integrator]
         fails = fn(ctypes.byref(self.c), _p(cons), _p(prims), _p(aux), dt, ctypes.byref(st))
         return fails, st

@@ -14,7 +14,8 @@
 //                Model::getPrimitiveVars once (srmhd.cc:390-562,
 //                srrmhd.cc:314-356); dumps prims and aux.
 //
-// Options: integrator=rk2 | ssp2 | rksplit (RK2, SSP2, RKSplit); init=file
+// Options: integrator=rk2 | ssp2 | rksplit | rk2b | rk3 | rk4 | rk4_10 (RK2,
+// SSP2, RKSplit and the RKPlus family, RKPlus.h:35-515); init=file
 // (prims_in=...) or one of the reference's initial states: kh
 // (KHInstabilitySingleFluid, mag=...), khrandom (KHRandomInstabilitySingleFluid,
 // mag=..., seed=...), ot (OTVortexSingleFluid), briowu (BrioWuSingleFluid,
@@ -39,6 +40,7 @@
 #include "RK2.h"
 #include "SSP2.h"
 #include "rkSplit.h"
+#include "RKPlus.h"
 #include "serialSaveData.h"
 #include "fluxVectorSplitting.h"
 #include "serialEnv.h"
@@ -299,8 +301,18 @@ int main(int argc, char *argv[]) {
     ti = new RK2(&data, model, bcs, &fvs);
   else if (integ == "rksplit")
     ti = new RKSplit(&data, model, bcs, &fvs);
-  else
+  else if (integ == "rk2b")
+    ti = new RK2B(&data, model, bcs, &fvs);
+  else if (integ == "rk3")
+    ti = new RK3(&data, model, bcs, &fvs);
+  else if (integ == "rk4")
+    ti = new RK4(&data, model, bcs, &fvs);
+  else if (integ == "rk4_10")
+    ti = new RK4_10(&data, model, bcs, &fvs);
+  else if (integ == "ssp2")
     ti = new SSP2(&data, model, bcs, &fvs);
+  else
+    throw std::invalid_argument("unknown integrator " + integ);
 
   sim.set(init, model, ti, bcs, &fvs);
 

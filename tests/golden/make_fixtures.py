@@ -219,6 +219,22 @@ def reference_examples():
               per3, 10, 4.0 / 3.0, 0.6, sigma=10.0, cp=1.0)
 
 
+def rkplus_cases():
+    """The RKPlus integrators (RKPlus.cc): RK2B, RK3, RK4 and RK4_10 on the
+    divergent-Bx KH state (so phi's source acts) and two resistive set-ups
+    with a conductivity the explicit source step resolves (sigma dt < 0.1)."""
+    gs = idt.kh2d_grid(32)
+    for integ in ("rk2b", "rk3", "rk4", "rk4_10"):
+        run_case(f"kh2d_32_weno3_{integ}", gs, kh_divb_prims(gs), "srmhd", 3, integ,
+                 ("periodic", "outflow", "periodic"), 4, [1, 4], cp=0.1, interior_only=True)
+    g1 = idt.Grid(100, 0, 0, 0.0, 1.0, -1.0, 1.0, 0.0, 1.0)
+    init_case("briowu_100_srrmhd_rk3", g1, ["init=briowu", "endTime=0.4"], "srrmhd", "rk3",
+              ("outflow", "outflow", "outflow"), 20, 2.0, 0.4, sigma=10.0)
+    go = idt.ot2d_grid(32)
+    run_case("ot2d_32_weno3_rk4", go, idt.ot_prims_resistive(go), "srrmhd", 3, "rk4",
+             ("periodic", "periodic", "periodic"), 3, [1, 3], cfl=0.5, sigma=10.0, cp=0.1, interior_only=True)
+
+
 def main():
     if not os.path.exists(REF_BIN):
         raise SystemExit(f"{REF_BIN} missing: build it with `make -C oracle/ref`")
@@ -243,6 +259,7 @@ def main():
              ("periodic", "outflow", "periodic"), 8, [8], cp=0.1)
     kh_example_save("kh_example_32_rksplit_save")
     reference_examples()
+    rkplus_cases()
 
 
 if __name__ == "__main__":

@@ -327,8 +327,69 @@ def test_rksplit_steps_match_reference(golden):
     assert np.abs(rc[8] - z["cons8"][8]).max() > 1e-5
 
 
+RKPLUS_FIXTURES = [("kh2d_32_weno3_rk2b", orc.SRMHD, "rk2b"), ("kh2d_32_weno3_rk3", orc.SRMHD, "rk3"),
+                   ("kh2d_32_weno3_rk4", orc.SRMHD, "rk4"), ("kh2d_32_weno3_rk4_10", orc.SRMHD, "rk4_10"),
+                   ("ot2d_32_weno3_rk4", orc.SRRMHD, "rk4")]
+
+
+@pytest.mark.parametrize("name, model, integrator", RKPLUS_FIXTURES)
+def test_rkplus_steps_match_reference(golden, name, model, integrator):
+    """RK2B, RK3, RK4 and RK4_10 (RKPlus.cc:56-726), rhs = source - F, on the
+    divergent-Bx KH state and on resistive Orszag-Tang, against the reference
+    build after the first and the last step."""
+    z = golden(name)
+    nx, ny, nz, Ng, weno, steps = (int(v) for v in z["meta"])
+    b = z["bounds"]
+    gam, cfl, sig, cp = (float(v) for v in z["params"])
+    case = orc.Case(model=model, weno_order=weno, periodic=tuple(int(v) for v in z["periodic"]), gamma=gam,
+                    cfl=cfl, sigma=sig, cp=cp, nx=nx, ny=ny, nz=nz, xmin=b[0], xmax=b[1], ymin=b[2], ymax=b[3])
+    o = orc.Oracle(case)
+    cons, prims, aux = o.set(z["prims0"])
+    cn = parity.SRMHD_CONS if model == orc.SRMHD else parity.SRRMHD_CONS
+    pn = parity.SRMHD_PRIMS if model == orc.SRMHD else parity.SRRMHD_PRIMS
+    t = 0.0
+    for it in range(steps):
+        dt = o.timestep(t, it)
+        assert dt == z["dts"][it]
+        fails, _ = o.step(cons, prims, aux, dt, integrator)
+        assert fails == 0
+        t += dt
+        if it + 1 in (1, steps):
+            for got, ref, names in ((cons, z[f"cons{it + 1}"], cn), (prims, z[f"prims{it + 1}"], pn)):
+                rows = parity.field_errors(parity.interior(got, 4, 2), ref, names)
+                assert max(r[4] for r in rows) <= 1e-9, parity.report(f"{name} step {it + 1}", rows)
+
+
+def test_rkplus_orders_differ_and_rk4_10_reset_matters(golden):
+    """The four schemes give different states after one step (so no
+    integrator silently falls back to another), and RK4_10's reset of the stage
+    primitives at its second preparation (RKPlus.cc:612-640) changes the
+    result: the oracle restated without it (fresh cons2prim of the new stage
+    state) moves away from the reference."""
+    z = golden("kh2d_32_weno3_rk2b")
+    grid = dict(nx=32, ny=32, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0)
+    outs = {}
+    for integ in ("rk2b", "rk3", "rk4", "rk4_10"):
+        o = orc.Oracle(orc.Case(weno_order=3, periodic=(1, 0, 1), cp=0.1, **grid))
+        c, p, a, dts, _ = o.run(z["prims0"], 1, integrator=integ)
+        outs[integ] = c
+    names = list(outs)
+    for m in range(len(names)):
+        for n in range(m + 1, len(names)):
+            assert np.abs(outs[names[m]] - outs[names[n]]).max() > 1e-12, (names[m], names[n])
+    z10 = golden("kh2d_32_weno3_rk4_10")
+    rows = {}
+    for quirks in (1, 0):
+        o = orc.Oracle(orc.Case(weno_order=3, periodic=(1, 0, 1), cp=0.1, quirks=quirks, **grid))
+        c, p, a, dts, _ = o.run(z10["prims0"], 1, integrator="rk4_10")
+        rows[quirks] = parity.field_errors(parity.interior(c, 4, 2), z10["cons1"], parity.SRMHD_CONS)
+    assert max(r[4] for r in rows[1]) <= 1e-9
+    assert max(r[4] for r in rows[0]) > 1e-7
+
+
 EXAMPLE_FIXTURES = [("briowu_100_rksplit", orc.SRMHD, "rksplit"), ("briowu_100_ssp2", orc.SRRMHD, "ssp2"),
-                    ("ot_64_rksplit", orc.SRMHD, "rksplit"), ("khrandom_64_rksplit", orc.SRMHD, "rksplit")]
+                    ("ot_64_rksplit", orc.SRMHD, "rksplit"), ("khrandom_64_rksplit", orc.SRMHD, "rksplit"),
+                    ("briowu_100_srrmhd_rk3", orc.SRRMHD, "rk3")]
 
 
 def example_case(z, model):
