@@ -3,6 +3,7 @@
 #include <stdexcept>
 
 #include "RK2.h"
+#include "RKPlus.h"
 #include "rkSplit.h"
 #include "SSP2.h"
 #include "gpu_link.h"
@@ -69,3 +70,47 @@ RKSplit::RKSplit(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, Mod
 }
 
 void RKSplit::setSource(double *cons, double *prims, double *aux) { model->sourceTerm(cons, prims, aux, data->source); }
+
+RKPlus::RKPlus(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : TimeIntegrator(data, model, bcs, fluxMethod, ext) {
+  // the engine has no model extensions (RKPlus::rhs adds their source when
+  // one is given, RKPlus.cc:26-38)
+  if (ext) throw std::invalid_argument("RKPlus: model extensions are not provided by the GPU engine");
+  fluxCont = new double[(size_t)data->Nx * data->Ny * data->Nz * data->Ncons]();
+}
+
+RKPlus::~RKPlus() { delete[] fluxCont; }
+
+void RKPlus::rhs(double *cons, double *prims, double *aux, double *rhsVec) {
+  Data *d = data;
+  fluxMethod->F(cons, prims, aux, d->f, fluxCont);
+  model->sourceTerm(cons, prims, aux, d->source);
+  for (int var = 0; var < d->Ncons; ++var)
+    for (int i = d->is; i < d->ie; ++i)
+      for (int j = d->js; j < d->je; ++j)
+        for (int k = d->ks; k < d->ke; ++k) rhsVec[ID(var, i, j, k)] = d->source[ID(var, i, j, k)] - fluxCont[ID(var, i, j, k)];
+}
+
+void RKPlus::step(double *cons, double *prims, double *aux, double dt) {
+  engine_step(this, cons, prims, aux, dt, "RKPlus::step");
+}
+
+RK2B::RK2B(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : RKPlus(data, model, bcs, fluxMethod, ext) {
+  data->gpu.integrator = MGPU_RK2B;
+}
+
+RK3::RK3(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : RKPlus(data, model, bcs, fluxMethod, ext) {
+  data->gpu.integrator = MGPU_RK3;
+}
+
+RK4::RK4(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : RKPlus(data, model, bcs, fluxMethod, ext) {
+  data->gpu.integrator = MGPU_RK4;
+}
+
+RK4_10::RK4_10(Data *data, Model *model, Bcs *bcs, FluxMethod *fluxMethod, ModelExtension *ext)
+    : RKPlus(data, model, bcs, fluxMethod, ext) {
+  data->gpu.integrator = MGPU_RK4_10;
+}
