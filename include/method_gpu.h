@@ -121,6 +121,20 @@ int mgpu_flux_vector(mgpu_engine *e, int dir, double *f);
  * state, written for every padded cell. */
 int mgpu_source(mgpu_engine *e, double *source);
 
+/* Model extensions (ModelExtension, modelExtension.h:14-59): MGPU_EXT_REGIME
+ * is REGIME (REGIME.h:81, REGIME.cc), the resistive extension of ideal SRMHD,
+ * whose source RKSplit::setSource (rkSplit.cc:15-25) and RKPlus::rhs
+ * (RKPlus.cc:26-38) add to the model's; the other integrators ignore it, as
+ * the reference's do. Replaces constructing REGIME(data, fluxMethod) and
+ * handing it to the integrator's constructor. SRMHD engines only. */
+enum { MGPU_EXT_NONE = 0, MGPU_EXT_REGIME = 1 };
+int mgpu_set_model_extension(mgpu_engine *e, int kind);
+
+/* REGIME::sourceExtension (REGIME.cc:83-138) of the resident state in the
+ * reference layout: variables 0..7 over the interior cells; phi and the
+ * ghost cells are zero. Needs the extension set. */
+int mgpu_source_extension(mgpu_engine *e, double *source);
+
 /* Model::getPrimitiveVars (srmhd.cc:390-562, srrmhd.cc:314-356) on the
  * resident cons with the resident prims as the starting guess. */
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);

@@ -24,7 +24,7 @@ INCLUDE = os.path.join(ROOT, "include")
 GPU_LIB = os.path.join(LIBDIR, "libmethod_gpu.so")
 HOST_LIB = os.path.join(LIBDIR, "libmethod_host.so")
 
-GPU_SOURCES = ["mgpu_kernels.hip", "mgpu_engine.hip", "mgpu_mlp.hip", "mgpu_batch.hip"]
+GPU_SOURCES = ["mgpu_kernels.hip", "mgpu_engine.hip", "mgpu_mlp.hip", "mgpu_batch.hip", "mgpu_regime.hip"]
 ARCH = "gfx950"
 
 

@@ -69,6 +69,8 @@ struct mgpu_engine {
   // the current from aux) and RK4's kept stage-four rhs
   double *rkU[4] = {}, *rkV[4] = {}, *rkA[4] = {}, *rkR = nullptr;
   double *rkF = nullptr;  // RK4_10: stored flux vectors [dir][var][cell]
+  int extension = MGPU_EXT_NONE;         // model extension (REGIME) added by RKSplit / RKPlus
+  double *rg_work = nullptr, *rg_src = nullptr;  // its padded work arrays and interior source
   double *staging = nullptr;
   size_t staging_elems = 0;
   C2PCounters *dstats = nullptr;
@@ -267,13 +269,28 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   return run_sweeps(e, c, kSweepImexFinal);
 }
 
-// RKSplit::step (rkSplit.cc:31-53): RK2::step, then cons += dt source and
-// finalise (cons2prim, with the SRMHD retry).
+// The model extension's source of the state with prims V into e->rg_src
+// (REGIME::sourceExtension, REGIME.cc:83-138), or null without an extension.
+const double *extension_source(mgpu_engine *e, const double *V) {
+  if (e->extension != MGPU_EXT_REGIME) return nullptr;
+  const double dx[3] = {e->dx, e->dy, e->dz};
+  launch_regime_source(e->geo, e->pad, dx, V, e->rg_work, e->rg_src, e->cfg.gamma, e->cfg.sigma,
+                       e->cfg.weno_order, e->stream);
+  mark(e, "regime_source");
+  return e->rg_src;
+}
+
+// RKSplit::step (rkSplit.cc:31-53): RK2::step, then cons += dt source (plus
+// the model extension's, rkSplit.cc:15-25) and finalise (cons2prim, with the
+// SRMHD retry).
 int rksplit_step(mgpu_engine *e, double dt, C2PCounters *st) {
   if (rk2_step(e, dt, st)) return -1;
-  if (e->cfg.model == MGPU_SRMHD)
+  if (e->cfg.model == MGPU_SRMHD) {
+    const double *ext = extension_source(e, e->V);
+    HIP_OK(hipGetLastError());
     launch_split_srmhd(e->geo, e->U, e->V, e->A, dt, e->cfg.gamma, e->cfg.cp, st, e->fail_count, e->fail_list,
-                       e->fail_flag, e->stream);
+                       e->fail_flag, ext, e->stream);
+  }
   else
     launch_split_srrmhd(e->geo, e->U, e->V, e->A, dt, e->cfg.gamma, e->cfg.sigma, e->cfg.cp, st, e->stream);
   HIP_OK(hipGetLastError());
@@ -306,6 +323,8 @@ int rk_rhs(mgpu_engine *e, RkStageArgs &a, const double *Ub, const double *Vb, c
   a.Ub = Ub;
   a.Ab = Ab;
   a.fnet = e->fnet;
+  a.ext = extension_source(e, Vb);
+  HIP_OK(hipGetLastError());
   return 0;
 }
 
@@ -327,6 +346,8 @@ int rk_rhs_stored(mgpu_engine *e, RkStageArgs &a, const double *Ub, const double
   a.Ub = Ub;
   a.Ab = Ab;
   a.fnet = e->fnet;
+  a.ext = extension_source(e, Vb);
+  HIP_OK(hipGetLastError());
   return 0;
 }
 
@@ -629,6 +650,8 @@ int mgpu_destroy(mgpu_engine *e) {
   }
   if (e->rkR) hipFree(e->rkR);
   if (e->rkF) hipFree(e->rkF);
+  if (e->rg_work) hipFree(e->rg_work);
+  if (e->rg_src) hipFree(e->rg_src);
   if (e->dstats) hipFree(e->dstats);
   if (e->dspeed) hipFree(e->dspeed);
   if (e->fail_count) hipFree(e->fail_count);
@@ -747,6 +770,37 @@ int mgpu_flux_vector(mgpu_engine *e, int dir, double *f) {
                               e->cfg.sigma, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(f, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_set_model_extension(mgpu_engine *e, int kind) {
+  if (!e) return fail("null engine");
+  if (kind != MGPU_EXT_NONE && kind != MGPU_EXT_REGIME) return fail("mgpu_set_model_extension: unknown kind %d", kind);
+  if (kind == MGPU_EXT_REGIME && e->cfg.model != MGPU_SRMHD)
+    return fail("mgpu_set_model_extension: REGIME extends the ideal SRMHD model only");
+  if (kind == MGPU_EXT_REGIME && !e->rg_work) {
+    const size_t nw = regime_work_doubles(e->pad), ns = (size_t)8 * e->geo.vstride;
+    if (hipMalloc(&e->rg_work, nw * sizeof(double)) != hipSuccess ||
+        hipMalloc(&e->rg_src, ns * sizeof(double)) != hipSuccess)
+      return fail("hipMalloc of the REGIME work arrays failed");
+  }
+  e->extension = kind;
+  return 0;
+}
+
+int mgpu_source_extension(mgpu_engine *e, double *source) {
+  if (!e || !source) return fail("null argument");
+  if (e->extension != MGPU_EXT_REGIME) return fail("mgpu_source_extension: no model extension is set");
+  if (ensure_staging(e)) return -1;
+  extension_source(e, e->V);
+  HIP_OK(hipGetLastError());
+  // variables 0..7 of the reference's 9-variable source array; phi's stays 0
+  HIP_OK(hipMemsetAsync(e->staging, 0, sizeof(double) * e->ncons * e->pad.ntot, e->stream));
+  launch_expand(e->geo, e->pad, 8, e->rg_src, e->staging, 1, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(source, e->staging, sizeof(double) * e->ncons * e->pad.ntot, hipMemcpyDeviceToHost,
+                        e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
   return 0;
 }

@@ -870,11 +870,13 @@ __global__ __launch_bounds__(kThreads) void k_source_srmhd(Geo g, PadGeo pg, con
 
 // RKSplit's source half-step and finalise (rkSplit.cc:36-53): cons += dt
 // source(cons) with SRMHD's only source, -phi / cp^2 on phi (srmhd.cc:233-250),
-// then cons2prim from the current primitives (srmhd.cc:390-562). The other
-// conserved variables are unchanged, so only phi is written back.
+// plus the model extension's source on variables 0..7 when ext is given, then
+// cons2prim from the current primitives (srmhd.cc:390-562). Without an
+// extension only phi changes, so only phi is written back.
 __global__ __launch_bounds__(kThreads) void k_split_srmhd(Geo g, double *U, double *V, double *A, double dt,
                                                            double gam, double cp, C2PCounters *st, int *fail_count,
-                                                           long long *fail_list, unsigned char *fail_flag) {
+                                                           long long *fail_list, unsigned char *fail_flag,
+                                                           const double *ext) {
   const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
   int cells = 0, iters = 0, fails = 0, maxit = 0;
   if (q < g.vstride) {
@@ -885,6 +887,15 @@ __global__ __launch_bounds__(kThreads) void k_split_srmhd(Geo g, double *U, doub
     const double src = -u[8] / (cp * cp);
     u[8] = u[8] + dt * src;
     U[8 * vs + q] = u[8];
+    if (ext) {
+      // the model extension's source on variables 0..7 (rkSplit.cc:15-25,
+      // where the model's own source there is zero)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        u[v] = u[v] + dt * ext[v * vs + q];
+        U[v * vs + q] = u[v];
+      }
+    }
     double v2, Z;
     srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
     const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
@@ -1023,6 +1034,11 @@ __global__ __launch_bounds__(kThreads) void k_rkplus_stage(RkStageArgs a) {
         for (int m = 0; m < 5; ++m) r[8 + m] = s5[m];
       } else {
         r[8] = -a.Ub[8 * vs + q] / (a.cp * a.cp);
+        if (a.ext) {
+          // the model extension's source (RKPlus.cc:26-38) on variables 0..7
+#pragma unroll
+          for (int v = 0; v < 8; ++v) r[v] = r[v] + a.ext[v * vs + q];
+        }
       }
 #pragma unroll
       for (int v = 0; v < NC; ++v) r[v] = r[v] - a.fnet[v * vs + q];
@@ -1301,9 +1317,9 @@ void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, 
 
 void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double cp,
                         C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
-                        hipStream_t s) {
+                        const double *ext, hipStream_t s) {
   hipLaunchKernelGGL(k_split_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, dt, gam, cp, st,
-                     fail_count, fail_list, fail_flag);
+                     fail_count, fail_list, fail_flag, ext);
 }
 
 void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double sigma,

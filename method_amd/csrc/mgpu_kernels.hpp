@@ -83,7 +83,7 @@ void launch_zero_int(int *p, hipStream_t s);
 // RKSplit's source half-step and cons2prim over interior cells.
 void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double cp,
                         C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
-                        hipStream_t s);
+                        const double *ext, hipStream_t s);
 void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double sigma,
                          double cp, C2PCounters *st, hipStream_t s);
 // Per-axis maximum fast magnetosonic speed over interior cells into out[3]
@@ -107,6 +107,7 @@ struct RkStageArgs {
   const double *R;
   double wR;
   double *Rout;
+  const double *ext;  // SRMHD model-extension source of the rhs state (variables 0..7), or null
   const double *Vg;
   double *Uout, *Vout, *Aout;
   double dt, gamma, sigma, cp;
@@ -119,6 +120,13 @@ void launch_rkplus_stage(const RkStageArgs &a, int model, hipStream_t s);
 // RK4_10::prepare2's cons update (RKPlus.cc:628-633) over interior cells:
 // U2 = U2/25 + 9 U1/25, then U1 = 15 U2 - 5 U1.
 void launch_rk410_prepare2(const Geo &g, int ncons, double *U1, double *U2, hipStream_t s);
+// REGIME::sourceExtension (REGIME.cc:83-138) of the prims V into src
+// (variables 0..7, interior cells), formed on the padded grid with the
+// reference's loop ranges; work holds regime_work_doubles(pg) doubles; dx are
+// the grid spacings.
+size_t regime_work_doubles(const PadGeo &pg);
+void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], const double *V, double *work,
+                          double *src, double gam, double sigma, int order, hipStream_t s);
 void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s);
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s);

@@ -1,0 +1,312 @@
+// The REGIME resistive extension of ideal SRMHD (REGIME.cc; Wright & Hawke
+// 2019) on the GPU: its source term, added by RKSplit::setSource
+// (rkSplit.cc:15-25) and RKPlus::rhs (RKPlus.cc:26-38).
+//
+// The reference forms the source on the ghost-padded grid with loop ranges
+// that leave zeros near the grid edge (q outside [2, N-2), the E-flux
+// reconstruction outside [shift, N-shift), K outside [1, N-1)). Those zeros
+// reach the first and last interior cells through the centred differences,
+// so the kernels here work on the same padded index space: a padded cell
+// reads its primitives from the interior cell Bcs::apply copies there, and
+// every range test is the reference's. Five launches over padded cells:
+//   E = -v x B                                    (REGIME.cc:150-154)
+//   q = 4th-order centred div E where it fits     (REGIME.cc:186-212)
+//   per axis, the WENO face flux of (E, B-flux)   (REGIME.cc:293-370)
+//   K, then per axis D = M K, M = -df/dw dw/ds_b  (REGIME.cc:226-290,551-774)
+//   source = centred difference of D, interior    (REGIME.cc:92-138)
+#include <hip/hip_runtime.h>
+
+#include "mgpu_device.hpp"
+#include "mgpu_kernels.hpp"
+
+namespace mgpu {
+namespace {
+
+constexpr int kRegimeThreads = 256;
+
+// the grid spacings as the reference's Data holds them (divisions use dx
+// itself, not a reciprocal)
+struct Spacing {
+  double d[3];
+};
+
+__device__ __forceinline__ void padded_pos(const PadGeo &pg, long long p, int pos[3]) {
+  pos[0] = (int)(p / ((long long)pg.N[1] * pg.N[2]));
+  pos[1] = (int)((p / pg.N[2]) % pg.N[1]);
+  pos[2] = (int)(p % pg.N[2]);
+}
+
+// interior cell whose values Bcs::apply copies into padded position pos
+__device__ __forceinline__ long long image_of(const Geo &g, const PadGeo &pg, const int pos[3]) {
+  long long q = 0;
+  for (int ax = 0; ax < 3; ++ax) q += (long long)remap(pos[ax] - pg.off[ax], g.n[ax], g.periodic[ax]) * g.stride[ax];
+  return q;
+}
+
+__device__ __forceinline__ long long pstride(const PadGeo &pg, int ax) {
+  return ax == 0 ? (long long)pg.N[1] * pg.N[2] : (ax == 1 ? pg.N[2] : 1);
+}
+
+__device__ __forceinline__ void load_prims(const double *V, long long vs, long long q, double P[8]) {
+#pragma unroll
+  for (int v = 0; v < 8; ++v) P[v] = V[v * vs + q];
+}
+
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, const double *V, double *E) {
+  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  int pos[3];
+  padded_pos(pg, p, pos);
+  double P[8];
+  load_prims(V, g.vstride, image_of(g, pg, pos), P);
+  E[p] = -(P[2] * P[7] - P[3] * P[6]);
+  E[pg.ntot + p] = -(P[3] * P[5] - P[1] * P[7]);
+  E[2 * pg.ntot + p] = -(P[1] * P[6] - P[2] * P[5]);
+}
+
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_q(Geo g, PadGeo pg, Spacing sp, const double *E,
+                                                             double *Q) {
+  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  int pos[3];
+  padded_pos(pg, p, pos);
+  bool in = true;
+  for (int ax = 0; ax < g.dims; ++ax) in = in && pos[ax] >= 2 && pos[ax] < pg.N[ax] - 2;
+  double acc = 0.0;
+  if (in) {
+    const double *dd = sp.d;
+    for (int ax = 0; ax < g.dims; ++ax) {
+      const double *Ea = E + ax * pg.ntot;
+      const long long s = pstride(pg, ax);
+      const double t = (-Ea[p + 2 * s] + 8 * Ea[p + s] - 8 * Ea[p - s] + Ea[p - 2 * s]) / (12 * dd[ax]);
+      acc = ax == 0 ? t : acc + t;
+    }
+  }
+  Q[p] = acc;
+}
+
+// B-flux of the stiff E equations along axis ax (REGIME.cc:299-303,324-328,349-353)
+__device__ __forceinline__ void stiff_flux(int ax, const double P[8], double fb[3]) {
+  if (ax == 0) {
+    fb[0] = 0.0;
+    fb[1] = P[7];
+    fb[2] = -P[6];
+  } else if (ax == 1) {
+    fb[0] = -P[7];
+    fb[1] = 0.0;
+    fb[2] = P[5];
+  } else {
+    fb[0] = P[6];
+    fb[1] = -P[5];
+    fb[2] = 0.0;
+  }
+}
+
+// FVS::fluxReconstruction of (cons = E, f = fb) along axis ax with 3
+// variables (fluxVectorSplitting.cc:4-56): the face value at the left face of
+// each padded cell with [shift, N - shift) along ax, zero elsewhere.
+template <int ORDER>
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_flux(Geo g, PadGeo pg, const double *V, const double *E,
+                                                                int ax, double *F) {
+  constexpr int R = Weno<ORDER>::R;
+  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  int pos[3];
+  padded_pos(pg, p, pos);
+  if (pos[ax] < R || pos[ax] >= pg.N[ax] - R) {
+    for (int v = 0; v < 3; ++v) F[v * pg.ntot + p] = 0.0;
+    return;
+  }
+  const long long s = pstride(pg, ax);
+  double fp[3][2 * R], fm[3][2 * R];
+  for (int m = 0; m < 2 * R; ++m) {
+    int cp[3] = {pos[0], pos[1], pos[2]};
+    cp[ax] += m - R;
+    double P[8], fb[3];
+    load_prims(V, g.vstride, image_of(g, pg, cp), P);
+    stiff_flux(ax, P, fb);
+    const long long pc = p + (m - R) * s;
+    for (int v = 0; v < 3; ++v) {
+      const double e = E[v * pg.ntot + pc];
+      fp[v][m] = 0.5 * (fb[v] + e);
+      fm[v][m] = 0.5 * (fb[v] - e);
+    }
+  }
+  for (int v = 0; v < 3; ++v)
+    F[v * pg.ntot + p] = Weno<ORDER>::face([&](int m) { return fp[v][m]; }, [&](int m) { return fm[v][m]; });
+}
+
+// dw/ds_b (12 x 3, rows 0, 4 and 11 zero) times alpha (REGIME.cc:226-290)
+__device__ __forceinline__ void regime_dwdsb(double q, double sig, const double P[8], const double E[3], double alpha,
+                                             double dw[12][3]) {
+  const double qsigsq = q * q + sig * sig, sigcu = sig * sig * sig, qch = q;
+  for (int l = 0; l < 12; ++l)
+    for (int m = 0; m < 3; ++m) dw[l][m] = 0.0;
+  dw[1][0] = -qch * (qch * qch + sig * sig * (1 + P[5] * P[5]));
+  dw[1][1] = -sig * (P[5] * P[6] * qch * sig - P[7] * qsigsq);
+  dw[1][2] = -sig * (P[5] * P[7] * qch * sig + P[6] * qsigsq);
+  dw[2][0] = -sig * (P[5] * P[6] * qch * sig + P[7] * qsigsq);
+  dw[2][1] = -qch * (qch * qch + sig * sig * (1 + P[6] * P[6]));
+  dw[2][2] = -sig * (P[6] * P[7] * qch * sig - P[5] * qsigsq);
+  dw[3][0] = -sig * (P[5] * P[7] * qch * sig - P[6] * qsigsq);
+  dw[3][1] = -sig * (P[6] * P[7] * qch * sig + P[5] * qsigsq);
+  dw[3][2] = -qch * (qch * qch + sig * sig * (1 + P[7] * P[7]));
+  dw[5][0] = -P[5] * sigcu * E[0];
+  dw[5][1] = -P[6] * sigcu * E[0] - sig * qsigsq * P[3];
+  dw[5][2] = -P[7] * sigcu * E[0] + sig * qsigsq * P[2];
+  dw[6][0] = -P[5] * sigcu * E[1] + sig * qsigsq * P[3];
+  dw[6][1] = -P[6] * sigcu * E[1];
+  dw[6][2] = -P[7] * sigcu * E[1] - sig * qsigsq * P[1];
+  dw[7][0] = -P[5] * sigcu * E[2] - sig * qsigsq * P[2];
+  dw[7][1] = -P[6] * sigcu * E[2] + sig * qsigsq * P[1];
+  dw[7][2] = -P[7] * sigcu * E[2];
+  dw[8][0] = -P[5] * P[5] * sigcu - sig * qsigsq;
+  dw[8][1] = -P[5] * P[6] * sigcu;
+  dw[8][2] = -P[5] * P[7] * sigcu;
+  dw[9][0] = -P[6] * P[5] * sigcu;
+  dw[9][1] = -P[6] * P[6] * sigcu - sig * qsigsq;
+  dw[9][2] = -P[6] * P[7] * sigcu;
+  dw[10][0] = -P[7] * P[5] * sigcu;
+  dw[10][1] = -P[7] * P[6] * sigcu;
+  dw[10][2] = -P[7] * P[7] * sigcu - sig * qsigsq;
+  for (int l = 0; l < 12; ++l)
+    for (int m = 0; m < 3; ++m) dw[l][m] *= alpha;
+}
+
+// Rows 1..4, columns 5..10 of df/dw along axis ax (REGIME.cc:404-548), and
+// its (row, ax+1) entries of rows 0 and 4.
+__device__ __forceinline__ void regime_dfdw(int ax, double gam, const double P[8], const double E[3], double J[5][6],
+                                            double &j0, double &j4) {
+  for (int l = 0; l < 5; ++l)
+    for (int n = 0; n < 6; ++n) J[l][n] = 0.0;
+  // column index n - 5 for w entries 5..10 (B, E)
+  if (ax == 0) {
+    j0 = P[0];
+    J[1][0] = -P[5]; J[1][1] = P[6]; J[1][2] = P[7]; J[1][3] = -E[0]; J[1][4] = E[1]; J[1][5] = E[2];
+    J[2][0] = -P[6]; J[2][1] = -P[5]; J[2][3] = -E[1]; J[2][4] = -E[0];
+    J[3][0] = -P[7]; J[3][2] = -P[5]; J[3][3] = -E[2]; J[3][5] = -E[0];
+    J[4][1] = -E[2]; J[4][2] = E[1]; J[4][4] = P[7]; J[4][5] = -P[6];
+  } else if (ax == 1) {
+    j0 = P[0];
+    J[1][0] = -P[6]; J[1][1] = -P[5]; J[1][3] = -E[1]; J[1][4] = -E[0];
+    J[2][0] = P[5]; J[2][1] = -P[6]; J[2][2] = P[7]; J[2][3] = E[0]; J[2][4] = -E[1]; J[2][5] = E[2];
+    J[3][1] = -P[7]; J[3][2] = -P[6]; J[3][4] = -E[2]; J[3][5] = -E[1];
+    J[4][0] = E[2]; J[4][2] = -E[0]; J[4][3] = -P[7]; J[4][5] = P[5];
+  } else {
+    j0 = P[0];
+    J[1][0] = -P[7]; J[1][2] = -P[5]; J[1][3] = -E[2]; J[1][5] = -E[0];
+    J[2][1] = -P[7]; J[2][2] = -P[6]; J[2][4] = -E[2]; J[2][5] = -E[1];
+    J[3][0] = P[5]; J[3][1] = P[6]; J[3][2] = -P[7]; J[3][3] = E[0]; J[3][4] = E[1]; J[3][5] = -E[2];
+    J[4][0] = -E[1]; J[4][1] = E[0]; J[4][3] = P[6]; J[4][4] = -P[5];
+  }
+  j4 = gam * P[4] / (gam - 1);
+}
+
+// K (REGIME.cc:293-370), then D = M K per active axis with M as
+// REGIME::set_Dx/Dy/Dz form it (REGIME.cc:551-774).
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, Spacing sp, const double *V,
+                                                             const double *E, const double *Q, const double *F,
+                                                             double *D, double gam, double sig) {
+  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const long long nt = pg.ntot;
+  int pos[3];
+  padded_pos(pg, p, pos);
+  const double *dd = sp.d;
+  double K[3] = {0.0, 0.0, 0.0};
+  for (int ax = 0; ax < g.dims; ++ax) {
+    if (pos[ax] < 1 || pos[ax] >= pg.N[ax] - 1) continue;
+    const long long s = pstride(pg, ax);
+    const double *Fa = F + (long long)ax * 3 * nt;
+    for (int v = 0; v < 3; ++v) K[v] += Fa[v * nt + p + s] / (2 * dd[ax]) - Fa[v * nt + p - s] / (2 * dd[ax]);
+  }
+  double P[8];
+  load_prims(V, g.vstride, image_of(g, pg, pos), P);
+  const double Ec[3] = {E[p], E[nt + p], E[2 * nt + p]};
+  const double q = Q[p];
+  const double qs = q * q + sig * sig;
+  const double alpha = 1 / (qs * (qs + sig * sig * (P[5] * P[5] + P[6] * P[6] + P[7] * P[7])));
+  double dw[12][3];
+  regime_dwdsb(q, sig, P, Ec, alpha, dw);
+  for (int ax = 0; ax < g.dims; ++ax) {
+    double J[5][6], j0, j4;
+    regime_dfdw(ax, gam, P, Ec, J, j0, j4);
+    const int a = 1 + ax;
+    double M[8][3];
+    for (int l = 0; l < 8; ++l)
+      for (int m = 0; m < 3; ++m) M[l][m] = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      M[0][m] -= j0 * dw[a][m];
+      M[4][m] -= j4 * dw[a][m];
+    }
+    // the induction rows with one unit entry each
+    if (ax == 0) {
+      for (int m = 0; m < 3; ++m) M[6][m] -= (-dw[10][m]);
+      for (int m = 0; m < 3; ++m) M[7][m] -= dw[9][m];
+    } else if (ax == 1) {
+      for (int m = 0; m < 3; ++m) M[5][m] -= dw[10][m];
+      for (int m = 0; m < 3; ++m) M[7][m] -= (-dw[8][m]);
+    } else {
+      for (int m = 0; m < 3; ++m) M[5][m] -= (-dw[9][m]);
+      for (int m = 0; m < 3; ++m) M[6][m] -= dw[8][m];
+    }
+    for (int l = 1; l < 5; ++l)
+      for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 6; ++n) M[l][m] -= J[l][n] * dw[5 + n][m];
+    double *Da = D + (long long)ax * 8 * nt;
+    for (int l = 0; l < 8; ++l) {
+      double acc = 0.0;
+      for (int m = 0; m < 3; ++m) acc += M[l][m] * K[m];
+      Da[l * nt + p] = acc;
+    }
+  }
+}
+
+// sourceExtension over interior cells (REGIME.cc:92-138): variables 0..7.
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_source(Geo g, PadGeo pg, Spacing sp, const double *D,
+                                                                  double *src) {
+  const long long q = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  const long long nt = pg.ntot;
+  const int c[3] = {(int)(q / g.stride[0]), (int)((q / g.stride[1]) % g.n[1]), (int)(q % g.n[2])};
+  const long long p =
+      ((long long)(c[0] + pg.off[0]) * pg.N[1] + (c[1] + pg.off[1])) * pg.N[2] + (c[2] + pg.off[2]);
+  const double *dd = sp.d;
+  for (int v = 0; v < 8; ++v) {
+    double acc = 0.0;
+    for (int ax = 0; ax < g.dims; ++ax) {
+      const long long s = pstride(pg, ax);
+      const double *Dv = D + ((long long)ax * 8 + v) * nt;
+      const double t = (Dv[p + s] - Dv[p - s]) / (2 * dd[ax]);
+      acc = ax == 0 ? t : acc + t;
+    }
+    src[v * g.vstride + q] = acc;
+  }
+}
+
+unsigned blocks(long long n) { return (unsigned)((n + kRegimeThreads - 1) / kRegimeThreads); }
+
+}  // namespace
+
+size_t regime_work_doubles(const PadGeo &pg) { return (size_t)pg.ntot * (3 + 1 + 3 * 3 + 3 * 8); }
+
+void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], const double *V, double *work,
+                          double *src, double gam, double sigma, int order, hipStream_t s) {
+  const long long nt = pg.ntot;
+  const Spacing sp = {{dx[0], dx[1], dx[2]}};
+  double *E = work, *Q = E + 3 * nt, *F = Q + nt, *D = F + 9 * nt;
+  hipLaunchKernelGGL(k_regime_e, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E);
+  hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, E, Q);
+  for (int ax = 0; ax < g.dims; ++ax) {
+    if (order == 3)
+      hipLaunchKernelGGL(k_regime_flux<3>, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E, ax,
+                         F + (long long)ax * 3 * nt);
+    else
+      hipLaunchKernelGGL(k_regime_flux<5>, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E, ax,
+                         F + (long long)ax * 3 * nt);
+  }
+  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, V, E, Q, F, D, gam, sigma);
+  hipLaunchKernelGGL(k_regime_source, dim3(blocks(g.vstride)), dim3(kRegimeThreads), 0, s, g, pg, sp, D, src);
+}
+
+}  // namespace mgpu

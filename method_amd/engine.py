@@ -22,12 +22,13 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmethod_gpu.so")
 SRMHD, SRRMHD = 0, 1
 RK2, SSP2, RKSPLIT = 0, 1, 2
 RK2B, RK3, RK4, RK4_10 = 3, 4, 5, 6  # RKPlus family (RKPlus.h:35-515)
+EXT_NONE, EXT_REGIME = 0, 1  # model extensions (modelExtension.h, REGIME.h:81)
 OUTFLOW, PERIODIC = 0, 1
 
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
     "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_sizes", "mgpu_batch_set_prims", "mgpu_batch_upload_state",
@@ -111,6 +112,8 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_fnet.argtypes = [E, dp]
     L.mgpu_flux_vector.argtypes = [E, ctypes.c_int, dp]
     L.mgpu_source.argtypes = [E, dp]
+    L.mgpu_set_model_extension.argtypes = [E, ctypes.c_int]
+    L.mgpu_source_extension.argtypes = [E, dp]
     L.mgpu_c2p.argtypes = [E, sp]
     L.mgpu_max_wavespeed.argtypes = [E, dp]
     fp = ctypes.POINTER(ctypes.c_float)
@@ -303,6 +306,17 @@ class Engine:
     def source(self) -> np.ndarray:
         out = self.zeros(self.ncons)
         self._check(self.L.mgpu_source(self._h, _dp(out)), "mgpu_source")
+        return out
+
+    def set_model_extension(self, kind: int):
+        """Hand a model extension (EXT_REGIME) to the integrator, as the
+        reference's REGIME(data, fluxMethod) passed to RKSplit / RKPlus."""
+        self._check(self.L.mgpu_set_model_extension(self._h, kind), "mgpu_set_model_extension")
+
+    def source_extension(self) -> np.ndarray:
+        """REGIME::sourceExtension of the resident state (interior cells)."""
+        out = self.zeros(self.ncons)
+        self._check(self.L.mgpu_source_extension(self._h, _dp(out)), "mgpu_source_extension")
         return out
 
     def c2p(self):

@@ -762,9 +762,10 @@ static double recon_down(int order, const double *a, long i0, long stride) {
 /* FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56): Lax-Friedrichs
  * split with alpha = 1, upwind WENO on f+, downwind on f-, summed; entries
  * outside [shift, N-shift) along dir are zero. */
-void oc_flux_reconstruction(const oc_config *c, const double *cons, const double *f, double *frecon, int dir) {
+/* FVS::fluxReconstruction with `nv` variables (the reference's vars
+ * argument, fluxVectorSplitting.cc:13-14). */
+static void flux_recon_n(const oc_config *c, const double *cons, const double *f, double *frecon, int dir, int nv) {
   geom g = mkgeom(c);
-  int nv = oc_ncons(c->model);
   long n = g.ntot * nv;
   double *fp = (double *)calloc(n, sizeof(double));
   double *fm = (double *)calloc(n, sizeof(double));
@@ -790,6 +791,10 @@ void oc_flux_reconstruction(const oc_config *c, const double *cons, const double
         }
   free(fp);
   free(fm);
+}
+
+void oc_flux_reconstruction(const oc_config *c, const double *cons, const double *f, double *frecon, int dir) {
+  flux_recon_n(c, cons, f, frecon, dir, oc_ncons(c->model));
 }
 
 /* FVS::F (fluxVectorSplitting.cc:58-131): per direction fluxVector then
@@ -918,6 +923,22 @@ long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, d
   return fails;
 }
 
+/* The model extension's source added to `src` over the interior
+ * (RKSplit::setSource, rkSplit.cc:15-25; RKPlus::rhs, RKPlus.cc:26-38). */
+static void add_extension_source(const oc_config *c, const double *cons, const double *prims, const double *aux,
+                                 double *src) {
+  if (!c->regime) return;
+  geom g = mkgeom(c);
+  int nc = oc_ncons(c->model);
+  double *ext = (double *)calloc(g.ntot * nc, sizeof(double));
+  oc_regime_source(c, cons, prims, aux, ext);
+  for (int v = 0; v < nc; v++)
+    for (int i = g.is; i < g.ie; i++)
+      for (int j = g.js; j < g.je; j++)
+        for (int k = g.ks; k < g.ke; k++) src[IX(v, i, j, k)] += ext[IX(v, i, j, k)];
+  free(ext);
+}
+
 /* RKSplit::step (rkSplit.cc:31-53): RK2::step, then cons += dt * source over
  * the interior (source from Model::sourceTerm on the whole grid), then
  * finalise. */
@@ -927,6 +948,7 @@ long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *au
   int nc = oc_ncons(c->model);
   double *src = (double *)calloc(g.ntot * nc, sizeof(double));
   oc_source(c, cons, prims, aux, src);
+  add_extension_source(c, cons, prims, aux, src);
   for (int v = 0; v < nc; v++)
     for (int i = g.is; i < g.ie; i++)
       for (int j = g.js; j < g.je; j++)
@@ -934,6 +956,216 @@ long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *au
   free(src);
   fails += finalise(c, cons, prims, aux, st);
   return fails;
+}
+
+/* --------------------------------------------------------------- REGIME */
+
+/* The REGIME resistive extension of ideal SRMHD (REGIME.cc; Wright & Hawke
+ * 2019), restated on the padded grid with the reference's loop ranges. Per
+ * cell: E = -v x B; the charge density q = div E by fourth-order central
+ * differences where the stencil fits ([2, N-2) on every active axis, zero
+ * elsewhere, REGIME.cc:186-212); alpha = 1 / ((q^2+s^2)((q^2+s^2) + s^2 B^2));
+ * dw/ds_b (12 x 3, REGIME.cc:226-290) times alpha; K = the centred difference
+ * of the stiff E-flux reconstruction (fluxReconstruction of E with flux
+ * (0, Bz, -By) etc., REGIME.cc:293-370); per axis M = -df/dw . dw/ds_b
+ * (8 x 3, REGIME.cc:467-774) and D = M K; the source is the centred
+ * difference of D (REGIME.cc:92-138). */
+#define DW(l, m) dw[(l) * 3 + (m)]
+static void regime_dwdsb(double q, double sig, const double *P, const double *E, double alpha, double *dw) {
+  double qsigsq = q * q + sig * sig, sigcu = sig * sig * sig, qch = q;
+  for (int n = 0; n < 36; n++) dw[n] = 0.0;
+  DW(1, 0) = -qch * (qch * qch + sig * sig * (1 + P[5] * P[5]));
+  DW(1, 1) = -sig * (P[5] * P[6] * qch * sig - P[7] * qsigsq);
+  DW(1, 2) = -sig * (P[5] * P[7] * qch * sig + P[6] * qsigsq);
+  DW(2, 0) = -sig * (P[5] * P[6] * qch * sig + P[7] * qsigsq);
+  DW(2, 1) = -qch * (qch * qch + sig * sig * (1 + P[6] * P[6]));
+  DW(2, 2) = -sig * (P[6] * P[7] * qch * sig - P[5] * qsigsq);
+  DW(3, 0) = -sig * (P[5] * P[7] * qch * sig - P[6] * qsigsq);
+  DW(3, 1) = -sig * (P[6] * P[7] * qch * sig + P[5] * qsigsq);
+  DW(3, 2) = -qch * (qch * qch + sig * sig * (1 + P[7] * P[7]));
+  DW(5, 0) = -P[5] * sigcu * E[0];
+  DW(5, 1) = -P[6] * sigcu * E[0] - sig * qsigsq * P[3];
+  DW(5, 2) = -P[7] * sigcu * E[0] + sig * qsigsq * P[2];
+  DW(6, 0) = -P[5] * sigcu * E[1] + sig * qsigsq * P[3];
+  DW(6, 1) = -P[6] * sigcu * E[1];
+  DW(6, 2) = -P[7] * sigcu * E[1] - sig * qsigsq * P[1];
+  DW(7, 0) = -P[5] * sigcu * E[2] - sig * qsigsq * P[2];
+  DW(7, 1) = -P[6] * sigcu * E[2] + sig * qsigsq * P[1];
+  DW(7, 2) = -P[7] * sigcu * E[2];
+  DW(8, 0) = -P[5] * P[5] * sigcu - sig * qsigsq;
+  DW(8, 1) = -P[5] * P[6] * sigcu;
+  DW(8, 2) = -P[5] * P[7] * sigcu;
+  DW(9, 0) = -P[6] * P[5] * sigcu;
+  DW(9, 1) = -P[6] * P[6] * sigcu - sig * qsigsq;
+  DW(9, 2) = -P[6] * P[7] * sigcu;
+  DW(10, 0) = -P[7] * P[5] * sigcu;
+  DW(10, 1) = -P[7] * P[6] * sigcu;
+  DW(10, 2) = -P[7] * P[7] * sigcu - sig * qsigsq;
+  for (int n = 0; n < 36; n++) dw[n] *= alpha;
+}
+
+/* df/dw of axis `dir` (8 x 12, the entries REGIME::set_Dx/Dy/Dz set), then
+ * D = M K with M = -df/dw . dw/ds_b formed as the reference forms it. */
+static void regime_D(int dir, double gam, const double *P, const double *E, const double *dw, const double *K,
+                     double *D) {
+  double J[8][12];
+  memset(J, 0, sizeof J);
+  double M[8][3];
+  memset(M, 0, sizeof M);
+  int a = 1 + dir; /* the velocity / momentum index of the axis */
+  if (dir == 0) {
+    J[0][0] = P[1]; J[0][1] = P[0];
+    J[1][4] = 1; J[1][5] = -P[5]; J[1][6] = P[6]; J[1][7] = P[7]; J[1][8] = -E[0]; J[1][9] = E[1]; J[1][10] = E[2];
+    J[2][5] = -P[6]; J[2][6] = -P[5]; J[2][8] = -E[1]; J[2][9] = -E[0];
+    J[3][5] = -P[7]; J[3][7] = -P[5]; J[3][8] = -E[2]; J[3][10] = -E[0];
+    J[4][1] = gam * P[4] / (gam - 1); J[4][4] = gam * P[1] / (gam - 1);
+    J[4][6] = -E[2]; J[4][7] = E[1]; J[4][9] = P[7]; J[4][10] = -P[6];
+    J[6][10] = -1;
+    J[7][9] = 1;
+  } else if (dir == 1) {
+    J[0][0] = P[2]; J[0][2] = P[0];
+    J[1][5] = -P[6]; J[1][6] = -P[5]; J[1][8] = -E[1]; J[1][9] = -E[0];
+    J[2][4] = 1; J[2][5] = P[5]; J[2][6] = -P[6]; J[2][7] = P[7]; J[2][8] = E[0]; J[2][9] = -E[1]; J[2][10] = E[2];
+    J[3][6] = -P[7]; J[3][7] = -P[6]; J[3][9] = -E[2]; J[3][10] = -E[1];
+    J[4][2] = gam * P[4] / (gam - 1); J[4][4] = gam * P[2] / (gam - 1);
+    J[4][5] = E[2]; J[4][7] = -E[0]; J[4][8] = -P[7]; J[4][10] = P[5];
+    J[5][10] = 1;
+    J[7][8] = -1;
+  } else {
+    J[0][0] = P[3]; J[0][3] = P[0];
+    J[1][5] = -P[7]; J[1][7] = -P[5]; J[1][8] = -E[2]; J[1][10] = -E[0];
+    J[2][6] = -P[7]; J[2][7] = -P[6]; J[2][9] = -E[2]; J[2][10] = -E[1];
+    J[3][4] = 1; J[3][5] = P[5]; J[3][6] = P[6]; J[3][7] = -P[7]; J[3][8] = E[0]; J[3][9] = E[1]; J[3][10] = -E[2];
+    J[4][3] = gam * P[4] / (gam - 1); J[4][4] = gam * P[3] / (gam - 1);
+    J[4][5] = -E[1]; J[4][6] = E[0]; J[4][8] = P[6]; J[4][9] = -P[5];
+    J[5][9] = -1;
+    J[6][8] = 1;
+  }
+  for (int m = 0; m < 3; m++) {
+    M[0][m] -= J[0][a] * DW(a, m);
+    M[4][m] -= J[4][a] * DW(a, m);
+  }
+  /* the two induction rows with a single unit entry */
+  if (dir == 0) {
+    for (int m = 0; m < 3; m++) M[6][m] -= (-DW(10, m));
+    for (int m = 0; m < 3; m++) M[7][m] -= DW(9, m);
+  } else if (dir == 1) {
+    for (int m = 0; m < 3; m++) M[5][m] -= DW(10, m);
+    for (int m = 0; m < 3; m++) M[7][m] -= (-DW(8, m));
+  } else {
+    for (int m = 0; m < 3; m++) M[5][m] -= (-DW(9, m));
+    for (int m = 0; m < 3; m++) M[6][m] -= DW(8, m);
+  }
+  for (int l = 1; l < 5; l++)
+    for (int m = 0; m < 3; m++)
+      for (int n = 5; n < 11; n++) M[l][m] -= J[l][n] * DW(n, m);
+  for (int l = 0; l < 8; l++) {
+    D[l] = 0.0;
+    for (int m = 0; m < 3; m++) D[l] += M[l][m] * K[m];
+  }
+}
+#undef DW
+
+void oc_regime_source(const oc_config *c, const double *cons, const double *prims, const double *aux, double *src) {
+  (void)cons;
+  (void)aux;
+  geom g = mkgeom(c);
+  long ntot = g.ntot;
+  int N[3] = {g.Nx, g.Ny, g.Nz};
+  long st[3] = {(long)g.Ny * g.Nz, g.Nz, 1};
+  double dd[3] = {g.dx, g.dy, g.dz};
+  double sig = c->sigma, gam = c->gamma;
+  double *E = (double *)calloc(3 * ntot, sizeof(double));
+  double *q = (double *)calloc(ntot, sizeof(double));
+  double *alpha = (double *)calloc(ntot, sizeof(double));
+  double *K = (double *)calloc(3 * ntot, sizeof(double));
+  double *fb = (double *)calloc(3 * ntot, sizeof(double));
+  double *fr = (double *)calloc(3 * ntot, sizeof(double));
+  double *D = (double *)calloc(8 * ntot, sizeof(double));
+  /* set_vars (REGIME.cc:141-224) */
+  for (long n = 0; n < ntot; n++) {
+    const double *P0 = prims + n;
+#define PV(v) P0[(long)(v) * ntot]
+    E[0 * ntot + n] = -(PV(2) * PV(7) - PV(3) * PV(6));
+    E[1 * ntot + n] = -(PV(3) * PV(5) - PV(1) * PV(7));
+    E[2 * ntot + n] = -(PV(1) * PV(6) - PV(2) * PV(5));
+  }
+  for (int i = 0; i < g.Nx; i++)
+    for (int j = 0; j < g.Ny; j++)
+      for (int k = 0; k < g.Nz; k++) {
+        int pos[3] = {i, j, k};
+        int in = 1;
+        for (int d = 0; d < g.dims; d++) in &= pos[d] >= 2 && pos[d] < N[d] - 2;
+        if (!in) continue;
+        long n = ((long)i * g.Ny + j) * g.Nz + k;
+        double acc = 0.0;
+        for (int d = 0; d < g.dims; d++) {
+          const double *Ed = E + d * ntot;
+          double t = (-Ed[n + 2 * st[d]] + 8 * Ed[n + st[d]] - 8 * Ed[n - st[d]] + Ed[n - 2 * st[d]]) / (12 * dd[d]);
+          acc = d == 0 ? t : acc + t;
+        }
+        q[n] = acc;
+      }
+  for (long n = 0; n < ntot; n++) {
+    const double *P0 = prims + n;
+    double qs = q[n] * q[n] + sig * sig;
+    alpha[n] = 1 / (qs * (qs + sig * sig * (PV(5) * PV(5) + PV(6) * PV(6) + PV(7) * PV(7))));
+  }
+  /* set_K (REGIME.cc:293-370): the stiff E fluxes reconstructed per axis */
+  for (int d = 0; d < g.dims; d++) {
+    for (long n = 0; n < ntot; n++) {
+      const double *P0 = prims + n;
+      double b[3];
+      if (d == 0) { b[0] = 0; b[1] = PV(7); b[2] = -PV(6); }
+      else if (d == 1) { b[0] = -PV(7); b[1] = 0; b[2] = PV(5); }
+      else { b[0] = PV(6); b[1] = -PV(5); b[2] = 0; }
+      for (int v = 0; v < 3; v++) fb[v * ntot + n] = b[v];
+    }
+    flux_recon_n(c, E, fb, fr, d, 3);
+    for (int v = 0; v < 3; v++)
+      for (int i = 0; i < g.Nx; i++)
+        for (int j = 0; j < g.Ny; j++)
+          for (int k = 0; k < g.Nz; k++) {
+            int pos[3] = {i, j, k};
+            if (pos[d] < 1 || pos[d] >= N[d] - 1) continue;
+            long n = ((long)i * g.Ny + j) * g.Nz + k;
+            K[v * ntot + n] += fr[v * ntot + n + st[d]] / (2 * dd[d]) - fr[v * ntot + n - st[d]] / (2 * dd[d]);
+          }
+  }
+  /* sourceExtension (REGIME.cc:83-138): D per axis, then its centred
+   * difference (= on the first axis, += on the others) */
+  for (int d = 0; d < g.dims; d++) {
+    for (long n = 0; n < ntot; n++) {
+      const double *P0 = prims + n;
+      double P[8], Ec[3], dw[36], Kc[3], Dc[8];
+      for (int v = 0; v < 8; v++) P[v] = PV(v);
+      for (int v = 0; v < 3; v++) {
+        Ec[v] = E[v * ntot + n];
+        Kc[v] = K[v * ntot + n];
+      }
+      regime_dwdsb(q[n], sig, P, Ec, alpha[n], dw);
+      regime_D(d, gam, P, Ec, dw, Kc, Dc);
+      for (int l = 0; l < 8; l++) D[l * ntot + n] = Dc[l];
+    }
+    for (int v = 0; v < 8; v++)
+      for (int i = 0; i < g.Nx; i++)
+        for (int j = 0; j < g.Ny; j++)
+          for (int k = 0; k < g.Nz; k++) {
+            int pos[3] = {i, j, k};
+            if (pos[d] < 1 || pos[d] >= N[d] - 1) continue;
+            long n = ((long)i * g.Ny + j) * g.Nz + k;
+            double t = (D[v * ntot + n + st[d]] - D[v * ntot + n - st[d]]) / (2 * dd[d]);
+            src[v * ntot + n] = d == 0 ? t : src[v * ntot + n] + t;
+          }
+  }
+#undef PV
+  free(E);
+  free(q);
+  free(alpha);
+  free(K);
+  free(fb);
+  free(fr);
+  free(D);
 }
 
 /* ------------------------------------------------------- RKPlus family */
@@ -974,6 +1206,7 @@ static void rkplus_rhs(const oc_config *c, const double *cons, const double *pri
   double *src = (double *)calloc(g.ntot * nc, sizeof(double));
   oc_fvs_F(c, cons, prims, aux, flux);
   oc_source(c, cons, prims, aux, src);
+  add_extension_source(c, cons, prims, aux, src);
   for (int v = 0; v < nc; v++)
     for (int i = g.is; i < g.ie; i++)
       for (int j = g.js; j < g.je; j++)

@@ -26,6 +26,7 @@ typedef struct {
   int weno_order;   /* 3 | 5 */
   int periodic[3];  /* per axis: 1 periodic, 0 outflow */
   int quirks;       /* 1: reproduce the reference's flux-index quirks */
+  int regime;       /* 1: the REGIME model extension's source (SRMHD) */
 } oc_config;
 
 typedef struct {
@@ -90,6 +91,9 @@ long oc_rk2_step(const oc_config *c, double *cons, double *prims, double *aux, d
 long oc_ssp2_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
 /* One RKSplit step (rkSplit.cc:31-53): RK2, split source, finalise. */
 long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, oc_stats *st);
+/* REGIME::sourceExtension (REGIME.cc:83-138) of the given state, every cell
+ * it writes (variables 0..7; variable 8 and the outer layer stay zero). */
+void oc_regime_source(const oc_config *c, const double *cons, const double *prims, const double *aux, double *src);
 /* RKPlus family (RKPlus.cc): method OC_RK2B, OC_RK3, OC_RK4 or OC_RK4_10. */
 enum { OC_RK2B = 0, OC_RK3 = 1, OC_RK4 = 2, OC_RK4_10 = 3 };
 long oc_rkplus_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, int method,

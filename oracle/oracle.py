@@ -32,7 +32,7 @@ class OcConfig(ctypes.Structure):
         ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
         ("gamma", ctypes.c_double), ("sigma", ctypes.c_double), ("cp", ctypes.c_double), ("cfl", ctypes.c_double),
         ("model", ctypes.c_int), ("weno_order", ctypes.c_int),
-        ("periodic", ctypes.c_int * 3), ("quirks", ctypes.c_int),
+        ("periodic", ctypes.c_int * 3), ("quirks", ctypes.c_int), ("regime", ctypes.c_int),
     ]
 
 
@@ -115,11 +115,12 @@ class Case:
     weno_order: int = 3
     periodic: tuple = (1, 0, 1)
     quirks: int = 1
+    regime: int = 0
 
     def cfg(self) -> OcConfig:
         c = OcConfig()
         for f in ("nx", "ny", "nz", "Ng", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax",
-                  "gamma", "sigma", "cp", "cfl", "model", "weno_order", "quirks"):
+                  "gamma", "sigma", "cp", "cfl", "model", "weno_order", "quirks", "regime"):
             setattr(c, f, getattr(self, f))
         for a in range(3):
             c.periodic[a] = int(self.periodic[a])

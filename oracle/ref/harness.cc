@@ -10,6 +10,8 @@
 //                listed in dump=..., plus one text line per step with t, dt.
 //   mode=fnet  : Simulation::set then one FVS::F (fluxVectorSplitting.cc:58-131)
 //                on the initial state; dumps fnet.
+//   mode=regime: Simulation::set then one REGIME::sourceExtension
+//                (REGIME.cc:83-138) of the initial state; dumps it.
 //   mode=c2p   : reads cons, prims and aux (the C2P guess) and calls
 //                Model::getPrimitiveVars once (srmhd.cc:390-562,
 //                srrmhd.cc:314-356); dumps prims and aux.
@@ -19,7 +21,8 @@
 // (prims_in=...) or one of the reference's initial states: kh
 // (KHInstabilitySingleFluid, mag=...), khrandom (KHRandomInstabilitySingleFluid,
 // mag=..., seed=...), ot (OTVortexSingleFluid), briowu (BrioWuSingleFluid,
-// dir=...) (initFunc.cc); save=1 calls SerialSaveData::saveAll
+// dir=...), currentsheet (CurrentSheetSingleFluid, dir=...) (initFunc.cc);
+// regime=1 hands a REGIME model extension to RKSplit / RKPlus; save=1 calls SerialSaveData::saveAll
 // (serialSaveData.cc:8-31) after the steps, writing under ./Data/Final.
 //
 // Boundary conditions: bcx/bcy/bcz = periodic | outflow. When the triple matches
@@ -41,6 +44,7 @@
 #include "SSP2.h"
 #include "rkSplit.h"
 #include "RKPlus.h"
+#include "REGIME.h"
 #include "serialSaveData.h"
 #include "fluxVectorSplitting.h"
 #include "serialEnv.h"
@@ -292,29 +296,44 @@ int main(int argc, char *argv[]) {
     init = new OTVortexSingleFluid(&data);
   else if (init_name == "briowu")
     init = new BrioWuSingleFluid(&data, argi("dir", 0));
+  else if (init_name == "currentsheet")
+    init = new CurrentSheetSingleFluid(&data, argi("dir", 0));
   else
     init = new FromFile(&data, arg("prims_in", ""));
 
+  // regime=1: the REGIME model extension (REGIME.h:81) handed to the integrator
+  REGIME *ext = argi("regime", 0) ? new REGIME(&data, &fvs) : nullptr;
   TimeIntegrator *ti = nullptr;
   const std::string integ = arg("integrator", "rk2");
   if (integ == "rk2")
     ti = new RK2(&data, model, bcs, &fvs);
   else if (integ == "rksplit")
-    ti = new RKSplit(&data, model, bcs, &fvs);
+    ti = new RKSplit(&data, model, bcs, &fvs, ext);
   else if (integ == "rk2b")
-    ti = new RK2B(&data, model, bcs, &fvs);
+    ti = new RK2B(&data, model, bcs, &fvs, ext);
   else if (integ == "rk3")
-    ti = new RK3(&data, model, bcs, &fvs);
+    ti = new RK3(&data, model, bcs, &fvs, ext);
   else if (integ == "rk4")
-    ti = new RK4(&data, model, bcs, &fvs);
+    ti = new RK4(&data, model, bcs, &fvs, ext);
   else if (integ == "rk4_10")
-    ti = new RK4_10(&data, model, bcs, &fvs);
+    ti = new RK4_10(&data, model, bcs, &fvs, ext);
   else if (integ == "ssp2")
     ti = new SSP2(&data, model, bcs, &fvs);
   else
     throw std::invalid_argument("unknown integrator " + integ);
 
   sim.set(init, model, ti, bcs, &fvs);
+
+  if (mode == "regime") {
+    // REGIME::sourceExtension of the set-up state (REGIME.cc:83-138)
+    REGIME *r = ext ? ext : new REGIME(&data, &fvs);
+    r->sourceExtension(data.cons, data.prims, data.aux, data.sourceExtension);
+    FILE *f = fopen((out + "_regime.bin").c_str(), "wb");
+    write_doubles(f, data.sourceExtension, ntot * data.Ncons);
+    fclose(f);
+    dump_state(out + "_step0.bin", &data);
+    return 0;
+  }
 
   if (mode == "fnet") {
     std::vector<double> fnet(ntot * data.Ncons, 0.0);

@@ -53,7 +53,7 @@ def _grid_args(g: idt.Grid, gamma, cfl, weno, integrator, bc, model, sigma=0.0, 
 
 
 def run_case(name, g, prims0, model, weno, integrator, bc, steps, dumps, gamma=4.0 / 3.0, cfl=0.4,
-             sigma=0.0, cp=0.1, interior_only=False, with_fnet=False):
+             sigma=0.0, cp=0.1, interior_only=False, with_fnet=False, extra=()):
     shape = g.shape
     with tempfile.TemporaryDirectory() as td:
         pin = os.path.join(td, "prims.bin")
@@ -61,7 +61,7 @@ def run_case(name, g, prims0, model, weno, integrator, bc, steps, dumps, gamma=4
         base = _grid_args(g, gamma, cfl, weno, integrator, bc, model, sigma, cp)
         out = os.path.join(td, "out")
         _run(base + ["mode=run", f"steps={steps}", f"prims_in={pin}", f"out={out}",
-                     "dump=" + ",".join(str(s) for s in dumps)], td)
+                     "dump=" + ",".join(str(s) for s in dumps)] + list(extra), td)
         log = open(out + "_log.txt").read().split("\n")
         dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
         fail_line = [l for l in log if l.startswith("c2p_calls")]
@@ -235,6 +235,60 @@ def rkplus_cases():
              ("periodic", "periodic", "periodic"), 3, [1, 3], cfl=0.5, sigma=10.0, cp=0.1, interior_only=True)
 
 
+def regime_source_case(name, g, init_args, weno, bc, gamma, sigma, prims0=None):
+    """REGIME::sourceExtension (REGIME.cc:83-138) of a set-up state: the
+    reference's post-set prims and the source over the interior cells."""
+    shape = g.shape
+    with tempfile.TemporaryDirectory() as td:
+        base = _grid_args(g, gamma, 0.4, weno, "rksplit", bc, "srmhd", sigma, 0.1)
+        out = os.path.join(td, "out")
+        extra = list(init_args)
+        if prims0 is not None:
+            pin = os.path.join(td, "prims.bin")
+            np.ascontiguousarray(prims0).tofile(pin)
+            extra.append(f"prims_in={pin}")
+        _run(base + ["mode=regime", f"out={out}", "regime=1"] + extra, td)
+        src = np.fromfile(out + "_regime.bin").reshape((9,) + shape)
+        c, p, a = _load_state(out + "_step0.bin", "srmhd", shape)
+    sl = (slice(None), slice(g.Ng, shape[0] - g.Ng), slice(g.Ng, shape[1] - g.Ng) if g.ny > 0 else slice(None),
+          slice(g.Ng, shape[2] - g.Ng) if g.nz > 0 else slice(None))
+    rec = {"prims0": p, "src": src[sl].copy(), "meta": np.array([g.nx, g.ny, g.nz, g.Ng, weno, 0]),
+           "bounds": np.array([g.xmin, g.xmax, g.ymin, g.ymax, g.zmin, g.zmax]),
+           "params": np.array([gamma, 0.4, sigma, 0.1]),
+           "periodic": np.array([1 if b == "periodic" else 0 for b in bc])}
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **rec)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
+def regime_cases():
+    """The REGIME extension (REGIME.cc): its source on four set-ups, and runs
+    of the reference's two REGIME examples (Examples/REGIME/*Serial/main.cc,
+    RKSplit) plus 2-D WENO5 with RK2B and 3-D with RKSplit."""
+    out3 = ("outflow", "outflow", "outflow")
+    mixed = ("periodic", "outflow", "periodic")
+    gcs = idt.Grid(128, 0, 0, -3.0, 3.0, -1.0, 1.0, -1.5, 1.5)
+    gbw = idt.Grid(128, 0, 0, 0.0, 1.0, -1.0, 1.0, -1.5, 1.5)
+    g2 = idt.kh2d_grid(32)
+    g3 = idt.kh3d_grid(12)
+    p2 = idt.kh_prims(g2, seed=3, shear=0.5, bz=0.3)
+    p3 = idt.kh_prims(g3, seed=4, shear=0.5, bz=0.3)
+    regime_source_case("regime_src_currentsheet_128", gcs, ["init=currentsheet"], 3, out3, 2.0, 1e4)
+    regime_source_case("regime_src_briowu_128", gbw, ["init=briowu"], 3, out3, 2.0, 50.0)
+    regime_source_case("regime_src_kh2d_32_weno5", g2, [], 5, mixed, 4.0 / 3.0, 100.0, p2)
+    regime_source_case("regime_src_kh3d_12_weno3", g3, [], 3, mixed, 4.0 / 3.0, 100.0, p3)
+    # Examples/REGIME/CurrentSheetSerial: gamma 2, sigma 1e4, cfl 0.2, WENO3, Outflow, RKSplit + REGIME
+    init_case("currentsheet_128_regime_rksplit", gcs, ["init=currentsheet", "regime=1", "endTime=7.0"], "srmhd",
+              "rksplit", out3, 20, 2.0, 0.2, sigma=1e4)
+    # Examples/REGIME/BrioWuSerial: gamma 2, sigma 50, cfl 0.15, WENO3, Outflow, RKSplit + REGIME
+    init_case("briowu_128_regime_rksplit", gbw, ["init=briowu", "regime=1", "endTime=0.4"], "srmhd", "rksplit",
+              out3, 20, 2.0, 0.15, sigma=50.0)
+    run_case("kh2d_32_weno5_regime_rk2b", g2, p2, "srmhd", 5, "rk2b", mixed, 3, [1, 3], sigma=100.0,
+             interior_only=True, extra=["regime=1"])
+    run_case("kh3d_12_weno3_regime_rksplit", g3, p3, "srmhd", 3, "rksplit", mixed, 2, [2], sigma=100.0,
+             interior_only=True, extra=["regime=1"])
+
+
 def main():
     if not os.path.exists(REF_BIN):
         raise SystemExit(f"{REF_BIN} missing: build it with `make -C oracle/ref`")
@@ -260,6 +314,7 @@ def main():
     kh_example_save("kh_example_32_rksplit_save")
     reference_examples()
     rkplus_cases()
+    regime_cases()
 
 
 if __name__ == "__main__":

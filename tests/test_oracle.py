@@ -4,6 +4,7 @@ Pins: the reference's own known-answer tests (Tests/CPU/Src/test_weno.cc,
 test_srmhd.cc, test_simulation.cc, test_boundaryConds.cc), the reference build's
 outputs stored under tests/golden/ (make_fixtures.py), and closed forms.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -448,3 +449,81 @@ def test_c2p_retry_from_neighbour_means(quirks):
         assert np.allclose(got, want, rtol=1e-12, atol=1e-14)
         # and the bad cells are not recovered (the quirk is visible)
         assert np.abs(gp[1][bad + (0,)] - prims[1][bad + (0,)]).max() > 1e-3
+
+
+REGIME_SRC = ["regime_src_currentsheet_128", "regime_src_briowu_128", "regime_src_kh2d_32_weno5",
+              "regime_src_kh3d_12_weno3"]
+
+
+def regime_case(z, regime=1):
+    nx, ny, nz, Ng, weno, _ = (int(v) for v in z["meta"])
+    b = z["bounds"]
+    gam, cfl, sig, cp = (float(v) for v in z["params"])
+    return orc.Case(nx=nx, ny=ny, nz=nz, xmin=b[0], xmax=b[1], ymin=b[2], ymax=b[3], zmin=b[4], zmax=b[5],
+                    gamma=gam, cfl=cfl, sigma=sig, cp=cp, weno_order=weno,
+                    periodic=tuple(int(v) for v in z["periodic"]), regime=regime)
+
+
+@pytest.mark.parametrize("name", REGIME_SRC)
+def test_regime_source_matches_reference(golden, name):
+    """REGIME::sourceExtension (REGIME.cc:83-138) of the reference's set-up
+    state, interior cells: 1-D current sheet (sigma 1e4) and Brio-Wu (sigma 50),
+    2-D KH with WENO5 (whose E-flux reconstruction leaves zero faces next to
+    the grid edge that reach the first interior cells) and 3-D KH."""
+    z = golden(name)
+    case = regime_case(z)
+    o = orc.Oracle(case)
+    cons, prims, aux = o.set(z["prims0"])
+    src = np.zeros_like(cons)
+    o.L.oc_regime_source(ctypes.byref(o.c), orc._p(cons), orc._p(prims), orc._p(aux), orc._p(src))
+    dims = 1 if case.ny == 0 else (2 if case.nz == 0 else 3)
+    got = parity.interior(src, 4, dims)
+    ref = z["src"]
+    assert np.abs(ref).max() > 0
+    scale = np.abs(ref).max(axis=tuple(range(1, ref.ndim)), keepdims=True)
+    assert (np.abs(got - ref) / np.maximum(scale, 1e-300)).max() <= 1e-13
+    assert np.all(got[8] == 0.0)
+
+
+def test_regime_source_of_uniform_state_is_zero():
+    """A uniform state at rest has E = 0, q = 0 and K = 0: no resistive source."""
+    grid = dict(nx=16, ny=16, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0)
+    o = orc.Oracle(orc.Case(sigma=100.0, periodic=(1, 1, 1), regime=1, **grid))
+    p0 = np.zeros((8, 24, 24, 1))
+    p0[0], p0[4], p0[5], p0[6], p0[7] = 1.0, 0.5, 0.3, -0.2, 0.4
+    cons, prims, aux = o.set(p0)
+    src = np.zeros_like(cons)
+    o.L.oc_regime_source(ctypes.byref(o.c), orc._p(cons), orc._p(prims), orc._p(aux), orc._p(src))
+    assert np.abs(src[:, 4:-4, 4:-4]).max() == 0.0
+
+
+# (fixture, integrator, tolerance). The current sheet (sigma 1e4) gets 5e-9:
+# the reference's cons2prim stops at xtol 1.49e-7 (srmhd.cc:401), which leaves
+# root differences that the same run without REGIME shows as 2.6e-10 in Sx,
+# and the stiff REGIME source at sigma 1e4 amplifies them into Sx, a field
+# 2000 times smaller than tau (its other fields agree to 2.3e-13). The
+# extension's source itself equals the reference's bit for bit.
+REGIME_RUNS = [("currentsheet_128_regime_rksplit", "rksplit", 5e-9), ("briowu_128_regime_rksplit", "rksplit", 1e-9),
+               ("kh2d_32_weno5_regime_rk2b", "rk2b", 1e-9), ("kh3d_12_weno3_regime_rksplit", "rksplit", 1e-9)]
+
+
+@pytest.mark.parametrize("name, integrator, tol", REGIME_RUNS)
+def test_regime_runs_match_reference(golden, name, integrator, tol):
+    """RKSplit and RK2B with the REGIME extension against the reference build:
+    its two REGIME examples at their own parameters, 2-D WENO5 and 3-D."""
+    z = golden(name)
+    case = regime_case(z)
+    steps = int(z["meta"][5])
+    o = orc.Oracle(case)
+    oc, op, oa, dts, st = o.run(z["prims0"], steps, integrator=integrator)
+    assert np.array_equal(np.array(dts), z["dts"]) and st.failures == 0
+    dims = 1 if case.ny == 0 else (2 if case.nz == 0 else 3)
+    for got, ref, names in ((oc, z[f"cons{steps}"], parity.SRMHD_CONS), (op, z[f"prims{steps}"], parity.SRMHD_PRIMS)):
+        got = parity.interior(got, 4, dims)
+        if ref.shape != got.shape:
+            ref = parity.interior(ref, 4, dims)
+        rows = parity.field_errors(got, ref, names)
+        assert max(r[4] for r in rows) <= tol, parity.report(name, rows)
+    # the extension acts: the same run without it differs
+    plain = orc.Oracle(regime_case(z, regime=0)).run(z["prims0"], steps, integrator=integrator)[0]
+    assert np.abs(plain - oc).max() > 1e-8
