@@ -135,6 +135,13 @@ int mgpu_set_model_extension(mgpu_engine *e, int kind);
  * ghost cells are zero. Needs the extension set. */
 int mgpu_source_extension(mgpu_engine *e, double *source);
 
+/* REGIME's public per-cell fields (REGIME.h:83-101: E, q, alpha, dwdsb, K as
+ * set by set_vars, set_dwdsb and set_K) of the resident state over the padded
+ * grid, each [component][padded cell]: E 3, q 1, alpha 1, dwdsb 12 x 3, K 3.
+ * Ghost cells hold their boundary images' primitives. Any pointer may be
+ * NULL. Needs the extension set. */
+int mgpu_regime_fields(mgpu_engine *e, double *E, double *q, double *alpha, double *dwdsb, double *K);
+
 /* Model::getPrimitiveVars (srmhd.cc:390-562, srrmhd.cc:314-356) on the
  * resident cons with the resident prims as the starting guess. */
 int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);

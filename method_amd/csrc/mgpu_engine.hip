@@ -805,6 +805,31 @@ int mgpu_source_extension(mgpu_engine *e, double *source) {
   return 0;
 }
 
+int mgpu_regime_fields(mgpu_engine *e, double *E, double *q, double *alpha, double *dwdsb, double *K) {
+  if (!e) return fail("null engine");
+  if (e->extension != MGPU_EXT_REGIME) return fail("mgpu_regime_fields: no model extension is set");
+  extension_source(e, e->V);
+  HIP_OK(hipGetLastError());
+  const RegimeFields f = regime_fields_of(e->pad, e->rg_work);
+  const size_t nb = sizeof(double) * e->pad.ntot;
+  if (E) HIP_OK(hipMemcpyAsync(E, f.E, 3 * nb, hipMemcpyDeviceToHost, e->stream));
+  if (q) HIP_OK(hipMemcpyAsync(q, f.q, nb, hipMemcpyDeviceToHost, e->stream));
+  if (alpha) HIP_OK(hipMemcpyAsync(alpha, f.alpha, nb, hipMemcpyDeviceToHost, e->stream));
+  if (K) HIP_OK(hipMemcpyAsync(K, f.K, 3 * nb, hipMemcpyDeviceToHost, e->stream));
+  if (dwdsb) {
+    double *tmp = nullptr;
+    HIP_OK(hipMalloc(&tmp, 36 * nb));
+    launch_regime_dwdsb(e->geo, e->pad, e->V, e->rg_work, e->cfg.sigma, tmp, e->stream);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = le == hipSuccess ? hipMemcpyAsync(dwdsb, tmp, 36 * nb, hipMemcpyDeviceToHost, e->stream) : le;
+    const hipError_t se = hipStreamSynchronize(e->stream);
+    hipFree(tmp);
+    if (ce != hipSuccess || se != hipSuccess) return fail("mgpu_regime_fields: dwdsb readback failed");
+  }
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
 int mgpu_source(mgpu_engine *e, double *source) {
   if (!e || !source) return fail("null argument");
   if (ensure_staging(e)) return -1;

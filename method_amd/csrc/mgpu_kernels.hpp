@@ -125,6 +125,15 @@ void launch_rk410_prepare2(const Geo &g, int ncons, double *U1, double *U2, hipS
 // reference's loop ranges; work holds regime_work_doubles(pg) doubles; dx are
 // the grid spacings.
 size_t regime_work_doubles(const PadGeo &pg);
+// Where REGIME's per-cell fields sit in the work array ([..][padded cell]):
+// E [3], q, face fluxes [3 axes][3], D [3 axes][8], K [3], alpha.
+struct RegimeFields {
+  double *E, *q, *F, *D, *K, *alpha;
+};
+RegimeFields regime_fields_of(const PadGeo &pg, double *work);
+// dw/ds_b [12*3][padded cell] from the fields of the last source evaluation.
+void launch_regime_dwdsb(const Geo &g, const PadGeo &pg, const double *V, const double *work, double sigma,
+                         double *out, hipStream_t s);
 void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], const double *V, double *work,
                           double *src, double gam, double sigma, int order, hipStream_t s);
 void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s);

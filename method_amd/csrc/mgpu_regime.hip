@@ -71,14 +71,17 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_q(Geo g, PadGeo pg, S
   int pos[3];
   padded_pos(pg, p, pos);
   bool in = true;
-  for (int ax = 0; ax < g.dims; ++ax) in = in && pos[ax] >= 2 && pos[ax] < pg.N[ax] - 2;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    if (ax < g.dims) in = in && pos[ax] >= 2 && pos[ax] < pg.N[ax] - 2;
   double acc = 0.0;
   if (in) {
-    const double *dd = sp.d;
-    for (int ax = 0; ax < g.dims; ++ax) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (ax >= g.dims) break;
       const double *Ea = E + ax * pg.ntot;
       const long long s = pstride(pg, ax);
-      const double t = (-Ea[p + 2 * s] + 8 * Ea[p + s] - 8 * Ea[p - s] + Ea[p - 2 * s]) / (12 * dd[ax]);
+      const double t = (-Ea[p + 2 * s] + 8 * Ea[p + s] - 8 * Ea[p - s] + Ea[p - 2 * s]) / (12 * sp.d[ax]);
       acc = ax == 0 ? t : acc + t;
     }
   }
@@ -102,29 +105,31 @@ __device__ __forceinline__ void stiff_flux(int ax, const double P[8], double fb[
   }
 }
 
-// FVS::fluxReconstruction of (cons = E, f = fb) along axis ax with 3
+// FVS::fluxReconstruction of (cons = E, f = fb) along axis AX with 3
 // variables (fluxVectorSplitting.cc:4-56): the face value at the left face of
-// each padded cell with [shift, N - shift) along ax, zero elsewhere.
-template <int ORDER>
+// each padded cell with [shift, N - shift) along AX, zero elsewhere. The axis
+// is a template parameter so every per-axis array index is a constant.
+template <int ORDER, int AX>
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_flux(Geo g, PadGeo pg, const double *V, const double *E,
-                                                                int ax, double *F) {
+                                                                double *F) {
   constexpr int R = Weno<ORDER>::R;
   const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
   padded_pos(pg, p, pos);
-  if (pos[ax] < R || pos[ax] >= pg.N[ax] - R) {
+  if (pos[AX] < R || pos[AX] >= pg.N[AX] - R) {
     for (int v = 0; v < 3; ++v) F[v * pg.ntot + p] = 0.0;
     return;
   }
-  const long long s = pstride(pg, ax);
+  const long long s = pstride(pg, AX);
   double fp[3][2 * R], fm[3][2 * R];
+#pragma unroll
   for (int m = 0; m < 2 * R; ++m) {
-    int cp[3] = {pos[0], pos[1], pos[2]};
-    cp[ax] += m - R;
+    const int cp[3] = {pos[0] + (AX == 0 ? m - R : 0), pos[1] + (AX == 1 ? m - R : 0),
+                       pos[2] + (AX == 2 ? m - R : 0)};
     double P[8], fb[3];
     load_prims(V, g.vstride, image_of(g, pg, cp), P);
-    stiff_flux(ax, P, fb);
+    stiff_flux(AX, P, fb);
     const long long pc = p + (m - R) * s;
     for (int v = 0; v < 3; ++v) {
       const double e = E[v * pg.ntot + pc];
@@ -206,19 +211,21 @@ __device__ __forceinline__ void regime_dfdw(int ax, double gam, const double P[8
 // REGIME::set_Dx/Dy/Dz form it (REGIME.cc:551-774).
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, Spacing sp, const double *V,
                                                              const double *E, const double *Q, const double *F,
-                                                             double *D, double gam, double sig) {
+                                                             double *D, double *Kout, double *Aout, double gam,
+                                                             double sig) {
   const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   const long long nt = pg.ntot;
   int pos[3];
   padded_pos(pg, p, pos);
-  const double *dd = sp.d;
   double K[3] = {0.0, 0.0, 0.0};
-  for (int ax = 0; ax < g.dims; ++ax) {
-    if (pos[ax] < 1 || pos[ax] >= pg.N[ax] - 1) continue;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (ax >= g.dims || pos[ax] < 1 || pos[ax] >= pg.N[ax] - 1) continue;
     const long long s = pstride(pg, ax);
     const double *Fa = F + (long long)ax * 3 * nt;
-    for (int v = 0; v < 3; ++v) K[v] += Fa[v * nt + p + s] / (2 * dd[ax]) - Fa[v * nt + p - s] / (2 * dd[ax]);
+    const double h2 = 2 * sp.d[ax];
+    for (int v = 0; v < 3; ++v) K[v] += Fa[v * nt + p + s] / h2 - Fa[v * nt + p - s] / h2;
   }
   double P[8];
   load_prims(V, g.vstride, image_of(g, pg, pos), P);
@@ -226,9 +233,13 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, S
   const double q = Q[p];
   const double qs = q * q + sig * sig;
   const double alpha = 1 / (qs * (qs + sig * sig * (P[5] * P[5] + P[6] * P[6] + P[7] * P[7])));
+  for (int v = 0; v < 3; ++v) Kout[v * nt + p] = K[v];
+  Aout[p] = alpha;
   double dw[12][3];
   regime_dwdsb(q, sig, P, Ec, alpha, dw);
-  for (int ax = 0; ax < g.dims; ++ax) {
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (ax >= g.dims) break;
     double J[5][6], j0, j4;
     regime_dfdw(ax, gam, P, Ec, J, j0, j4);
     const int a = 1 + ax;
@@ -271,41 +282,85 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_source(Geo g, PadGeo 
   const int c[3] = {(int)(q / g.stride[0]), (int)((q / g.stride[1]) % g.n[1]), (int)(q % g.n[2])};
   const long long p =
       ((long long)(c[0] + pg.off[0]) * pg.N[1] + (c[1] + pg.off[1])) * pg.N[2] + (c[2] + pg.off[2]);
-  const double *dd = sp.d;
   for (int v = 0; v < 8; ++v) {
     double acc = 0.0;
-    for (int ax = 0; ax < g.dims; ++ax) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (ax >= g.dims) break;
       const long long s = pstride(pg, ax);
       const double *Dv = D + ((long long)ax * 8 + v) * nt;
-      const double t = (Dv[p + s] - Dv[p - s]) / (2 * dd[ax]);
+      const double t = (Dv[p + s] - Dv[p - s]) / (2 * sp.d[ax]);
       acc = ax == 0 ? t : acc + t;
     }
     src[v * g.vstride + q] = acc;
   }
 }
 
+// dw/ds_b of every padded cell into dw[12*3][cell] (REGIME.cc:226-290), from
+// the fields k_regime_d left (q, alpha).
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_dwdsb(Geo g, PadGeo pg, const double *V, const double *E,
+                                                                 const double *Q, const double *A, double sig,
+                                                                 double *out) {
+  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const long long nt = pg.ntot;
+  int pos[3];
+  padded_pos(pg, p, pos);
+  double P[8];
+  load_prims(V, g.vstride, image_of(g, pg, pos), P);
+  const double Ec[3] = {E[p], E[nt + p], E[2 * nt + p]};
+  double dw[12][3];
+  regime_dwdsb(Q[p], sig, P, Ec, A[p], dw);
+  for (int l = 0; l < 12; ++l)
+    for (int m = 0; m < 3; ++m) out[(l * 3 + m) * nt + p] = dw[l][m];
+}
+
 unsigned blocks(long long n) { return (unsigned)((n + kRegimeThreads - 1) / kRegimeThreads); }
 
 }  // namespace
 
-size_t regime_work_doubles(const PadGeo &pg) { return (size_t)pg.ntot * (3 + 1 + 3 * 3 + 3 * 8); }
+// E (3), q, face fluxes (3 axes x 3), D (3 axes x 8), K (3), alpha
+size_t regime_work_doubles(const PadGeo &pg) { return (size_t)pg.ntot * (3 + 1 + 3 * 3 + 3 * 8 + 3 + 1); }
+
+RegimeFields regime_fields_of(const PadGeo &pg, double *work) {
+  const long long nt = pg.ntot;
+  RegimeFields f;
+  f.E = work;
+  f.q = f.E + 3 * nt;
+  f.F = f.q + nt;
+  f.D = f.F + 9 * nt;
+  f.K = f.D + 24 * nt;
+  f.alpha = f.K + 3 * nt;
+  return f;
+}
+
+void launch_regime_dwdsb(const Geo &g, const PadGeo &pg, const double *V, const double *work, double sigma,
+                         double *out, hipStream_t s) {
+  const RegimeFields f = regime_fields_of(pg, const_cast<double *>(work));
+  hipLaunchKernelGGL(k_regime_dwdsb, dim3(blocks(pg.ntot)), dim3(kRegimeThreads), 0, s, g, pg, V, f.E, f.q, f.alpha,
+                     sigma, out);
+}
 
 void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], const double *V, double *work,
                           double *src, double gam, double sigma, int order, hipStream_t s) {
   const long long nt = pg.ntot;
   const Spacing sp = {{dx[0], dx[1], dx[2]}};
-  double *E = work, *Q = E + 3 * nt, *F = Q + nt, *D = F + 9 * nt;
+  const RegimeFields fl = regime_fields_of(pg, work);
+  double *E = fl.E, *Q = fl.q, *F = fl.F, *D = fl.D;
   hipLaunchKernelGGL(k_regime_e, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E);
   hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, E, Q);
-  for (int ax = 0; ax < g.dims; ++ax) {
-    if (order == 3)
-      hipLaunchKernelGGL(k_regime_flux<3>, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E, ax,
-                         F + (long long)ax * 3 * nt);
-    else
-      hipLaunchKernelGGL(k_regime_flux<5>, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E, ax,
-                         F + (long long)ax * 3 * nt);
+  const dim3 grid(blocks(nt)), block(kRegimeThreads);
+  if (order == 3) {
+    hipLaunchKernelGGL((k_regime_flux<3, 0>), grid, block, 0, s, g, pg, V, E, F);
+    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<3, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt);
+    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<3, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt);
+  } else {
+    hipLaunchKernelGGL((k_regime_flux<5, 0>), grid, block, 0, s, g, pg, V, E, F);
+    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<5, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt);
+    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<5, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt);
   }
-  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, V, E, Q, F, D, gam, sigma);
+  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, V, E, Q, F, D, fl.K,
+                     fl.alpha, gam, sigma);
   hipLaunchKernelGGL(k_regime_source, dim3(blocks(g.vstride)), dim3(kRegimeThreads), 0, s, g, pg, sp, D, src);
 }
 

@@ -28,7 +28,7 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
     "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_sizes", "mgpu_batch_set_prims", "mgpu_batch_upload_state",
@@ -114,6 +114,7 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_source.argtypes = [E, dp]
     L.mgpu_set_model_extension.argtypes = [E, ctypes.c_int]
     L.mgpu_source_extension.argtypes = [E, dp]
+    L.mgpu_regime_fields.argtypes = [E, dp, dp, dp, dp, dp]
     L.mgpu_c2p.argtypes = [E, sp]
     L.mgpu_max_wavespeed.argtypes = [E, dp]
     fp = ctypes.POINTER(ctypes.c_float)
@@ -318,6 +319,18 @@ class Engine:
         out = self.zeros(self.ncons)
         self._check(self.L.mgpu_source_extension(self._h, _dp(out)), "mgpu_source_extension")
         return out
+
+    def regime_fields(self):
+        """REGIME's per-cell fields over the padded grid: E (3), q, alpha,
+        dwdsb (12 x 3) and K (3), each with the grid shape after its
+        components."""
+        shape = self.cfg.shape
+        E, K = np.zeros((3,) + shape), np.zeros((3,) + shape)
+        q, alpha = np.zeros(shape), np.zeros(shape)
+        dw = np.zeros((12, 3) + shape)
+        self._check(self.L.mgpu_regime_fields(self._h, _dp(E), _dp(q), _dp(alpha), _dp(dw), _dp(K)),
+                    "mgpu_regime_fields")
+        return E, q, alpha, dw, K
 
     def c2p(self):
         st = MgpuStats()

@@ -145,6 +145,22 @@ def ot_prims_resistive(grid: Grid, rho: float = 1.0, pressure: float = 1.0) -> n
 
 # Workload grids -------------------------------------------------------------
 
+def current_sheet_prims(grid: Grid, sigma: float, direction: int = 0) -> np.ndarray:
+    """CurrentSheetSingleFluid (initFunc.cc:191-226): rho 1, p 50, at rest, and
+    one field component B0 erf(s sqrt(sigma) / 2) along axis `direction`
+    (By(x), Bz(y) or Bx(z)), the resistive current sheet of the REGIME
+    examples; gamma 2 on [-3, 3] along that axis."""
+    p = np.zeros((8,) + grid.shape)
+    p[0], p[4] = 1.0, 50.0
+    centres = grid.centres()[direction]
+    erf = np.vectorize(math.erf)
+    b = erf(0.5 * centres * math.sqrt(sigma))
+    shape = [1, 1, 1]
+    shape[direction] = -1
+    p[(6, 7, 5)[direction]] = b.reshape(shape)
+    return p
+
+
 def kh2d_grid(n: int = 64) -> Grid:
     """2-D KH grid on [-0.5,0.5]x[-1,1] (n x n interior cells, Ng=4)."""
     return Grid(n, n, 0, -0.5, 0.5, -1.0, 1.0)

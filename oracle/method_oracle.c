@@ -1066,22 +1066,19 @@ static void regime_D(int dir, double gam, const double *P, const double *E, cons
 }
 #undef DW
 
-void oc_regime_source(const oc_config *c, const double *cons, const double *prims, const double *aux, double *src) {
-  (void)cons;
-  (void)aux;
+/* set_vars and set_K (REGIME.cc:141-224,293-370): E, q, alpha and K of every
+ * padded cell from the prims array as given (ghosts included). */
+static void regime_fields(const oc_config *c, const double *prims, double *E, double *q, double *alpha, double *K) {
   geom g = mkgeom(c);
   long ntot = g.ntot;
   int N[3] = {g.Nx, g.Ny, g.Nz};
   long st[3] = {(long)g.Ny * g.Nz, g.Nz, 1};
   double dd[3] = {g.dx, g.dy, g.dz};
-  double sig = c->sigma, gam = c->gamma;
-  double *E = (double *)calloc(3 * ntot, sizeof(double));
-  double *q = (double *)calloc(ntot, sizeof(double));
-  double *alpha = (double *)calloc(ntot, sizeof(double));
-  double *K = (double *)calloc(3 * ntot, sizeof(double));
+  double sig = c->sigma;
   double *fb = (double *)calloc(3 * ntot, sizeof(double));
   double *fr = (double *)calloc(3 * ntot, sizeof(double));
-  double *D = (double *)calloc(8 * ntot, sizeof(double));
+  memset(q, 0, sizeof(double) * ntot);
+  memset(K, 0, sizeof(double) * 3 * ntot);
   /* set_vars (REGIME.cc:141-224) */
   for (long n = 0; n < ntot; n++) {
     const double *P0 = prims + n;
@@ -1132,6 +1129,41 @@ void oc_regime_source(const oc_config *c, const double *cons, const double *prim
             K[v * ntot + n] += fr[v * ntot + n + st[d]] / (2 * dd[d]) - fr[v * ntot + n - st[d]] / (2 * dd[d]);
           }
   }
+#undef PV
+  free(fb);
+  free(fr);
+}
+
+void oc_regime_fields(const oc_config *c, const double *prims, double *E, double *q, double *alpha, double *dwdsb,
+                      double *K) {
+  geom g = mkgeom(c);
+  long ntot = g.ntot;
+  regime_fields(c, prims, E, q, alpha, K);
+  for (long n = 0; n < ntot; n++) {
+    double P[8], Ec[3], dw[36];
+    for (int v = 0; v < 8; v++) P[v] = prims[(long)v * ntot + n];
+    for (int v = 0; v < 3; v++) Ec[v] = E[v * ntot + n];
+    regime_dwdsb(q[n], c->sigma, P, Ec, alpha[n], dw);
+    for (int l = 0; l < 36; l++) dwdsb[l * ntot + n] = dw[l];
+  }
+}
+
+void oc_regime_source(const oc_config *c, const double *cons, const double *prims, const double *aux, double *src) {
+  (void)cons;
+  (void)aux;
+  geom g = mkgeom(c);
+  long ntot = g.ntot;
+  int N[3] = {g.Nx, g.Ny, g.Nz};
+  long st[3] = {(long)g.Ny * g.Nz, g.Nz, 1};
+  double dd[3] = {g.dx, g.dy, g.dz};
+  double sig = c->sigma, gam = c->gamma;
+  double *E = (double *)calloc(3 * ntot, sizeof(double));
+  double *q = (double *)calloc(ntot, sizeof(double));
+  double *alpha = (double *)calloc(ntot, sizeof(double));
+  double *K = (double *)calloc(3 * ntot, sizeof(double));
+  double *D = (double *)calloc(8 * ntot, sizeof(double));
+  regime_fields(c, prims, E, q, alpha, K);
+#define PV(v) P0[(long)(v) * ntot]
   /* sourceExtension (REGIME.cc:83-138): D per axis, then its centred
    * difference (= on the first axis, += on the others) */
   for (int d = 0; d < g.dims; d++) {
@@ -1163,8 +1195,6 @@ void oc_regime_source(const oc_config *c, const double *cons, const double *prim
   free(q);
   free(alpha);
   free(K);
-  free(fb);
-  free(fr);
   free(D);
 }
 

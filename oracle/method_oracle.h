@@ -94,6 +94,11 @@ long oc_rksplit_step(const oc_config *c, double *cons, double *prims, double *au
 /* REGIME::sourceExtension (REGIME.cc:83-138) of the given state, every cell
  * it writes (variables 0..7; variable 8 and the outer layer stay zero). */
 void oc_regime_source(const oc_config *c, const double *cons, const double *prims, const double *aux, double *src);
+/* REGIME's per-cell fields (set_vars, set_dwdsb, set_K: REGIME.cc:141-370) of
+ * the prims array as given: E [3], q, alpha, dwdsb [12][3] and K [3], each
+ * [..][cell] over the padded grid. */
+void oc_regime_fields(const oc_config *c, const double *prims, double *E, double *q, double *alpha, double *dwdsb,
+                      double *K);
 /* RKPlus family (RKPlus.cc): method OC_RK2B, OC_RK3, OC_RK4 or OC_RK4_10. */
 enum { OC_RK2B = 0, OC_RK3 = 1, OC_RK4 = 2, OC_RK4_10 = 3 };
 long oc_rkplus_step(const oc_config *c, double *cons, double *prims, double *aux, double dt, int method,

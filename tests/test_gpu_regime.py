@@ -120,3 +120,95 @@ def test_regime_needs_ideal_model_and_is_required_for_its_source():
     with eng.Engine(eng.Config(**grid)) as e:
         with pytest.raises(eng.EngineError, match="no model extension"):
             e.source_extension()
+
+
+def _gpu_sheet(grid, direction, end_time=0.1):
+    """The reference's REGIME symmetry set-ups (test_REGIME.cc:943-1088) on the
+    engine: current sheet, gamma 2, sigma 100, cfl 0.4, WENO3, Outflow,
+    RKSplit + REGIME, evolved to t = 0.1."""
+    from method_amd import initdata as idt
+
+    cfg = eng.Config(gamma=2.0, sigma=100.0, cfl=0.4, cp=0.1, weno_order=3, integrator=eng.RKSPLIT,
+                     bc=(eng.OUTFLOW,) * 3, **grid)
+    with eng.Engine(cfg) as e:
+        e.set_prims(idt.current_sheet_prims(idt.Grid(**grid), 100.0, direction))
+        e.set_model_extension(eng.EXT_REGIME)
+        while e.time()[0] < end_time:
+            assert e.update_time(end_time) == 0
+        return e.download()
+
+
+def test_regime_sheet_symmetries():
+    """YAxis / ZAxis / RotSymmetries of the reference's REGIME tests, to their
+    1e-15, and the 3-D sheet against the oracle."""
+    say("gpu: REGIME current-sheet symmetries")
+    c2, p2, _ = _gpu_sheet(dict(nx=10, ny=10, nz=0, xmin=-3.0, xmax=3.0, ymin=-1.0, ymax=1.0), 0)
+    assert np.abs(np.diff(p2[:, 4:-4, 4:-4, 0], axis=2)).max() <= 1e-15
+    g3 = dict(nx=10, ny=10, nz=10, xmin=-3.0, xmax=3.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0)
+    c3, p3, _ = _gpu_sheet(g3, 0)
+    inner = p3[:, 4:-4, 4:-4, 4:-4]
+    assert np.abs(np.diff(inner, axis=2)).max() <= 1e-15 and np.abs(np.diff(inner, axis=3)).max() <= 1e-15
+    grid = dict(nx=10, ny=10, nz=10, xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0, zmin=-3.0, zmax=3.0)
+    ca, _, _ = _gpu_sheet(grid, 0)
+    cb, _, _ = _gpu_sheet(grid, 1)
+    cc, _, _ = _gpu_sheet(grid, 2)
+    a = ca[6, 4:-4, 4:-4, 4:-4]
+    b = np.transpose(cb[7, 4:-4, 4:-4, 4:-4], (1, 2, 0))
+    c = np.transpose(cc[5, 4:-4, 4:-4, 4:-4], (2, 0, 1))
+    say(f"  rotations: max |A - B| {np.abs(a - b).max():.3e}, max |A - C| {np.abs(a - c).max():.3e}")
+    assert np.abs(a - b).max() <= 1e-15 and np.abs(a - c).max() <= 1e-15
+    # the 3-D sheet against the oracle
+    from method_amd import initdata as idt
+    o = orc.Oracle(orc.Case(gamma=2.0, sigma=100.0, cfl=0.4, cp=0.1, periodic=(0, 0, 0), regime=1, **g3))
+    oc, op, oa = o.set(idt.current_sheet_prims(idt.Grid(**g3), 100.0, 0))
+    t, it = 0.0, 0
+    while t < 0.1:
+        dt = o.timestep(t, it, 0.1)
+        assert o.step(oc, op, oa, dt, "rksplit")[0] == 0
+        t += dt
+        it += 1
+    rows = parity.field_errors(parity.interior(c3, 4, 3), parity.interior(oc, 4, 3), parity.SRMHD_CONS)
+    say(parity.report("3-D current sheet + REGIME vs oracle", rows))
+    for n, dm, fm, rf, rg, pw in rows:
+        assert rg <= 1e-9, f"{n}: {rg:.3e}"
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_regime_fields_match_reference_known_answers(axis):
+    """The reference's DataAssignment1D/2D/3D known answers
+    (test_REGIME.cc:36-353) on the GPU's REGIME fields: E, q = dE/ds, alpha
+    and dw/ds_b at the cell where the linear coordinate is 1 (its stencils lie
+    inside the interior, so the engine's boundary images do not enter)."""
+    from method_amd import initdata as idt
+    from tests.test_oracle import REGIME_KAT
+
+    name = ("1D", "2D", "3D")[axis]
+    n = (100, 10, 10)[axis]
+    lo, hi = ((0.01, 2.01), (0.1, 2.1), (0.1, 2.1))[axis]
+    grid = dict(nx=n if axis == 0 else 10, ny=0 if axis == 0 else 10, nz=10 if axis == 2 else 0,
+                xmin=lo if axis == 0 else 0.1, xmax=hi if axis == 0 else 2.1, ymin=0.1 if axis else 0.0,
+                ymax=2.1 if axis else 1.0, zmin=0.1 if axis == 2 else 0.0, zmax=2.1 if axis == 2 else 1.0)
+    g = idt.Grid(**grid)
+    s = g.centres()[axis]
+    mid = len(s) // 2 - 1
+    sb = s.reshape([-1 if a == axis else 1 for a in range(3)])
+    p = np.zeros((8,) + g.shape)
+    lin_v, const_v, lin_b, const_b = [(2, 3, 6, 7), (3, 1, 7, 5), (1, 2, 5, 6)][axis]
+    p[0], p[4] = 0.1, 0.66
+    p[lin_v] = 0.41 * sb
+    p[const_v] = 0.51
+    p[lin_b] = 0.22 * sb
+    p[const_b] = 0.33
+    say(f"gpu: REGIME known answers {name}")
+    cfg = eng.Config(gamma=2.0, sigma=50.0, bc=(eng.PERIODIC,) * 3, **grid)
+    with eng.Engine(cfg) as e:
+        e.upload(prims=p)
+        e.set_model_extension(eng.EXT_REGIME)
+        E, q, alpha, dw, K = e.regime_fields()
+    cell = tuple(mid if a <= axis else 0 for a in range(3))
+    e_exp = -(0.41 * 0.33 - 0.51 * 0.22)
+    assert abs(E[axis][cell] - e_exp) <= 1e-15
+    assert abs(q[cell] - e_exp) <= 1e-15
+    assert abs(alpha[cell] - 1.3825277485504367e-07) <= 1e-13
+    for l, m, want, tol in REGIME_KAT[name]:
+        assert abs(dw[l, m][cell] - want) <= tol, (name, l, m)

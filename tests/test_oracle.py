@@ -527,3 +527,152 @@ def test_regime_runs_match_reference(golden, name, integrator, tol):
     # the extension acts: the same run without it differs
     plain = orc.Oracle(regime_case(z, regime=0)).run(z["prims0"], steps, integrator=integrator)[0]
     assert np.abs(plain - oc).max() > 1e-8
+
+
+# Known answers of the reference's REGIME tests (Tests/CPU/Src/test_REGIME.cc:36-353,
+# DataAssignment1D/2D/3D): a state linear along one axis (v and B components
+# 0.41 s, 0.22 s with s the coordinate, the rest constant), sigma 50, gamma 2;
+# E, q = d E / d s, alpha and every dw/ds_b entry at the cell where s = 1.
+# Entries: (l, m, expected, tolerance) as the reference's EXPECT_NEAR.
+REGIME_KAT = {
+    "1D": [
+        (1, 0, 7.984099454632758e-06, 1e-11), (1, 1, 0.005702928181880542, 1e-08), (1, 2, -0.0038019521212536945, 1e-08),
+        (2, 0, -0.005702928181880542, 1e-08), (2, 1, 8.37052978575575e-06, 1e-11), (2, 2, 5.79645496684485e-07, 1e-12),
+        (3, 0, 0.0038019521212536945, 1e-08), (3, 1, 5.79645496684485e-07, 1e-12), (3, 2, 8.853567699659486e-06, 1e-11),
+        (5, 0, 0.0, 1e-15), (5, 1, -0.008725791205832886, 1e-08), (5, 2, 0.007217193838855632, 1e-08),
+        (6, 0, 0.008813616281088111, 1e-08), (6, 1, 0.0, 1e-15), (6, 2, 0.0, 1e-15),
+        (7, 0, -0.007085456225972795, 1e-08), (7, 1, 0.0, 1e-15), (7, 2, 0.0, 1e-15),
+        (8, 0, -0.01728160055115316, 1e-08), (8, 1, 0.0, 1e-15), (8, 2, 0.0, 1e-15),
+        (9, 0, 0.0, 1e-15), (9, 1, -0.01811802983929816, 1e-08), (9, 2, -0.0012546439322175, 1e-09),
+        (10, 0, 0.0, 1e-15), (10, 1, -0.0012546439322175, 1e-09), (10, 2, -0.01916356644947941, 1e-08),
+        (0, 0, 0.0, 1e-15), (0, 1, 0.0, 1e-15), (0, 2, 0.0, 1e-15),
+        (4, 0, 0.0, 1e-15), (4, 1, 0.0, 1e-15), (4, 2, 0.0, 1e-15),
+        (11, 0, 0.0, 1e-15), (11, 1, 0.0, 1e-15), (11, 2, 0.0, 1e-15),
+    ],
+    "2D": [
+        (1, 0, 8.853567699659485e-06, 1e-11), (1, 1, 0.0038019521212536945, 1e-08), (1, 2, 5.796454966844849e-07, 1e-12),
+        (2, 0, -0.0038019521212536945, 1e-08), (2, 1, 7.984099454632758e-06, 1e-11), (2, 2, 0.005702928181880542, 1e-08),
+        (3, 0, 5.796454966844848e-07, 1e-11), (3, 1, -0.005702928181880542, 1e-08), (3, 2, 8.370529785755748e-06, 1e-11),
+        (5, 0, 0.0, 1e-15), (5, 1, -0.007085456225972795, 1e-08), (5, 2, 0.0, 1e-15),
+        (6, 0, 0.007217193838855632, 1e-08), (6, 1, 0.0, 1e-15), (6, 2, -0.008725791205832886, 1e-08),
+        (7, 0, 0.0, 1e-15), (7, 1, 0.008813616281088111, 1e-08), (7, 2, 0.0, 1e-15),
+        (8, 0, -0.01916356644947941, 1e-08), (8, 1, 0.0, 1e-15), (8, 2, -0.0012546439322175, 1e-09),
+        (9, 0, 0.0, 1e-15), (9, 1, -0.01728160055115316, 1e-08), (9, 2, 0.0, 1e-10),
+        (10, 0, -0.0012546439322175, 1e-09), (10, 1, 0.0, 1e-10), (10, 2, -0.01811802983929816, 1e-08),
+        (0, 0, 0.0, 1e-15), (0, 1, 0.0, 1e-15), (0, 2, 0.0, 1e-15),
+        (4, 0, 0.0, 1e-15), (4, 1, 0.0, 1e-15), (4, 2, 0.0, 1e-15),
+        (11, 0, 0.0, 1e-15), (11, 1, 0.0, 1e-15), (11, 2, 0.0, 1e-15),
+    ],
+    "3D": [
+        (1, 0, 8.370529785755748e-06, 1e-11), (1, 1, 5.796454966844848e-07, 1e-12), (1, 2, -0.005702928181880542, 1e-08),
+        (2, 0, 5.796454966844849e-07, 1e-12), (2, 1, 8.853567699659485e-06, 1e-11), (2, 2, 0.0038019521212536945, 1e-08),
+        (3, 0, 0.005702928181880542, 1e-08), (3, 1, -0.0038019521212536945, 1e-08), (3, 2, 7.984099454632758e-06, 1e-11),
+        (5, 0, 0.0, 1e-15), (5, 1, 0.0, 1e-15), (5, 2, 0.008813616281088111, 1e-08),
+        (6, 0, 0.0, 1e-15), (6, 1, 0.0, 1e-15), (6, 2, -0.007085456225972795, 1e-08),
+        (7, 0, -0.008725791205832886, 1e-08), (7, 1, 0.007217193838855632, 1e-08), (7, 2, 0.0, 1e-15),
+        (8, 0, -0.01811802983929816, 1e-08), (8, 1, -0.0012546439322175, 1e-09), (8, 2, 0.0, 1e-15),
+        (9, 0, -0.0012546439322175, 1e-09), (9, 1, -0.01916356644947941, 1e-08), (9, 2, 0.0, 1e-15),
+        (10, 0, 0.0, 1e-10), (10, 1, 0.0, 1e-10), (10, 2, -0.01728160055115316, 1e-08),
+        (0, 0, 0.0, 1e-15), (0, 1, 0.0, 1e-15), (0, 2, 0.0, 1e-15),
+        (4, 0, 0.0, 1e-15), (4, 1, 0.0, 1e-15), (4, 2, 0.0, 1e-15),
+        (11, 0, 0.0, 1e-15), (11, 1, 0.0, 1e-15), (11, 2, 0.0, 1e-15),
+    ],
+}
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_regime_fields_match_reference_known_answers(axis):
+    """REGIME::set_vars / set_dwdsb on the reference tests' linear states
+    (prims set on every padded cell, no boundary conditions applied)."""
+    name = ("1D", "2D", "3D")[axis]
+    n = (100, 10, 10)[axis]
+    lo, hi = ((0.01, 2.01), (0.1, 2.1), (0.1, 2.1))[axis]
+    grid = dict(nx=n if axis == 0 else 10, ny=0 if axis == 0 else 10, nz=10 if axis == 2 else 0,
+                xmin=lo if axis == 0 else 0.1, xmax=hi if axis == 0 else 2.1, ymin=0.1 if axis else 0.0,
+                ymax=2.1 if axis else 1.0, zmin=0.1 if axis == 2 else 0.0, zmax=2.1 if axis == 2 else 1.0)
+    case = orc.Case(gamma=2.0, sigma=50.0, periodic=(1, 1, 1), **grid)
+    shape = case.shape
+    N = shape[axis]
+    dx = (hi - lo) / n
+    s = lo + (np.arange(N) + 0.5 - 4) * dx   # cell centres, simulation.cc:42-50
+    mid = N // 2 - 1
+    assert abs(s[mid] - 1.0) < 1e-15
+    sb = s.reshape([-1 if a == axis else 1 for a in range(3)])
+    p = np.zeros((8,) + shape)
+    # the reference's three set-ups are one state rotated: along x (vy, By
+    # linear), along y (vz, Bz linear), along z (vx, Bx linear)
+    lin_v, const_v, lin_b, const_b = [(2, 3, 6, 7), (3, 1, 7, 5), (1, 2, 5, 6)][axis]
+    p[0], p[4] = 0.1, 0.66
+    p[lin_v] = 0.41 * sb
+    p[const_v] = 0.51
+    p[lin_b] = 0.22 * sb
+    p[const_b] = 0.33
+    ntot = int(np.prod(shape))
+    E, q, alpha = np.zeros(3 * ntot), np.zeros(ntot), np.zeros(ntot)
+    dw, K = np.zeros(36 * ntot), np.zeros(3 * ntot)
+    o = orc.Oracle(case)
+    o.L.oc_regime_fields(ctypes.byref(o.c), orc._p(p), orc._p(E), orc._p(q), orc._p(alpha), orc._p(dw), orc._p(K))
+    dims = axis + 1
+    cell = [mid if a < dims else 0 for a in range(3)]
+    c = np.ravel_multi_index(cell, shape)
+    e_exp = -(0.41 * 0.33 - 0.51 * 0.22)
+    assert abs(E[axis * ntot + c] - e_exp) <= 1e-15
+    assert abs(q[c] - e_exp) <= 1e-15
+    assert abs(alpha[c] - 1.3825277485504367e-07) <= 1e-13
+    for l, m, want, tol in REGIME_KAT[name]:
+        assert abs(dw[(l * 3 + m) * ntot + c] - want) <= tol, (name, l, m)
+
+
+def test_current_sheet_generator_matches_reference(golden):
+    """idt.current_sheet_prims equals the reference's CurrentSheetSingleFluid
+    state (initFunc.cc:191-226) on the interior, bit for bit."""
+    from method_amd import initdata as idt
+
+    z = golden("regime_src_currentsheet_128")
+    g = idt.Grid(128, 0, 0, -3.0, 3.0, -1.0, 1.0, -1.5, 1.5)
+    p = idt.current_sheet_prims(g, 1e4)
+    assert np.array_equal(p[:, 4:-4], z["prims0"][:, 4:-4])
+
+
+def _sheet_run(grid, direction, end_time=0.1):
+    """The reference's REGIME symmetry set-ups (test_REGIME.cc:943-1088):
+    current sheet, gamma 2, sigma 100, cfl 0.4, cp 0.1, WENO3, Outflow,
+    RKSplit + REGIME, evolved to t = 0.1."""
+    from method_amd import initdata as idt
+
+    g = idt.Grid(**grid)
+    o = orc.Oracle(orc.Case(gamma=2.0, sigma=100.0, cfl=0.4, cp=0.1, periodic=(0, 0, 0), regime=1, **grid))
+    cons, prims, aux = o.set(idt.current_sheet_prims(g, 100.0, direction))
+    t, it = 0.0, 0
+    while t < end_time:
+        dt = o.timestep(t, it, end_time)
+        fails, _ = o.step(cons, prims, aux, dt, "rksplit")
+        assert fails == 0
+        t += dt
+        it += 1
+    return cons, prims
+
+
+def test_regime_sheet_is_independent_of_y_and_z():
+    """YAxisSymmetries / ZAxisSymmetries: a sheet along x stays uniform in y
+    (2-D) and in y and z (3-D) to 1e-15."""
+    _, p2 = _sheet_run(dict(nx=10, ny=10, nz=0, xmin=-3.0, xmax=3.0, ymin=-1.0, ymax=1.0), 0)
+    inner = p2[:, 4:-4, 4:-4, 0]
+    assert np.abs(np.diff(inner, axis=2)).max() <= 1e-15
+    _, p3 = _sheet_run(dict(nx=10, ny=10, nz=10, xmin=-3.0, xmax=3.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0), 0)
+    inner = p3[:, 4:-4, 4:-4, 4:-4]
+    assert np.abs(np.diff(inner, axis=2)).max() <= 1e-15 and np.abs(np.diff(inner, axis=3)).max() <= 1e-15
+    assert np.abs(np.diff(inner[6], axis=0)).max() > 1e-3  # the sheet varies along x
+
+
+def test_regime_sheet_rotations_agree():
+    """RotSymmetries (test_REGIME.cc:1031-1088): the sheet along x, y and z
+    on [-3, 3]^3 gives the same field, with indices and components rotated."""
+    grid = dict(nx=10, ny=10, nz=10, xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0, zmin=-3.0, zmax=3.0)
+    ca, _ = _sheet_run(grid, 0)
+    cb, _ = _sheet_run(grid, 1)
+    cc, _ = _sheet_run(grid, 2)
+    a = ca[6, 4:-4, 4:-4, 4:-4]
+    b = np.transpose(cb[7, 4:-4, 4:-4, 4:-4], (1, 2, 0))   # dB[7, k, i, j] at (i, j, k)
+    c = np.transpose(cc[5, 4:-4, 4:-4, 4:-4], (2, 0, 1))   # dC[5, j, k, i] at (i, j, k)
+    assert np.abs(a - b).max() <= 1e-15 and np.abs(a - c).max() <= 1e-15
