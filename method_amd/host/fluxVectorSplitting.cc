@@ -26,6 +26,7 @@ void FVS::fluxReconstruction(double *, double *, double *, double *, double *, i
 void FVS::F(double *cons, double *prims, double *aux, double *f, double *fnet) {
   (void)aux;
   (void)f;
+  if (weno) data->gpu.wenoOrder = weno->order;
   mgpu_engine *e = method_host::engine(data);
   method_host::check(mgpu_upload_state(e, cons, prims, nullptr), "FVS::F");
   method_host::check(mgpu_fnet(e, fnet), "FVS::F");

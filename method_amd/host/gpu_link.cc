@@ -36,6 +36,15 @@ mgpu_config config_of(const Data *d) {
 
 }  // namespace
 
+// the model extension an integrator asked for, given to the engine once
+mgpu_engine *with_extension(Data *d, mgpu_engine *e) {
+  if (d->gpu.extension != d->gpu.extensionApplied) {
+    check(mgpu_set_model_extension(e, d->gpu.extension), "mgpu_set_model_extension");
+    d->gpu.extensionApplied = d->gpu.extension;
+  }
+  return e;
+}
+
 int check(int rc, const char *what) {
   if (rc < 0) throw std::runtime_error(std::string(what) + ": " + mgpu_last_error());
   return rc;
@@ -46,7 +55,9 @@ mgpu_engine *engine(Data *d) {
   mgpu_config c = config_of(d);
   if (d->gpu.engine) {
     // engine configuration is fixed at creation; rebuild if the objects around Data changed it
-    if (std::memcmp(&c, &d->gpu.created, sizeof c) == 0) return d->gpu.engine;
+    if (std::memcmp(&c, &d->gpu.created, sizeof c) == 0) return with_extension(d, d->gpu.engine);
+    // the state only the old engine holds goes back to the host arrays first
+    if (d->gpu.hostStale) check(mgpu_download_state(d->gpu.engine, d->cons, d->prims, d->aux), "engine rebuild");
     release(d);
   }
   mgpu_engine *e = nullptr;
@@ -55,7 +66,8 @@ mgpu_engine *engine(Data *d) {
   d->gpu.created = c;
   d->gpu.deviceCurrent = false;
   d->gpu.hostStale = false;
-  return e;
+  d->gpu.extensionApplied = MGPU_EXT_NONE;  // a new engine has none
+  return with_extension(d, e);
 }
 
 void release(Data *d) {

@@ -9,6 +9,8 @@ namespace method_host {
 
 // Creates (or re-creates after a configuration change) the engine for `d`.
 mgpu_engine *engine(Data *d);
+// Gives the engine the model extension d's integrator was handed, once.
+mgpu_engine *with_extension(Data *d, mgpu_engine *e);
 // Throws std::runtime_error with the engine's message when rc < 0.
 int check(int rc, const char *what);
 // Destroys the engine owned by `d`.

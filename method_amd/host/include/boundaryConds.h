@@ -13,9 +13,13 @@ class Bcs {
   Bcs(Data *data) : data(data) { data->bcsSet = 1; }
   virtual ~Bcs() {}
   virtual void apply(double *cons, double *prims = nullptr, double *aux = nullptr);
+  // Makes this object's rule the one the engine applies (the integrator and
+  // Simulation::set call it, so the rule in force is the handed object's).
+  void bind();
 
  protected:
   void setAxes(int px, int py, int pz);
+  int axisRule[3] = {MGPU_BC_OUTFLOW, MGPU_BC_OUTFLOW, MGPU_BC_OUTFLOW};
 };
 
 // Outflow on every face (boundaryConds.cc:5-145).

@@ -9,7 +9,11 @@
 class FVS : public FluxMethod {
  public:
   WenoBase *weno;
-  FVS(Data *data, WenoBase *weno, Model *model) : FluxMethod(data, model), weno(weno) {}
+  // The engine reconstructs with this object's order, whatever other WENO
+  // objects the program holds.
+  FVS(Data *data, WenoBase *weno, Model *model) : FluxMethod(data, model), weno(weno) {
+    if (weno) data->gpu.wenoOrder = weno->order;
+  }
   virtual ~FVS() {}
   // Reconstructed face fluxes are never formed as a separate array by the
   // engine; this throws std::logic_error.

@@ -57,6 +57,15 @@ class BrioWuSingleFluid : public InitialFunc {
   BrioWuSingleFluid(Data *data, int dir = 0);
 };
 
+// Current sheet (initFunc.cc:191-226): rho = 1, p = 50, at rest, with one
+// field component erf(s sqrt(sigma) / 2) across the sheet at s = 0 (By along
+// x for direction 0, Bz along y for 1, Bx along z for 2). Needs x in [-3, 3]
+// and gamma = 2.
+class CurrentSheetSingleFluid : public InitialFunc {
+ public:
+  CurrentSheetSingleFluid(Data *data, int direction = 0);
+};
+
 // Kelvin-Helmholtz with two interfaces at y = 0.25 and 0.75 displaced by ten
 // random Fourier modes (initFunc.cc:612-680): rho = 1 / 0.1 and vx = +-0.5
 // outside / between them, p = 1, Bz = 0.1 when mag is non-zero. The mode

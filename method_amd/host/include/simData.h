@@ -21,6 +21,8 @@ struct GpuLink {
   int bc[3] = {1, 1, 1};// set by the Bcs subclass
   int integrator = 0;   // set by the time integrator
   int quirks = 1;       // reproduce the reference's flux-index quirks
+  int extension = 0;    // MGPU_EXT_*, set by an integrator handed a model extension
+  int extensionApplied = 0;   // extension the current engine was given
   int device = 0;
   mgpu_engine *engine = nullptr;
   mgpu_config created;         // configuration the engine was created with

@@ -21,6 +21,14 @@ class TimeIntegratorBase {
   virtual ~TimeIntegratorBase() {}
   virtual void step(double *cons, double *prims, double *aux, double dt = 0) = 0;
   virtual void finalise(double *cons, double *prims, double *aux) = 0;
+  // Makes this integrator, its boundary conditions, its flux method's
+  // reconstruction order and its model extension the engine's configuration
+  // (each step and Simulation::set call it).
+  void bind();
+
+ protected:
+  int engineIntegrator = -1;            // MGPU_RK2, MGPU_SSP2, ...
+  int engineExtension = MGPU_EXT_NONE;  // MGPU_EXT_*
 };
 
 class TimeIntegrator : public TimeIntegratorBase {

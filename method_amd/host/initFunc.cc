@@ -127,6 +127,28 @@ OTVortexSingleFluid::OTVortexSingleFluid(Data *data) : InitialFunc(data) {
       }
 }
 
+CurrentSheetSingleFluid::CurrentSheetSingleFluid(Data *data, int direction) : InitialFunc(data) {
+  Data *d = data;
+  if (d->Nprims > 15)
+    throw std::invalid_argument(
+        "Trying to implement a single fluid initial state on incorrect model.\n\tModel has wrong number of primitive "
+        "variables to be single fluid model.");
+  if (d->xmin != -3.0 || d->xmax != 3.0) throw std::invalid_argument("Domain has incorrect values. Expected x E [-3, 3]\n");
+  if (d->gamma != 2.0) throw std::invalid_argument("Expected the index gamma = 2\n");
+  const int comp = direction == 0 ? 6 : (direction == 1 ? 7 : (direction == 2 ? 5 : -1));
+  const double root = std::sqrt(d->sigma);
+  for (int i = 0; i < d->Nx; i++)
+    for (int j = 0; j < d->Ny; j++)
+      for (int k = 0; k < d->Nz; k++) {
+        double *P = d->prims;
+        P[ID(0, i, j, k)] = 1.0;
+        P[ID(4, i, j, k)] = 50.0;
+        if (comp < 0) continue;
+        const double s = direction == 0 ? d->x[i] : (direction == 1 ? d->y[j] : d->z[k]);
+        P[ID(comp, i, j, k)] = std::erf(0.5 * s * root);
+      }
+}
+
 BrioWuSingleFluid::BrioWuSingleFluid(Data *data, int dir) : InitialFunc(data) {
   Data *d = data;
   if (d->Nprims >= 16)

@@ -66,6 +66,9 @@ void Simulation::set(InitialFunc *init, Model *model, TimeIntegrator *timeInt, B
   this->bcs = bcs;
   this->fluxMethod = fluxMethod;
   this->save = save;
+  // the engine runs with the handed objects' settings
+  this->bcs->bind();
+  this->timeInt->bind();
   this->model->primsToAll(d->cons, d->prims, d->aux);
   this->bcs->apply(d->cons, d->prims, d->aux);
 }

@@ -153,3 +153,92 @@ def test_host_api_rkplus_rhs_matches_oracle(golden, tmp_path):
     # eps moves the WENO weights by ~1e-7 relative
     for n_, dm, fm, rf, rg, pw in rows:
         assert rg <= 1e-9, n_
+
+
+REGIME_EXE = os.path.join(ROOT, "method_amd", "lib", "regime_main")
+# case -> (run fixture, source fixture, integrator, padded shape, dims, steps, tolerance); the current
+# sheet's bar is the reference cons2prim's tolerance amplified by the stiff source at sigma 1e4
+REGIME_CASES = {
+    "currentsheet": ("currentsheet_128_regime_rksplit", "regime_src_currentsheet_128", (136, 1, 1), 1, 20, 5e-9),
+    "briowu": ("briowu_128_regime_rksplit", "regime_src_briowu_128", (136, 1, 1), 1, 20, 1e-9),
+    "kh2d": ("kh2d_32_weno5_regime_rk2b", "regime_src_kh2d_32_weno5", (40, 40, 1), 2, 3, 1e-9),
+}
+
+
+def _run_regime(tmp_path, case, steps, prims0=None):
+    out = tmp_path / f"{case}_{steps}.bin"
+    cmd = [REGIME_EXE, case, str(out), str(steps)]
+    if prims0 is not None:
+        pin = tmp_path / "prims0.bin"
+        np.ascontiguousarray(prims0).tofile(pin)
+        cmd.append(str(pin))
+    print("run:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the driver's last line gives the final time (Simulation::updateTime reports t = ... too)
+    return np.fromfile(out), float(r.stdout.rsplit("t = ", 1)[1].split()[0])
+
+
+@pytest.mark.parametrize("case", sorted(REGIME_CASES))
+def test_host_api_regime_run_matches_reference(golden, tmp_path, case):
+    """REGIME handed to RKSplit (the reference's two REGIME example set-ups,
+    CurrentSheetSingleFluid and BrioWuSingleFluid built by the host API) or
+    to RK2B (2-D WENO5 KH), against the reference build's steps."""
+    run, _, shape, dims, steps, tol = REGIME_CASES[case]
+    z = golden(run)
+    a, t = _run_regime(tmp_path, case, steps, z["prims0"] if case == "kh2d" else None)
+    n = int(np.prod(shape))
+    c, p = a[:9 * n].reshape((9,) + shape), a[9 * n:17 * n].reshape((8,) + shape)
+    assert abs(t - float(np.sum(z["dts"][:steps]))) <= 1e-14
+    for got, ref, names in ((c, z[f"cons{steps}"], parity.SRMHD_CONS), (p, z[f"prims{steps}"], parity.SRMHD_PRIMS)):
+        got = parity.interior(got, 4, dims)
+        if ref.shape != got.shape:
+            ref = parity.interior(ref, 4, dims)
+        rows = parity.field_errors(got, ref, names)
+        print(parity.report(f"host API REGIME {case}, {steps} steps vs reference", rows), flush=True)
+        for n_, dm, fm, rf, rg, pw in rows:
+            assert rg <= tol, f"{n_}: {rg:.3e}"
+
+
+@pytest.mark.parametrize("case", sorted(REGIME_CASES))
+def test_host_api_regime_source_and_fields(golden, tmp_path, case):
+    """REGIME::sourceExtension of the set-up state against the reference
+    build; E from set_vars is -v x B of the host's primitives; q, alpha, K and
+    dwdsb from set_vars / set_K / set_dwdsb equal the engine's fields for the
+    same state."""
+    from method_amd import engine as eng
+
+    run, srcname, shape, dims, _, _ = REGIME_CASES[case]
+    zs, zr = golden(srcname), golden(run)
+    a, _ = _run_regime(tmp_path, case, 0, zr["prims0"] if case == "kh2d" else None)
+    n = int(np.prod(shape))
+    parts, off = {}, 0
+    for name, k in (("src", 9), ("E", 3), ("q", 1), ("alpha", 1), ("K", 3), ("dwdsb", 36)):
+        parts[name] = a[off:off + k * n].reshape((k,) + shape)
+        off += k * n
+    assert off == a.size
+    rows = parity.field_errors(parity.interior(parts["src"], 4, dims), zs["src"], parity.SRMHD_CONS)
+    print(parity.report(f"host API REGIME::sourceExtension {case} vs reference", rows), flush=True)
+    for n_, dm, fm, rf, rg, pw in rows:
+        assert rf <= 1e-9, f"{n_}: {rf:.3e}"
+        if fm == 0.0:
+            assert dm == 0.0, f"{n_} must stay exactly zero"
+    P = zs["prims0"]
+    v, B = parity.interior(P[1:4], 4, dims), parity.interior(P[5:8], 4, dims)
+    assert np.allclose(parity.interior(parts["E"], 4, dims), -np.cross(v, B, axis=0), rtol=0,
+                       atol=1e-14 * max(1.0, np.abs(B).max()))
+    nx, ny, nz = (int(x) for x in zs["meta"][:3])
+    b = zs["bounds"]
+    gam, cfl, sig, cp = (float(x) for x in zs["params"])
+    bc = tuple(eng.PERIODIC if int(x) else eng.OUTFLOW for x in zs["periodic"])
+    cfg = eng.Config(nx=nx, ny=ny, nz=nz, xmin=b[0], xmax=b[1], ymin=b[2], ymax=b[3], zmin=b[4], zmax=b[5],
+                     gamma=gam, sigma=sig, cp=cp, cfl=cfl, weno_order=int(zs["meta"][4]), bc=bc)
+    with eng.Engine(cfg) as e:
+        e.set_prims(P)
+        e.set_model_extension(eng.EXT_REGIME)
+        E, q, alpha, dw, K = e.regime_fields()
+    assert np.array_equal(parts["E"], E)
+    assert np.array_equal(parts["q"][0], q)
+    assert np.array_equal(parts["alpha"][0], alpha)
+    assert np.array_equal(parts["K"], K)
+    assert np.array_equal(parts["dwdsb"], dw.reshape((36,) + shape))

@@ -5,6 +5,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -108,7 +109,9 @@ def test_initial_states_match_reference(tmp_path, golden, case, fixture):
 @pytest.mark.parametrize("example", ["KelvinHelmholtz/SingleFluidIdealRandomSerial",
                                      "OrszagTangVortex/SingleFluidIdealSerial",
                                      "BrioWuShockTube/SingleFluidIdealSerial",
-                                     "BrioWuShockTube/SingleFluidResistiveSerial"])
+                                     "BrioWuShockTube/SingleFluidResistiveSerial",
+                                     "REGIME/CurrentSheetSerial",
+                                     "REGIME/BrioWuSerial"])
 def test_more_reference_examples_build(tmp_path, example):
     """More of the reference's serial example mains, unmodified, compile and
     link against the host API."""
@@ -117,3 +120,125 @@ def test_more_reference_examples_build(tmp_path, example):
                         os.path.join(ROOT, "include"), src, "-o", str(tmp_path / "m"), "-L", LIB, "-lmethod_host",
                         "-lmethod_gpu", f"-Wl,-rpath,{LIB}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+INIT_MAIN = r"""
+#include <cstdio>
+#include "boundaryConds.h"
+#include "initFunc.h"
+#include "serialEnv.h"
+#include "simData.h"
+#include "simulation.h"
+#include "srmhd.h"
+int main(int argc, char **argv) {
+  SerialEnv env(&argc, &argv, 1, 1, 1);
+  Data a(128, 0, 0, -3.0, 3.0, -1.0, 1.0, -1.5, 1.5, 7.0, &env, 0.2, 4, 2.0, 10000);
+  SRMHD ma(&a);
+  Outflow oa(&a);
+  Simulation sa(&a, &env);
+  CurrentSheetSingleFluid ia(&a);
+  Data b(128, 0, 0, 0.0, 1.0, -1.0, 1.0, -1.5, 1.5, 0.4, &env, 0.15, 4, 2.0, 50);
+  SRMHD mb(&b);
+  Outflow ob(&b);
+  Simulation sb(&b, &env);
+  BrioWuSingleFluid ib(&b);
+  FILE *f = fopen(argv[1], "wb");
+  fwrite(a.prims, sizeof(double), (size_t)a.Nprims * a.Nx, f);
+  fwrite(b.prims, sizeof(double), (size_t)b.Nprims * b.Nx, f);
+  fclose(f);
+  return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_host_regime_example_initial_data_matches_reference(golden, tmp_path):
+    """CurrentSheetSingleFluid and BrioWuSingleFluid of the host API, on the
+    reference's two REGIME example set-ups, equal the reference build's
+    initial primitives bit for bit (no GPU call is made)."""
+    src, exe, out = tmp_path / "m.cc", tmp_path / "m", tmp_path / "p.bin"
+    src.write_text(INIT_MAIN)
+    subprocess.run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "method_amd", "host", "include"), "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", LIB, "-lmethod_host",
+                    "-lmethod_gpu", f"-Wl,-rpath,{LIB}"], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out).reshape(2, 8, 136, 1, 1)
+    assert np.array_equal(got[0], golden("currentsheet_128_regime_rksplit")["prims0"])
+    assert np.array_equal(got[1], golden("briowu_128_regime_rksplit")["prims0"])
+
+
+WENO_MAIN = r"""
+#include <cstdio>
+#include "fluxVectorSplitting.h"
+#include "serialEnv.h"
+#include "srmhd.h"
+int main(int argc, char **argv) {
+  SerialEnv env(&argc, &argv, 1, 1, 1);
+  Data d(16, 0, 0, 0.0, 1.0, -1.0, 1.0, -1.5, 1.5, 0.4, &env, 0.15, 4, 2.0, 50);
+  SRMHD m(&d);
+  Weno3 w3(&d);
+  Weno5 w5(&d);
+  FVS a(&d, &w3, &m);
+  printf("%d ", d.gpu.wenoOrder);
+  FVS b(&d, &w5, &m);
+  printf("%d\n", d.gpu.wenoOrder);
+  return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_host_fvs_takes_its_own_weno_order(tmp_path):
+    """With several WENO objects alive, the engine reconstructs with the order
+    of the one the FVS object was given (no GPU call is made)."""
+    src, exe = tmp_path / "w.cc", tmp_path / "w"
+    src.write_text(WENO_MAIN)
+    subprocess.run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "method_amd", "host", "include"), "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", LIB, "-lmethod_host",
+                    "-lmethod_gpu", f"-Wl,-rpath,{LIB}"], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split() == ["3", "5"]
+
+
+BIND_MAIN = r"""
+#include <cstdio>
+#include "boundaryConds.h"
+#include "fluxVectorSplitting.h"
+#include "rkSplit.h"
+#include "serialEnv.h"
+#include "srmhd.h"
+int main(int argc, char **argv) {
+  SerialEnv env(&argc, &argv, 1, 1, 1);
+  Data d(16, 0, 0, 0.0, 1.0, -1.0, 1.0, -1.5, 1.5, 0.4, &env, 0.15, 4, 2.0, 50);
+  SRMHD m(&d);
+  Weno3 w3(&d);
+  Weno5 w5(&d);
+  FVS f3(&d, &w3, &m);
+  Outflow out(&d);
+  Periodic per(&d);
+  RKSplit a(&d, &m, &out, &f3);
+  Flow flow(&d);
+  a.bind();
+  printf("%d %d %d %d %d\n", d.gpu.integrator, d.gpu.wenoOrder, d.gpu.bc[0], d.gpu.bc[1], d.gpu.bc[2]);
+  return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_host_integrator_binds_its_own_objects(tmp_path):
+    """With several boundary and WENO objects alive, an integrator's bind()
+    (run by every step and Simulation::set) makes the engine use the ones it
+    was handed (no GPU call is made)."""
+    src, exe = tmp_path / "b.cc", tmp_path / "b"
+    src.write_text(BIND_MAIN)
+    subprocess.run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "method_amd", "host", "include"), "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", LIB, "-lmethod_host",
+                    "-lmethod_gpu", f"-Wl,-rpath,{LIB}"], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from method_amd import engine as eng
+    want = [eng.RKSPLIT, 3] + [eng.OUTFLOW] * 3
+    assert [int(x) for x in r.stdout.split()] == want
