@@ -39,6 +39,11 @@ def parse():
     ap.add_argument("--no-srrmhd", action="store_true", help="skip the SRRMHD SSP2 1024^2 leg")
     ap.add_argument("--srrmhd-steps", type=int, default=10)
     ap.add_argument("--mlp-steps", type=int, default=12000)
+    ap.add_argument("--no-regime", action="store_true", help="skip the SRMHD + REGIME RKSplit leg")
+    ap.add_argument("--regime-n", type=int, default=256)
+    ap.add_argument("--regime-steps", type=int, default=10)
+    ap.add_argument("--regime-only", action="store_true",
+                    help="run only the REGIME leg and print its JSON (for profiling its kernels)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-members", type=int, default=64)
@@ -188,6 +193,46 @@ def srrmhd_leg(steps: int):
             "secant_steps_per_cell_solve": st["iterations"] / max(st["cells"], 1), "failures": st["failures"]}
 
 
+# Minimal HBM bytes of REGIME's source per cell: the eight primitives read and
+# the eight source values written once (every intermediate field kept on chip).
+REGIME_SOURCE_BYTES = 16 * 8
+
+
+def regime_leg(n: int, steps: int):
+    """Ideal SRMHD + the REGIME resistive extension, 3-D Kelvin-Helmholtz n^3,
+    WENO5, RKSplit (RK2, then the split source with REGIME's added), sigma 100."""
+    from method_amd import engine as eng
+    from method_amd import initdata as idt
+
+    g = idt.kh3d_grid(n)
+    cfg = eng.Config(nx=n, ny=n, nz=n, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                     weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), cfl=0.4, sigma=100.0,
+                     integrator=eng.RKSPLIT)
+    cells = cfg.interior_cells
+    with eng.Engine(cfg) as e:
+        e.set_prims(idt.kh_prims(g, seed=1234, bz=0.1))
+        e.set_model_extension(eng.EXT_REGIME)
+        for _ in range(2):
+            e.update_time()
+        base = 0.4 * (1.0 / n) / np.sqrt(3.0)
+        dts = [base * 0.1] * steps
+        e.sync()
+        w0 = time.perf_counter()
+        ev_ms, slots = e.timed_steps(dts)
+        wall = time.perf_counter() - w0
+        _, st = e.step(dts[-1], stats=True)
+    src_ms = slots.get("regime_source", 0.0)
+    padded = (n + 8) ** 3
+    achieved = REGIME_SOURCE_BYTES * padded / (src_ms * 1e-3) / 1e9 if src_ms > 0 else None
+    return {"workload": f"ideal SRMHD + REGIME 3-D Kelvin-Helmholtz {n}^3, WENO5, RKSplit, sigma 100, FP64",
+            "zone_updates_per_s": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "event_ms_per_step": ev_ms / steps, "launch_ms": slots,
+            "regime_source_roofline": {"bound": "hbm", "bytes_per_padded_cell": REGIME_SOURCE_BYTES,
+                                       "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": achieved / HBM_PEAK_GBS if achieved else None},
+            "newton_steps_per_cell": st["iterations"] / max(st["cells"], 1), "failures": st["failures"]}
+
+
 def batch_leg(steps: int, members: int):
     """64 independent 2-D ideal SRMHD KH runs of uneven size, WENO5, RK2, per-member
     dt: the batched engine (autotuned persistent stage kernels under a hipGraph)
@@ -248,6 +293,9 @@ def main():
         sys.exit(2)
     if args.batch_only:
         print(json.dumps({"batched_ensemble": batch_leg(args.batch_steps, args.batch_members)}))
+        return
+    if args.regime_only:
+        print(json.dumps({"regime_rksplit": regime_leg(args.regime_n, args.regime_steps)}))
         return
     from method_amd import engine as eng
     from method_amd import initdata as idt
@@ -331,6 +379,9 @@ def main():
     if not args.no_srrmhd:
         print("bench: SRRMHD SSP2 1024^2", file=sys.stderr, flush=True)
         out["srrmhd_ssp2"] = srrmhd_leg(args.srrmhd_steps)
+    if not args.no_regime:
+        print(f"bench: SRMHD + REGIME RKSplit {args.regime_n}^3", file=sys.stderr, flush=True)
+        out["regime_rksplit"] = regime_leg(args.regime_n, args.regime_steps)
     if not args.no_batch:
         print("bench: batched ensemble", file=sys.stderr, flush=True)
         out["batched_ensemble"] = batch_leg(args.batch_steps, args.batch_members)
