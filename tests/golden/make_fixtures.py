@@ -289,9 +289,29 @@ def regime_cases():
              interior_only=True, extra=["regime=1"])
 
 
+def regime_rkplus_cases():
+    """REGIME with the remaining RKPlus schemes (RK3, RK4 with its stored rhs,
+    RK4_10 whose stage after the second preparation forms the extension
+    source from the step's own primitives), 2-D KH WENO5, 3 steps."""
+    mixed = ("periodic", "outflow", "periodic")
+    g2 = idt.kh2d_grid(32)
+    p2 = idt.kh_prims(g2, seed=3, shear=0.5, bz=0.3)
+    for integ in ("rk3", "rk4", "rk4_10"):
+        run_case(f"kh2d_32_weno5_regime_{integ}", g2, p2, "srmhd", 5, integ, mixed, 3, [1, 3], sigma=100.0,
+                 interior_only=True, extra=["regime=1"])
+
+
+GROUPS = {"regime_rkplus": regime_rkplus_cases}
+
+
 def main():
     if not os.path.exists(REF_BIN):
         raise SystemExit(f"{REF_BIN} missing: build it with `make -C oracle/ref`")
+    if len(sys.argv) > 1:
+        # only the named groups
+        for name in sys.argv[1:]:
+            GROUPS[name]()
+        return
     # Ideal SRMHD 2-D Kelvin-Helmholtz, 64x64, WENO3, RK2, cfl 0.4, 10 steps,
     # periodic x / outflow y (the reference's Flow), seed 1234.
     g = idt.kh2d_grid(64)
@@ -315,6 +335,7 @@ def main():
     reference_examples()
     rkplus_cases()
     regime_cases()
+    regime_rkplus_cases()
 
 
 if __name__ == "__main__":

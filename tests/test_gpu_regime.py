@@ -13,7 +13,7 @@ from tests import parity
 
 pytestmark = pytest.mark.gpu
 
-INTEG = {"rksplit": eng.RKSPLIT, "rk2b": eng.RK2B}
+INTEG = {"rksplit": eng.RKSPLIT, "rk2b": eng.RK2B, "rk3": eng.RK3, "rk4": eng.RK4, "rk4_10": eng.RK4_10}
 
 
 def say(msg):
@@ -59,7 +59,9 @@ def test_regime_source_matches_reference(golden, name):
 # the current sheet's bar is the reference cons2prim's tolerance amplified by
 # the stiff source at sigma 1e4 (the oracle's test gives the reasoning)
 RUNS = [("currentsheet_128_regime_rksplit", "rksplit", 5e-9), ("briowu_128_regime_rksplit", "rksplit", 1e-9),
-        ("kh2d_32_weno5_regime_rk2b", "rk2b", 1e-9), ("kh3d_12_weno3_regime_rksplit", "rksplit", 1e-9)]
+        ("kh2d_32_weno5_regime_rk2b", "rk2b", 1e-9), ("kh3d_12_weno3_regime_rksplit", "rksplit", 1e-9),
+        ("kh2d_32_weno5_regime_rk3", "rk3", 1e-9), ("kh2d_32_weno5_regime_rk4", "rk4", 1e-9),
+        ("kh2d_32_weno5_regime_rk4_10", "rk4_10", 1e-9)]
 
 
 @pytest.mark.parametrize("name, integrator, tol", RUNS)

@@ -504,13 +504,16 @@ def test_regime_source_of_uniform_state_is_zero():
 # 2000 times smaller than tau (its other fields agree to 2.3e-13). The
 # extension's source itself equals the reference's bit for bit.
 REGIME_RUNS = [("currentsheet_128_regime_rksplit", "rksplit", 5e-9), ("briowu_128_regime_rksplit", "rksplit", 1e-9),
-               ("kh2d_32_weno5_regime_rk2b", "rk2b", 1e-9), ("kh3d_12_weno3_regime_rksplit", "rksplit", 1e-9)]
+               ("kh2d_32_weno5_regime_rk2b", "rk2b", 1e-9), ("kh3d_12_weno3_regime_rksplit", "rksplit", 1e-9),
+               ("kh2d_32_weno5_regime_rk3", "rk3", 1e-9), ("kh2d_32_weno5_regime_rk4", "rk4", 1e-9),
+               ("kh2d_32_weno5_regime_rk4_10", "rk4_10", 1e-9)]
 
 
 @pytest.mark.parametrize("name, integrator, tol", REGIME_RUNS)
 def test_regime_runs_match_reference(golden, name, integrator, tol):
-    """RKSplit and RK2B with the REGIME extension against the reference build:
-    its two REGIME examples at their own parameters, 2-D WENO5 and 3-D."""
+    """RKSplit and the RKPlus schemes with the REGIME extension against the
+    reference build: its two REGIME examples at their own parameters, 2-D
+    WENO5 (RK2B, RK3, RK4, RK4_10) and 3-D."""
     z = golden(name)
     case = regime_case(z)
     steps = int(z["meta"][5])
