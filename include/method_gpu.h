@@ -263,6 +263,14 @@ int mgpu_batch_sync(mgpu_batch *b);
  * affected. */
 int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz);
 
+/* Plan of the REGIME source (no reference counterpart; the reference forms
+ * D = M K per axis in arrays, REGIME.cc:514-774, and differences them,
+ * REGIME.cc:92-138): 1 (default) forms D in on-chip memory inside the launch
+ * that differences it; 0 stores D, K and alpha in the work array and
+ * differences them in a second launch. The source is identical bit for bit;
+ * mgpu_regime_fields always uses the second form, which leaves its fields. */
+int mgpu_set_regime_plan(mgpu_engine *e, int fused);
+
 /* Block until the engine's queued work is done. */
 int mgpu_sync(mgpu_engine *e);
 

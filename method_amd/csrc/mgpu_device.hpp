@@ -21,6 +21,29 @@ struct Geo {
   int dims;
 };
 
+// Logical block of hardware block bid out of nb. Blocks are dispatched round
+// robin over the 8 XCDs; this gives each XCD a contiguous run of logical
+// blocks, so work on neighbouring cells shares that XCD's L2.
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, local = bid >> 3;
+  return x * q + (x < r ? x : r) + local;
+}
+
+// Logical block of hardware block bid out of nb for grids whose neighbours
+// lie both nearby (rows) and far apart (planes): the 8 XCDs sweep the grid
+// together in chunks of 8 * run logical blocks, each XCD taking the same
+// contiguous run of `run` blocks of every chunk. With a chunk of one plane an
+// XCD keeps the same rows of every plane, so the neighbours along rows and
+// across planes are both in its own L2. Blocks past the last whole chunk
+// keep their own index.
+__device__ __forceinline__ int chunk_block(int bid, int nb, int run) {
+  const int chunk = 8 * run, full = nb / chunk * chunk;
+  if (bid >= full) return bid;
+  const int x = bid & 7, local = bid >> 3;
+  return local / run * chunk + x * run + local % run;
+}
+
 __device__ __forceinline__ int remap(int i, int n, int periodic) {
   if (i < 0) return periodic ? i + n : 0;
   if (i >= n) return periodic ? i - n : n - 1;

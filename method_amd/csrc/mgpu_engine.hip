@@ -70,6 +70,7 @@ struct mgpu_engine {
   double *rkU[4] = {}, *rkV[4] = {}, *rkA[4] = {}, *rkR = nullptr;
   double *rkF = nullptr;  // RK4_10: stored flux vectors [dir][var][cell]
   int extension = MGPU_EXT_NONE;         // model extension (REGIME) added by RKSplit / RKPlus
+  int regime_fused = 1;                  // REGIME's D and source in one launch
   double *rg_work = nullptr, *rg_src = nullptr;  // its padded work arrays and interior source
   double *staging = nullptr;
   size_t staging_elems = 0;
@@ -271,11 +272,11 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
 
 // The model extension's source of the state with prims V into e->rg_src
 // (REGIME::sourceExtension, REGIME.cc:83-138), or null without an extension.
-const double *extension_source(mgpu_engine *e, const double *V) {
+const double *extension_source(mgpu_engine *e, const double *V, bool keep_fields = false) {
   if (e->extension != MGPU_EXT_REGIME) return nullptr;
   const double dx[3] = {e->dx, e->dy, e->dz};
   launch_regime_source(e->geo, e->pad, dx, V, e->rg_work, e->rg_src, e->cfg.gamma, e->cfg.sigma,
-                       e->cfg.weno_order, e->stream);
+                       e->cfg.weno_order, e->regime_fused && !keep_fields, e->stream);
   mark(e, "regime_source");
   return e->rg_src;
 }
@@ -808,7 +809,7 @@ int mgpu_source_extension(mgpu_engine *e, double *source) {
 int mgpu_regime_fields(mgpu_engine *e, double *E, double *q, double *alpha, double *dwdsb, double *K) {
   if (!e) return fail("null engine");
   if (e->extension != MGPU_EXT_REGIME) return fail("mgpu_regime_fields: no model extension is set");
-  extension_source(e, e->V);
+  extension_source(e, e->V, true);
   HIP_OK(hipGetLastError());
   const RegimeFields f = regime_fields_of(e->pad, e->rg_work);
   const size_t nb = sizeof(double) * e->pad.ntot;
@@ -937,6 +938,12 @@ int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
 int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz) {
   if (!e) return fail("null engine");
   e->fused_yz = fused_yz ? 1 : 0;
+  return 0;
+}
+
+int mgpu_set_regime_plan(mgpu_engine *e, int fused) {
+  if (!e) return fail("null engine");
+  e->regime_fused = fused ? 1 : 0;
   return 0;
 }
 

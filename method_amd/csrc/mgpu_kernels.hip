@@ -20,14 +20,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ int xcd_block(int bid, int nb) {
-  // Blocks are dispatched round robin over the 8 XCDs; give each XCD a
-  // contiguous run of tiles so neighbouring tiles share its L2.
-  const int q = nb >> 3, r = nb & 7;
-  const int x = bid & 7, local = bid >> 3;
-  return x * q + (x < r ? x : r) + local;
-}
-
 // Block-level reduction of C2P counters into one of the stats buckets.
 __device__ __forceinline__ void reduce_stats(C2PCounters *st, int cells, int iters, int fails, int maxit) {
   if (!st) return;

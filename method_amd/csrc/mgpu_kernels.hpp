@@ -134,8 +134,12 @@ RegimeFields regime_fields_of(const PadGeo &pg, double *work);
 // dw/ds_b [12*3][padded cell] from the fields of the last source evaluation.
 void launch_regime_dwdsb(const Geo &g, const PadGeo &pg, const double *V, const double *work, double sigma,
                          double *out, hipStream_t s);
+// REGIME's source of the primitives V into src (8 x interior cells). fused:
+// D and its differences in one launch (nothing but E, q and the face fluxes
+// in the work array); otherwise two launches that also leave D, K and alpha
+// there. The two give the same source bit for bit.
 void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], const double *V, double *work,
-                          double *src, double gam, double sigma, int order, hipStream_t s);
+                          double *src, double gam, double sigma, int order, bool fused, hipStream_t s);
 void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, hipStream_t s);
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s);

@@ -16,18 +16,28 @@
 //   source = centred difference of D, interior    (REGIME.cc:92-138)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mgpu_device.hpp"
 #include "mgpu_kernels.hpp"
+
+// No implicit a*b+c contraction in this file's arithmetic: whether the
+// compiler fuses a product into an add depends on the surrounding code, and
+// the one-launch and two-launch forms of D must round identically. (The
+// shared WENO above keeps its explicit fma form.)
+#pragma clang fp contract(off)
 
 namespace mgpu {
 namespace {
 
 constexpr int kRegimeThreads = 256;
 
-// the grid spacings as the reference's Data holds them (divisions use dx
-// itself, not a reciprocal)
+// The grid spacings as the reference's Data holds them, and 1 / (2 d) for
+// the centred differences of K and of D (the reference divides by 2 d; the
+// product differs from that quotient by at most one rounding).
 struct Spacing {
   double d[3];
+  double ih2[3];
 };
 
 __device__ __forceinline__ void padded_pos(const PadGeo &pg, long long p, int pos[3]) {
@@ -53,7 +63,7 @@ __device__ __forceinline__ void load_prims(const double *V, long long vs, long l
 }
 
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, const double *V, double *E) {
-  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  const long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
   padded_pos(pg, p, pos);
@@ -65,8 +75,8 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, c
 }
 
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_q(Geo g, PadGeo pg, Spacing sp, const double *E,
-                                                             double *Q) {
-  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+                                                             double *Q, int run) {
+  const long long p = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
   padded_pos(pg, p, pos);
@@ -111,9 +121,9 @@ __device__ __forceinline__ void stiff_flux(int ax, const double P[8], double fb[
 // is a template parameter so every per-axis array index is a constant.
 template <int ORDER, int AX>
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_flux(Geo g, PadGeo pg, const double *V, const double *E,
-                                                                double *F) {
+                                                                double *F, int run) {
   constexpr int R = Weno<ORDER>::R;
-  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  const long long p = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
   padded_pos(pg, p, pos);
@@ -207,76 +217,234 @@ __device__ __forceinline__ void regime_dfdw(int ax, double gam, const double P[8
   j4 = gam * P[4] / (gam - 1);
 }
 
-// K (REGIME.cc:293-370), then D = M K per active axis with M as
-// REGIME::set_Dx/Dy/Dz form it (REGIME.cc:551-774).
-__global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, Spacing sp, const double *V,
-                                                             const double *E, const double *Q, const double *F,
-                                                             double *D, double *Kout, double *Aout, double gam,
-                                                             double sig) {
-  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
-  if (p >= pg.ntot) return;
+// K at padded cell p (REGIME.cc:293-370): centred differences of the E-face
+// fluxes, zero outside [1, N - 1) along each axis.
+__device__ __forceinline__ void regime_k(const Geo &g, const PadGeo &pg, const Spacing &sp, const double *F,
+                                         long long p, const int pos[3], double K[3]) {
   const long long nt = pg.ntot;
-  int pos[3];
-  padded_pos(pg, p, pos);
-  double K[3] = {0.0, 0.0, 0.0};
+  K[0] = K[1] = K[2] = 0.0;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
     if (ax >= g.dims || pos[ax] < 1 || pos[ax] >= pg.N[ax] - 1) continue;
     const long long s = pstride(pg, ax);
     const double *Fa = F + (long long)ax * 3 * nt;
-    const double h2 = 2 * sp.d[ax];
-    for (int v = 0; v < 3; ++v) K[v] += Fa[v * nt + p + s] / h2 - Fa[v * nt + p - s] / h2;
+    for (int v = 0; v < 3; ++v) K[v] += (Fa[v * nt + p + s] - Fa[v * nt + p - s]) * sp.ih2[ax];
   }
-  double P[8];
-  load_prims(V, g.vstride, image_of(g, pg, pos), P);
-  const double Ec[3] = {E[p], E[nt + p], E[2 * nt + p]};
-  const double q = Q[p];
-  const double qs = q * q + sig * sig;
-  const double alpha = 1 / (qs * (qs + sig * sig * (P[5] * P[5] + P[6] * P[6] + P[7] * P[7])));
-  for (int v = 0; v < 3; ++v) Kout[v * nt + p] = K[v];
-  Aout[p] = alpha;
+}
+
+// The per-cell quantities D is formed from: primitives, E, q, alpha
+// (REGIME.cc:141-224) and dw/ds_b (REGIME.cc:226-290).
+struct RegimeCell {
+  double P[8], Ec[3], q, alpha;
   double dw[12][3];
-  regime_dwdsb(q, sig, P, Ec, alpha, dw);
+};
+
+__device__ __forceinline__ void regime_cell(const Geo &g, const PadGeo &pg, const double *V, const double *E,
+                                            const double *Q, long long p, const int pos[3], double sig,
+                                            RegimeCell &c) {
+  const long long nt = pg.ntot;
+  load_prims(V, g.vstride, image_of(g, pg, pos), c.P);
+  c.Ec[0] = E[p];
+  c.Ec[1] = E[nt + p];
+  c.Ec[2] = E[2 * nt + p];
+  c.q = Q[p];
+  const double qs = c.q * c.q + sig * sig;
+  c.alpha = 1 / (qs * (qs + sig * sig * (c.P[5] * c.P[5] + c.P[6] * c.P[6] + c.P[7] * c.P[7])));
+  regime_dwdsb(c.q, sig, c.P, c.Ec, c.alpha, c.dw);
+}
+
+// D = M K along axis AX with M as REGIME::set_Dx/Dy/Dz form it
+// (REGIME.cc:551-774): M = -df/dw dw/ds_b.
+template <int AX>
+__device__ __forceinline__ void regime_d_axis(const RegimeCell &c, const double K[3], double gam, double D[8]) {
+  double J[5][6], j0, j4;
+  regime_dfdw(AX, gam, c.P, c.Ec, J, j0, j4);
+  constexpr int a = 1 + AX;
+  double M[8][3];
+  for (int l = 0; l < 8; ++l)
+    for (int m = 0; m < 3; ++m) M[l][m] = 0.0;
+  for (int m = 0; m < 3; ++m) {
+    M[0][m] -= j0 * c.dw[a][m];
+    M[4][m] -= j4 * c.dw[a][m];
+  }
+  // the induction rows with one unit entry each
+  if (AX == 0) {
+    for (int m = 0; m < 3; ++m) M[6][m] -= (-c.dw[10][m]);
+    for (int m = 0; m < 3; ++m) M[7][m] -= c.dw[9][m];
+  } else if (AX == 1) {
+    for (int m = 0; m < 3; ++m) M[5][m] -= c.dw[10][m];
+    for (int m = 0; m < 3; ++m) M[7][m] -= (-c.dw[8][m]);
+  } else {
+    for (int m = 0; m < 3; ++m) M[5][m] -= (-c.dw[9][m]);
+    for (int m = 0; m < 3; ++m) M[6][m] -= c.dw[8][m];
+  }
+  for (int l = 1; l < 5; ++l)
+    for (int m = 0; m < 3; ++m)
+      for (int n = 0; n < 6; ++n) M[l][m] = fma(-J[l][n], c.dw[5 + n][m], M[l][m]);
+  for (int l = 0; l < 8; ++l) D[l] = fma(M[l][2], K[2], fma(M[l][1], K[1], M[l][0] * K[0]));
+}
+
+// K (REGIME.cc:293-370), then D = M K per active axis, into the work arrays
+// (the fields the C ABI reads back, and the two-launch source).
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, Spacing sp, const double *V,
+                                                             const double *E, const double *Q, const double *F,
+                                                             double *D, double *Kout, double *Aout, double gam,
+                                                             double sig, int run) {
+  const long long p = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
+  if (p >= pg.ntot) return;
+  const long long nt = pg.ntot;
+  int pos[3];
+  padded_pos(pg, p, pos);
+  double K[3];
+  regime_k(g, pg, sp, F, p, pos, K);
+  RegimeCell c;
+  regime_cell(g, pg, V, E, Q, p, pos, sig, c);
+  for (int v = 0; v < 3; ++v) Kout[v * nt + p] = K[v];
+  Aout[p] = c.alpha;
+  double Dl[8];
+  regime_d_axis<0>(c, K, gam, Dl);
+  for (int l = 0; l < 8; ++l) D[l * nt + p] = Dl[l];
+  if (g.dims > 1) {
+    regime_d_axis<1>(c, K, gam, Dl);
+    for (int l = 0; l < 8; ++l) D[(8 + l) * nt + p] = Dl[l];
+  }
+  if (g.dims > 2) {
+    regime_d_axis<2>(c, K, gam, Dl);
+    for (int l = 0; l < 8; ++l) D[(16 + l) * nt + p] = Dl[l];
+  }
+}
+
+// D at padded position ppos along axis AX, zero outside the padded grid.
+template <int AX>
+__device__ __forceinline__ void regime_d_at(const Geo &g, const PadGeo &pg, const Spacing &sp, const double *V,
+                                            const double *E, const double *Q, const double *F, const int ppos[3],
+                                            double gam, double sig, double D[8]) {
+  bool in = true;
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    if (ax >= g.dims) break;
-    double J[5][6], j0, j4;
-    regime_dfdw(ax, gam, P, Ec, J, j0, j4);
-    const int a = 1 + ax;
-    double M[8][3];
-    for (int l = 0; l < 8; ++l)
-      for (int m = 0; m < 3; ++m) M[l][m] = 0.0;
-    for (int m = 0; m < 3; ++m) {
-      M[0][m] -= j0 * dw[a][m];
-      M[4][m] -= j4 * dw[a][m];
+  for (int ax = 0; ax < 3; ++ax) in = in && ppos[ax] >= 0 && ppos[ax] < pg.N[ax];
+  if (!in) {
+    for (int l = 0; l < 8; ++l) D[l] = 0.0;
+    return;
+  }
+  const long long p = ((long long)ppos[0] * pg.N[1] + ppos[1]) * pg.N[2] + ppos[2];
+  double K[3];
+  regime_k(g, pg, sp, F, p, ppos, K);
+  RegimeCell c;
+  regime_cell(g, pg, V, E, Q, p, ppos, sig, c);
+  regime_d_axis<AX>(c, K, gam, D);
+}
+
+// Interior cells per block of the fused D + source kernel, by grid dimension.
+template <int DIMS>
+struct DsTile {
+  static constexpr int T0 = DIMS == 3 ? 4 : (DIMS == 2 ? 16 : 256);
+  static constexpr int T1 = DIMS == 3 ? 8 : (DIMS == 2 ? 16 : 1);
+  static constexpr int T2 = DIMS == 3 ? 8 : 1;
+  // D along axis a is needed on the tile widened by one cell each way along a
+  static constexpr int kItems0 = (T0 + 2) * T1 * T2;
+  static constexpr int kItems1 = DIMS > 1 ? T0 * (T1 + 2) * T2 : 0;
+  static constexpr int kItems2 = DIMS > 2 ? T0 * T1 * (T2 + 2) : 0;
+  static constexpr int kItems = kItems0 + kItems1 + kItems2;
+  static_assert(T0 * T1 * T2 == kRegimeThreads, "one interior cell per thread");
+};
+
+// D = M K and the source's centred differences of it in one launch: each
+// block forms D along every axis on its tile widened by one cell along that
+// axis, in LDS, then differences it (REGIME.cc:92-138), so D, K and alpha
+// never go to HBM. LDS holds, per axis a, D_a on the tile widened along a.
+// Pass 1: every thread forms K and the cell quantities of its own tile cell
+// once and D along each axis there. Pass 2: the cells one step outside the
+// tile's faces, each only along the axis crossing that face. D is the same
+// per-cell computation as k_regime_d's.
+template <int DIMS>
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_ds(Geo g, PadGeo pg, Spacing sp, const double *V,
+                                                              const double *E, const double *Q, const double *F,
+                                                              double *src, double gam, double sig, int run) {
+  using T = DsTile<DIMS>;
+  constexpr int T0 = T::T0, T1 = T::T1, T2 = T::T2;
+  constexpr int B1 = T::kItems0, B2 = T::kItems0 + T::kItems1;  // first LDS slot of axes 1 and 2
+  __shared__ double sD[8][T::kItems];
+  const int nt0 = (g.n[0] + T0 - 1) / T0, nt1 = (g.n[1] + T1 - 1) / T1, nt2 = (g.n[2] + T2 - 1) / T2;
+  const int lb = chunk_block(blockIdx.x, gridDim.x, run);
+  const int t2 = lb % nt2, t1 = (lb / nt2) % nt1, t0 = lb / (nt2 * nt1);
+  if (t0 >= nt0) return;  // whole block: no barrier is skipped by part of it
+  const int o[3] = {t0 * T0 + pg.off[0], t1 * T1 + pg.off[1], t2 * T2 + pg.off[2]};  // padded tile origin
+  const int tid = threadIdx.x;
+  const int c2 = tid % T2, c1 = (tid / T2) % T1, c0 = tid / (T2 * T1);
+  // LDS slot of widened-tile cell (i0, i1, i2) of axis 0, 1, 2
+  auto slot0 = [&](int i0, int i1, int i2) { return (i0 * T1 + i1) * T2 + i2; };
+  auto slot1 = [&](int i0, int i1, int i2) { return B1 + (i0 * (T1 + 2) + i1) * T2 + i2; };
+  auto slot2 = [&](int i0, int i1, int i2) { return B2 + (i0 * T1 + i1) * (T2 + 2) + i2; };
+  {
+    // pass 1: the thread's own tile cell, D along every axis
+    const int pp[3] = {o[0] + c0, o[1] + c1, o[2] + c2};
+    bool in = true;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) in = in && pp[ax] < pg.N[ax];
+    double D0[8] = {}, D1[8] = {}, D2[8] = {};
+    if (in) {
+      const long long p = ((long long)pp[0] * pg.N[1] + pp[1]) * pg.N[2] + pp[2];
+      double K[3];
+      regime_k(g, pg, sp, F, p, pp, K);
+      RegimeCell c;
+      regime_cell(g, pg, V, E, Q, p, pp, sig, c);
+      regime_d_axis<0>(c, K, gam, D0);
+      if (DIMS > 1) regime_d_axis<(DIMS > 1 ? 1 : 0)>(c, K, gam, D1);
+      if (DIMS > 2) regime_d_axis<(DIMS > 2 ? 2 : 0)>(c, K, gam, D2);
     }
-    // the induction rows with one unit entry each
-    if (ax == 0) {
-      for (int m = 0; m < 3; ++m) M[6][m] -= (-dw[10][m]);
-      for (int m = 0; m < 3; ++m) M[7][m] -= dw[9][m];
-    } else if (ax == 1) {
-      for (int m = 0; m < 3; ++m) M[5][m] -= dw[10][m];
-      for (int m = 0; m < 3; ++m) M[7][m] -= (-dw[8][m]);
-    } else {
-      for (int m = 0; m < 3; ++m) M[5][m] -= (-dw[9][m]);
-      for (int m = 0; m < 3; ++m) M[6][m] -= dw[8][m];
+    for (int l = 0; l < 8; ++l) sD[l][slot0(c0 + 1, c1, c2)] = D0[l];
+    if (DIMS > 1)
+      for (int l = 0; l < 8; ++l) sD[l][slot1(c0, c1 + 1, c2)] = D1[l];
+    if (DIMS > 2)
+      for (int l = 0; l < 8; ++l) sD[l][slot2(c0, c1, c2 + 1)] = D2[l];
+  }
+  {
+    // pass 2: the face neighbours of the tile, one axis each
+    constexpr int H0 = 2 * T1 * T2, H1 = DIMS > 1 ? 2 * T0 * T2 : 0, H2 = DIMS > 2 ? 2 * T0 * T1 : 0;
+    for (int h = tid; h < H0 + H1 + H2; h += kRegimeThreads) {
+      double D[8];
+      int slot;
+      if (h < H0) {
+        const int side = h / (T1 * T2), r = h % (T1 * T2), i1 = r / T2, i2 = r % T2;
+        const int pp[3] = {o[0] + (side ? T0 : -1), o[1] + i1, o[2] + i2};
+        regime_d_at<0>(g, pg, sp, V, E, Q, F, pp, gam, sig, D);
+        slot = slot0(side ? T0 + 1 : 0, i1, i2);
+      } else if (DIMS > 1 && h < H0 + H1) {
+        const int hh = h - H0, side = hh / (T0 * T2), r = hh % (T0 * T2), i0 = r / T2, i2 = r % T2;
+        const int pp[3] = {o[0] + i0, o[1] + (side ? T1 : -1), o[2] + i2};
+        regime_d_at<(DIMS > 1 ? 1 : 0)>(g, pg, sp, V, E, Q, F, pp, gam, sig, D);
+        slot = slot1(i0, side ? T1 + 1 : 0, i2);
+      } else {
+        const int hh = h - H0 - H1, side = hh / (T0 * T1), r = hh % (T0 * T1), i0 = r / T1, i1 = r % T1;
+        const int pp[3] = {o[0] + i0, o[1] + i1, o[2] + (side ? T2 : -1)};
+        regime_d_at<(DIMS > 2 ? 2 : 0)>(g, pg, sp, V, E, Q, F, pp, gam, sig, D);
+        slot = slot2(i0, i1, side ? T2 + 1 : 0);
+      }
+      for (int l = 0; l < 8; ++l) sD[l][slot] = D[l];
     }
-    for (int l = 1; l < 5; ++l)
-      for (int m = 0; m < 3; ++m)
-        for (int n = 0; n < 6; ++n) M[l][m] -= J[l][n] * dw[5 + n][m];
-    double *Da = D + (long long)ax * 8 * nt;
-    for (int l = 0; l < 8; ++l) {
-      double acc = 0.0;
-      for (int m = 0; m < 3; ++m) acc += M[l][m] * K[m];
-      Da[l * nt + p] = acc;
+  }
+  __syncthreads();
+  const int c[3] = {o[0] - pg.off[0] + c0, o[1] - pg.off[1] + c1, o[2] - pg.off[2] + c2};
+  if (c[0] >= g.n[0] || c[1] >= g.n[1] || c[2] >= g.n[2]) return;
+  // LDS slots of the cell's two neighbours along each axis
+  const int lo[3] = {slot0(c0, c1, c2), slot1(c0, c1, c2), slot2(c0, c1, c2)};
+  const int hi[3] = {slot0(c0 + 2, c1, c2), slot1(c0, c1 + 2, c2), slot2(c0, c1, c2 + 2)};
+  const long long q = c[0] * g.stride[0] + c[1] * g.stride[1] + (long long)c[2];
+  for (int v = 0; v < 8; ++v) {
+    double acc = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < DIMS; ++ax) {
+      const double t = (sD[v][hi[ax]] - sD[v][lo[ax]]) * sp.ih2[ax];
+      acc = ax == 0 ? t : acc + t;
     }
+    src[v * g.vstride + q] = acc;
   }
 }
 
 // sourceExtension over interior cells (REGIME.cc:92-138): variables 0..7.
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_source(Geo g, PadGeo pg, Spacing sp, const double *D,
-                                                                  double *src) {
-  const long long q = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+                                                                  double *src, int run) {
+  const long long q = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
   if (q >= g.vstride) return;
   const long long nt = pg.ntot;
   const int c[3] = {(int)(q / g.stride[0]), (int)((q / g.stride[1]) % g.n[1]), (int)(q % g.n[2])};
@@ -289,7 +457,7 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_source(Geo g, PadGeo 
       if (ax >= g.dims) break;
       const long long s = pstride(pg, ax);
       const double *Dv = D + ((long long)ax * 8 + v) * nt;
-      const double t = (Dv[p + s] - Dv[p - s]) / (2 * sp.d[ax]);
+      const double t = (Dv[p + s] - Dv[p - s]) * sp.ih2[ax];
       acc = ax == 0 ? t : acc + t;
     }
     src[v * g.vstride + q] = acc;
@@ -301,7 +469,7 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_source(Geo g, PadGeo 
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_dwdsb(Geo g, PadGeo pg, const double *V, const double *E,
                                                                  const double *Q, const double *A, double sig,
                                                                  double *out) {
-  const long long p = (long long)blockIdx.x * kRegimeThreads + threadIdx.x;
+  const long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   const long long nt = pg.ntot;
   int pos[3];
@@ -342,26 +510,51 @@ void launch_regime_dwdsb(const Geo &g, const PadGeo &pg, const double *V, const 
 }
 
 void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], const double *V, double *work,
-                          double *src, double gam, double sigma, int order, hipStream_t s) {
+                          double *src, double gam, double sigma, int order, bool fused, hipStream_t s) {
   const long long nt = pg.ntot;
-  const Spacing sp = {{dx[0], dx[1], dx[2]}};
+  const Spacing sp = {{dx[0], dx[1], dx[2]}, {1.0 / (2 * dx[0]), 1.0 / (2 * dx[1]), 1.0 / (2 * dx[2])}};
   const RegimeFields fl = regime_fields_of(pg, work);
   double *E = fl.E, *Q = fl.q, *F = fl.F, *D = fl.D;
   hipLaunchKernelGGL(k_regime_e, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E);
-  hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, E, Q);
+  // an eighth of the blocks of one padded (stencil kernels) or interior
+  // plane: each XCD then keeps the same rows of every plane, so both its
+  // x and its y neighbours stay in its L2
+  const int run_pad = (int)std::max<long long>(1, (long long)pg.N[1] * pg.N[2] / kRegimeThreads / 8);
+  const int run_int = (int)std::max<long long>(1, (long long)g.n[1] * g.n[2] / kRegimeThreads / 8);
+  hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, E, Q, run_pad);
   const dim3 grid(blocks(nt)), block(kRegimeThreads);
   if (order == 3) {
-    hipLaunchKernelGGL((k_regime_flux<3, 0>), grid, block, 0, s, g, pg, V, E, F);
-    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<3, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt);
-    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<3, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt);
+    hipLaunchKernelGGL((k_regime_flux<3, 0>), grid, block, 0, s, g, pg, V, E, F, run_pad);
+    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<3, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt, run_pad);
+    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<3, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt, run_pad);
   } else {
-    hipLaunchKernelGGL((k_regime_flux<5, 0>), grid, block, 0, s, g, pg, V, E, F);
-    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<5, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt);
-    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<5, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt);
+    hipLaunchKernelGGL((k_regime_flux<5, 0>), grid, block, 0, s, g, pg, V, E, F, run_pad);
+    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<5, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt, run_pad);
+    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<5, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt, run_pad);
+  }
+  if (fused) {
+    if (g.dims == 3) {
+      using T = DsTile<3>;
+      const unsigned nb = (unsigned)(((g.n[0] + T::T0 - 1) / T::T0) * ((g.n[1] + T::T1 - 1) / T::T1) *
+                                     ((g.n[2] + T::T2 - 1) / T::T2));
+      const int run = std::max(1, ((g.n[1] + T::T1 - 1) / T::T1) * ((g.n[2] + T::T2 - 1) / T::T2) / 8);
+      hipLaunchKernelGGL(k_regime_ds<3>, dim3(nb), block, 0, s, g, pg, sp, V, E, Q, F, src, gam, sigma, run);
+    } else if (g.dims == 2) {
+      using T = DsTile<2>;
+      const unsigned nb = (unsigned)(((g.n[0] + T::T0 - 1) / T::T0) * ((g.n[1] + T::T1 - 1) / T::T1));
+      const int run = std::max(1, ((g.n[1] + T::T1 - 1) / T::T1) / 8);
+      hipLaunchKernelGGL(k_regime_ds<2>, dim3(nb), block, 0, s, g, pg, sp, V, E, Q, F, src, gam, sigma, run);
+    } else {
+      using T = DsTile<1>;
+      const unsigned nb = (unsigned)((g.n[0] + T::T0 - 1) / T::T0);
+      hipLaunchKernelGGL(k_regime_ds<1>, dim3(nb), block, 0, s, g, pg, sp, V, E, Q, F, src, gam, sigma, 1);
+    }
+    return;
   }
   hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, V, E, Q, F, D, fl.K,
-                     fl.alpha, gam, sigma);
-  hipLaunchKernelGGL(k_regime_source, dim3(blocks(g.vstride)), dim3(kRegimeThreads), 0, s, g, pg, sp, D, src);
+                     fl.alpha, gam, sigma, run_pad);
+  hipLaunchKernelGGL(k_regime_source, dim3(blocks(g.vstride)), dim3(kRegimeThreads), 0, s, g, pg, sp, D, src,
+                     run_int);
 }
 
 }  // namespace mgpu

@@ -28,7 +28,7 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_set_regime_plan", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
     "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_sizes", "mgpu_batch_set_prims", "mgpu_batch_upload_state",
@@ -121,6 +121,7 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_set_c2p_guess.argtypes = [E, ctypes.c_int, fp, fp]
     L.mgpu_sync.argtypes = [E]
     L.mgpu_set_sweep_plan.argtypes = [E, ctypes.c_int]
+    L.mgpu_set_regime_plan.argtypes = [E, ctypes.c_int]
     L.mgpu_timed_c2p.argtypes = [E, ctypes.c_int, dp, sp]
     L.mgpu_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(E)]
     L.mgpu_mlp_destroy.argtypes = [E]
@@ -292,6 +293,11 @@ class Engine:
     def set_sweep_plan(self, fused_yz: bool):
         """3-D grids: separate z and y sweeps (default) or one fused launch."""
         self._check(self.L.mgpu_set_sweep_plan(self._h, 1 if fused_yz else 0), "mgpu_set_sweep_plan")
+
+    def set_regime_plan(self, fused: bool):
+        """REGIME source: D formed on chip in the differencing launch (default)
+        or stored and differenced by a second launch; bit-identical."""
+        self._check(self.L.mgpu_set_regime_plan(self._h, 1 if fused else 0), "mgpu_set_regime_plan")
 
     # model / flux-method calls on the resident state ----------------------
     def fnet(self) -> np.ndarray:

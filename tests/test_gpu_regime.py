@@ -214,3 +214,41 @@ def test_regime_fields_match_reference_known_answers(axis):
     assert abs(alpha[cell] - 1.3825277485504367e-07) <= 1e-13
     for l, m, want, tol in REGIME_KAT[name]:
         assert abs(dw[l, m][cell] - want) <= tol, (name, l, m)
+
+
+@pytest.mark.parametrize("name", ["regime_src_currentsheet_128", "regime_src_briowu_128",
+                                  "regime_src_kh2d_32_weno5", "regime_src_kh3d_12_weno3"])
+def test_regime_fused_source_equals_two_launch_source_bitwise(golden, name):
+    """D formed on chip in the differencing launch (the default plan) gives
+    the source of the two-launch plan, which stores D, bit for bit."""
+    z = golden(name)
+    cfg = config_of(z)
+    say(f"gpu: REGIME fused vs two-launch source {name}")
+    out = []
+    for fused in (True, False):
+        with eng.Engine(cfg) as e:
+            e.set_prims(z["prims0"])
+            e.set_model_extension(eng.EXT_REGIME)
+            e.set_regime_plan(fused)
+            out.append(e.source_extension())
+    assert np.array_equal(out[0], out[1])
+    assert np.abs(out[0]).max() > 0
+
+
+def test_regime_fused_plan_run_equals_two_launch_run_bitwise(golden):
+    """Two RKSplit steps of 3-D KH with REGIME under both plans: the same
+    state bit for bit."""
+    z = golden("kh3d_12_weno3_regime_rksplit")
+    cfg = config_of(z)
+    say("gpu: REGIME fused vs two-launch, kh3d 12^3 RKSplit 2 steps")
+    states = []
+    for fused in (True, False):
+        with eng.Engine(cfg) as e:
+            e.set_prims(z["prims0"])
+            e.set_model_extension(eng.EXT_REGIME)
+            e.set_regime_plan(fused)
+            for _ in range(2):
+                assert e.update_time() == 0
+            states.append(e.download())
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
