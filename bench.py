@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-c2p", action="store_true", help="skip the cons2prim guess comparison")
     ap.add_argument("--no-srrmhd", action="store_true", help="skip the SRRMHD SSP2 1024^2 leg")
     ap.add_argument("--srrmhd-steps", type=int, default=10)
+    ap.add_argument("--srrmhd-only", action="store_true",
+                    help="run only the SRRMHD SSP2 leg and print its JSON (for profiling its kernels)")
     ap.add_argument("--mlp-steps", type=int, default=12000)
     ap.add_argument("--no-regime", action="store_true", help="skip the SRMHD + REGIME RKSplit leg")
     ap.add_argument("--regime-n", type=int, default=256)
@@ -293,6 +295,9 @@ def main():
         sys.exit(2)
     if args.batch_only:
         print(json.dumps({"batched_ensemble": batch_leg(args.batch_steps, args.batch_members)}))
+        return
+    if args.srrmhd_only:
+        print(json.dumps({"srrmhd_ssp2": srrmhd_leg(args.srrmhd_steps)}))
         return
     if args.regime_only:
         print(json.dumps({"regime_rksplit": regime_leg(args.regime_n, args.regime_steps)}))
