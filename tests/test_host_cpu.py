@@ -242,3 +242,57 @@ def test_host_integrator_binds_its_own_objects(tmp_path):
     from method_amd import engine as eng
     want = [eng.RKSPLIT, 3] + [eng.OUTFLOW] * 3
     assert [int(x) for x in r.stdout.split()] == want
+
+
+EXT_MAIN = r"""
+#include <cstdio>
+#include <stdexcept>
+#include "REGIME.h"
+#include "RKPlus.h"
+#include "boundaryConds.h"
+#include "fluxVectorSplitting.h"
+#include "rkSplit.h"
+#include "serialEnv.h"
+#include "simulation.h"
+#include "srmhd.h"
+class OtherExtension : public ModelExtension {
+ public:
+  OtherExtension(Data *d) : ModelExtension(d) { sourceExists = true; }
+};
+int main(int argc, char **argv) {
+  SerialEnv env(&argc, &argv, 1, 1, 1);
+  Data d(16, 0, 0, 0.0, 1.0, -1.0, 1.0, -1.5, 1.5, 0.4, &env, 0.15, 4, 2.0, 50);
+  SRMHD m(&d);
+  Weno3 w(&d);
+  FVS f(&d, &w, &m);
+  Outflow b(&d);
+  Simulation sim(&d, &env);
+  OtherExtension other(&d);
+  int rejected = 0;
+  try { RKSplit t(&d, &m, &b, &f, &other); } catch (const std::invalid_argument &) { ++rejected; }
+  try { RK3 t(&d, &m, &b, &f, &other); } catch (const std::invalid_argument &) { ++rejected; }
+  REGIME r(&d, &f);
+  RKSplit ok(&d, &m, &b, &f, &r);
+  int logic = 0;
+  try { r.set_Dx(d.cons, d.prims, d.aux); } catch (const std::logic_error &) { ++logic; }
+  try { r.set_dfydw(d.cons, d.prims, d.aux); } catch (const std::logic_error &) { ++logic; }
+  printf("%d %d %d %d\n", rejected, logic, (int)r.sourceExists, d.sourceExtension != nullptr);
+  return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_host_model_extension_errors(tmp_path):
+    """A model extension other than REGIME is refused by RKSplit and the
+    RKPlus schemes; REGIME's per-axis matrix setters throw std::logic_error;
+    REGIME's constructor sets sourceExists and allocates Data::sourceExtension
+    as the reference's does (no GPU call is made)."""
+    src, exe = tmp_path / "x.cc", tmp_path / "x"
+    src.write_text(EXT_MAIN)
+    subprocess.run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "method_amd", "host", "include"), "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", LIB, "-lmethod_host",
+                    "-lmethod_gpu", f"-Wl,-rpath,{LIB}"], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split() == ["2", "2", "1", "1"]
