@@ -2,6 +2,7 @@
 ABI: its source against the reference build's REGIME::sourceExtension, and
 RKSplit / RK2B runs with it against the reference's trajectories and the
 oracle. Needs an MI355X."""
+import ctypes
 import sys
 
 import numpy as np
@@ -252,3 +253,39 @@ def test_regime_fused_plan_run_equals_two_launch_run_bitwise(golden):
             states.append(e.download())
     for a, b in zip(*states):
         assert np.array_equal(a, b)
+
+
+# ragged grids for the source's tiles (4x8x8 in 3-D, 16x16 in 2-D), mixed boundaries, both orders
+RAGGED = [(13, 10, 9, 5, (1, 0, 1)), (9, 17, 11, 3, (0, 1, 0)), (37, 21, 0, 5, (1, 0, 1)), (19, 35, 0, 3, (0, 0, 0))]
+
+
+@pytest.mark.parametrize("nx, ny, nz, order, per", RAGGED)
+def test_regime_source_on_ragged_grids_matches_oracle(nx, ny, nz, order, per):
+    """REGIME's source on grids that leave partial tiles at every face, with
+    periodic and outflow axes, against the oracle's restatement of
+    REGIME::sourceExtension; both source plans."""
+    from method_amd import initdata as idt
+
+    bounds = dict(xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    g = idt.Grid(nx, ny, nz, **bounds)
+    prims0 = idt.kh_prims(g, seed=nx * 100 + ny, shear=0.5, bz=0.3)
+    case = orc.Case(nx=nx, ny=ny, nz=nz, weno_order=order, periodic=per, sigma=100.0, regime=1, **bounds)
+    o = orc.Oracle(case)
+    cons, prims, aux = o.set(prims0)
+    want = np.zeros_like(cons)
+    o.L.oc_regime_source(ctypes.byref(o.c), orc._p(cons), orc._p(prims), orc._p(aux), orc._p(want))
+    dims = 2 if nz == 0 else 3
+    bc = tuple(eng.PERIODIC if p else eng.OUTFLOW for p in per)
+    cfg = eng.Config(nx=nx, ny=ny, nz=nz, weno_order=order, bc=bc, sigma=100.0, integrator=eng.RKSPLIT, **bounds)
+    for fused in (True, False):
+        say(f"gpu: REGIME source {nx}x{ny}x{nz} WENO{order} periodic {per}, fused={fused}")
+        with eng.Engine(cfg) as e:
+            e.set_prims(prims0)
+            e.set_model_extension(eng.EXT_REGIME)
+            e.set_regime_plan(fused)
+            got = e.source_extension()
+        rows = parity.field_errors(parity.interior(got, 4, dims), parity.interior(want, 4, dims), parity.SRMHD_CONS)
+        say(parity.report(f"REGIME source vs oracle, {nx}x{ny}x{nz}", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rf <= 1e-9, f"{n}: {rf:.3e}"
+        assert np.all(got[8] == 0.0)
