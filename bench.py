@@ -24,6 +24,12 @@ HBM_PEAK_GBS = 8000.0
 # V^n guess, writes U^{n+1}, V^{n+1}, A^{n+1}).
 BYTES_PER_ZONE_UPDATE = 784
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kh3d_256_weno5_rk2_traffic.json")
+# FP64 FLOPs and VALU instructions per zone update from the rocprof
+# instruction counters of the same workload (the file names the command).
+FLOPS_FILE = os.path.join(ROOT, "profiles", "kh3d_256_weno5_rk2_flops.json")
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
+# VALU issue: one wave64 instruction per 4 cycles per SIMD, 256 CUs x 4 SIMDs at 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4 * 64
 
 
 def parse():
@@ -348,6 +354,16 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_step")
+    compute = None
+    if os.path.exists(FLOPS_FILE) and n == 256:
+        with open(FLOPS_FILE) as f:
+            fl = json.load(f)
+        step_s = event_ms / args.steps * 1e-3
+        tflops = fl["fp64_flops_per_zone_update"] * cells / step_s / 1e12
+        compute = {"bound": "fp64 valu", "fp64_flops_per_zone_update": fl["fp64_flops_per_zone_update"],
+                   "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
+                   "valu_instructions_per_zone_update": fl["valu_instructions_per_zone_update"],
+                   "valu_issue_frac": fl["valu_instructions_per_zone_update"] * cells / step_s / VALU_ISSUE_PER_S}
 
     out = {
         "metric": METRIC,
@@ -378,6 +394,7 @@ def main():
             "launch_ms": slots,
             "launch_ms_sum": step_kernel_ms,
         },
+        "compute_roofline": compute,
         "c2p": c2p,
     }
     e.close()
