@@ -242,3 +242,78 @@ def test_host_api_regime_source_and_fields(golden, tmp_path, case):
     assert np.array_equal(parts["alpha"][0], alpha)
     assert np.array_equal(parts["K"], K)
     assert np.array_equal(parts["dwdsb"], dw.reshape((36,) + shape))
+
+
+RHS_MAIN = r"""
+// Tests/CPU/Src/test_RKPlus.cc CalculatesRHSWell{NoREGIME,REGIME}, through the host API
+#include <cmath>
+#include <cstdio>
+#include <vector>
+#include "REGIME.h"
+#include "RKPlus.h"
+#include "boundaryConds.h"
+#include "fluxVectorSplitting.h"
+#include "initFunc.h"
+#include "serialEnv.h"
+#include "simulation.h"
+#include "srmhd.h"
+#include "weno.h"
+int main(int argc, char **argv) {
+  const bool with_regime = argc > 1 && argv[1][0] == '1';
+  SerialEnv env(&argc, &argv, 1, 1, 1, 1);
+  Data d(10, 10, 10, 0, 1, 0, 1, 0, 1, 0.004, &env, 0.5, 3);
+  Weno3 weno(&d);
+  SRMHD model(&d);
+  FVS fluxMethod(&d, &weno, &model);
+  REGIME modelExtension(&d, &fluxMethod);
+  Flow bcs(&d);
+  Simulation sim(&d, &env);
+  OTVortexSingleFluid init(&d);
+  RK2B timeInt(&d, &model, &bcs, &fluxMethod, with_regime ? &modelExtension : nullptr);
+  sim.set(&init, &model, &timeInt, &bcs, &fluxMethod);
+  const size_t n = (size_t)d.Nx * d.Ny * d.Nz * d.Ncons;
+  std::vector<double> srcR(n, 0.0), srcM(n, 0.0), flux(n, 0.0), rhs(n, 0.0);
+  if (with_regime) modelExtension.sourceExtension(d.cons, d.prims, d.aux, srcR.data());
+  model.sourceTerm(d.cons, d.prims, d.aux, srcM.data());
+  fluxMethod.F(d.cons, d.prims, d.aux, d.f, flux.data());
+  timeInt.rhs(d.cons, d.prims, d.aux, rhs.data());
+  double worst = 0.0, scale = 0.0, reg = 0.0;
+  for (int var = 0; var < d.Ncons; var++)
+    for (int i = d.is; i < d.ie; i++)
+      for (int j = d.js; j < d.je; j++)
+        for (int k = d.ks; k < d.ke; k++) {
+          const size_t c = ((size_t)(var * d.Nx + i) * d.Ny + j) * d.Nz + k;
+          const double want = srcM[c] + srcR[c] - flux[c];
+          worst = std::fmax(worst, std::fabs(rhs[c] - want));
+          scale = std::fmax(scale, std::fabs(want));
+          reg = std::fmax(reg, std::fabs(srcR[c]));
+        }
+  printf("%.3e %.3e %.3e\n", worst, scale, reg);
+  return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("with_regime", [0, 1])
+def test_host_api_rkplus_rhs_reference_test(tmp_path, with_regime):
+    """The reference's RKPlus rhs tests (Tests/CPU/Src/test_RKPlus.cc:50-145)
+    on their own set-up (3-D Orszag-Tang 10^3, Ng 3, WENO3, Flow, RK2B, with
+    and without REGIME), through the host API: rhs = model source (+ REGIME
+    source) - F at the tests' 1e-15."""
+    import shutil
+
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    lib = os.path.join(ROOT, "method_amd", "lib")
+    src, exe = tmp_path / "rhs.cc", tmp_path / "rhs"
+    src.write_text(RHS_MAIN)
+    subprocess.run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "method_amd", "host", "include"), "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", lib, "-lmethod_host",
+                    "-lmethod_gpu", f"-Wl,-rpath,{lib}"], check=True, capture_output=True, text=True)
+    print("run:", exe, with_regime, flush=True)
+    r = subprocess.run([str(exe), str(with_regime)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    worst, scale, reg = (float(x) for x in r.stdout.split()[-3:])
+    print(f"rhs vs source + extension - F: {worst:.3e} (scale {scale:.3e}, REGIME source {reg:.3e})", flush=True)
+    assert scale > 0 and worst <= 1e-15
+    assert (reg > 0) == bool(with_regime)
