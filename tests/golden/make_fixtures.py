@@ -301,7 +301,47 @@ def regime_rkplus_cases():
                  interior_only=True, extra=["regime=1"])
 
 
-GROUPS = {"regime_rkplus": regime_rkplus_cases}
+def evolve_steps(g, cfl, end_time):
+    """The number of updateTime calls Simulation::evolve makes to reach
+    end_time on a 2-D grid (simulation.cc:104-133: light-speed dt, slow start)."""
+    base = cfl * min((g.xmax - g.xmin) / g.nx, (g.ymax - g.ymin) / g.ny) / np.sqrt(2.0)
+    t, it = 0.0, 0
+    while t < end_time:
+        dt = base * (0.1 if it < 15 else (0.25 if it < 30 else 1.0))
+        if t + dt > end_time:
+            dt = end_time - t
+        t += dt
+        it += 1
+    return it
+
+
+def reference_test_cases():
+    """The set-ups of the reference's RK2 and RKSplit serial-output tests
+    (Tests/CPU/Src/test_rk2.cc, test_rkSplit.cc): 40x40 on [0,1]^2, WENO3,
+    Orszag-Tang (cfl 0.6, gamma 2, one updateTime), random KH (seed 10, cfl 0.6,
+    gamma 4/3, sigma 10, cp 1, evolve to 0.004) and Brio-Wu (the Data defaults:
+    cfl 0.5, gamma 5/3, sigma 1e3, cp 0.1, evolve to 0.004), under periodic,
+    outflow and Flow boundaries."""
+    g = idt.Grid(40, 40, 0, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    per = ("periodic", "periodic", "periodic")
+    out = ("outflow", "outflow", "outflow")
+    flow = ("periodic", "outflow", "outflow")
+    kh = ["init=khrandom", "mag=1", "seed=10", "endTime=0.004"]
+    bw = ["init=briowu", "endTime=0.004"]
+    ot = ["init=ot", "endTime=0.004"]
+    nkh, nbw = evolve_steps(g, 0.6, 0.004), evolve_steps(g, 0.5, 0.004)
+    init_case("reftest_rk2_ot40_periodic", g, ot, "srmhd", "rk2", per, 1, 2.0, 0.6)
+    init_case("reftest_rk2_ot40_outflow", g, ot, "srmhd", "rk2", out, 1, 2.0, 0.6)
+    init_case("reftest_rk2_khrandom40_flow", g, kh, "srmhd", "rk2", flow, nkh, 4.0 / 3.0, 0.6, sigma=10.0, cp=1.0)
+    init_case("reftest_rk2_briowu40_periodic", g, bw, "srmhd", "rk2", per, nbw, 5.0 / 3.0, 0.5, sigma=1e3, cp=0.1)
+    # test_rkSplit.cc's Orszag-Tang cases run RK2 with the KH parameters to 0.004
+    init_case("reftest_rk2_ot40_outflow_evolve", g, ot, "srmhd", "rk2", out, nkh, 4.0 / 3.0, 0.6, sigma=10.0, cp=1.0)
+    init_case("reftest_rksplit_khrandom40_periodic", g, kh, "srmhd", "rksplit", per, nkh, 4.0 / 3.0, 0.6, sigma=10.0,
+              cp=1.0)
+    init_case("reftest_rksplit_briowu40_flow", g, bw, "srmhd", "rksplit", flow, nbw, 5.0 / 3.0, 0.5, sigma=1e3, cp=0.1)
+
+
+GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases}
 
 
 def main():
@@ -336,6 +376,7 @@ def main():
     rkplus_cases()
     regime_cases()
     regime_rkplus_cases()
+    reference_test_cases()
 
 
 if __name__ == "__main__":

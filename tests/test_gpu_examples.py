@@ -13,7 +13,15 @@ from tests import parity
 pytestmark = pytest.mark.gpu
 
 CASES = [("briowu_100_rksplit", eng.SRMHD, eng.RKSPLIT, 0.4), ("briowu_100_ssp2", eng.SRRMHD, eng.SSP2, 0.4),
-         ("ot_64_rksplit", eng.SRMHD, eng.RKSPLIT, 1.0), ("khrandom_64_rksplit", eng.SRMHD, eng.RKSPLIT, 3.0)]
+         ("ot_64_rksplit", eng.SRMHD, eng.RKSPLIT, 1.0), ("khrandom_64_rksplit", eng.SRMHD, eng.RKSPLIT, 3.0),
+         # the set-ups of the reference's RK2 / RKSplit serial-output tests (Tests/CPU/Src/test_rk2.cc,
+         # test_rkSplit.cc): 40x40, periodic / outflow / Flow, end time 0.004
+         ("reftest_rk2_ot40_periodic", eng.SRMHD, eng.RK2, 0.004), ("reftest_rk2_ot40_outflow", eng.SRMHD, eng.RK2, 0.004),
+         ("reftest_rk2_ot40_outflow_evolve", eng.SRMHD, eng.RK2, 0.004),
+         ("reftest_rk2_khrandom40_flow", eng.SRMHD, eng.RK2, 0.004),
+         ("reftest_rk2_briowu40_periodic", eng.SRMHD, eng.RK2, 0.004),
+         ("reftest_rksplit_khrandom40_periodic", eng.SRMHD, eng.RKSPLIT, 0.004),
+         ("reftest_rksplit_briowu40_flow", eng.SRMHD, eng.RKSPLIT, 0.004)]
 
 
 @pytest.mark.parametrize("name, model, integrator, end_time", CASES)

@@ -388,9 +388,18 @@ def test_rkplus_orders_differ_and_rk4_10_reset_matters(golden):
     assert max(r[4] for r in rows[0]) > 1e-7
 
 
-EXAMPLE_FIXTURES = [("briowu_100_rksplit", orc.SRMHD, "rksplit"), ("briowu_100_ssp2", orc.SRRMHD, "ssp2"),
-                    ("ot_64_rksplit", orc.SRMHD, "rksplit"), ("khrandom_64_rksplit", orc.SRMHD, "rksplit"),
-                    ("briowu_100_srrmhd_rk3", orc.SRRMHD, "rk3")]
+EXAMPLE_FIXTURES = [("briowu_100_rksplit", orc.SRMHD, "rksplit", 1e10), ("briowu_100_ssp2", orc.SRRMHD, "ssp2", 1e10),
+                    ("ot_64_rksplit", orc.SRMHD, "rksplit", 1e10), ("khrandom_64_rksplit", orc.SRMHD, "rksplit", 1e10),
+                    ("briowu_100_srrmhd_rk3", orc.SRRMHD, "rk3", 1e10)]
+# the set-ups of the reference's RK2 / RKSplit serial-output tests
+# (Tests/CPU/Src/test_rk2.cc, test_rkSplit.cc), end time 0.004
+REFTEST_FIXTURES = [("reftest_rk2_ot40_periodic", orc.SRMHD, "rk2", 0.004),
+                    ("reftest_rk2_ot40_outflow", orc.SRMHD, "rk2", 0.004),
+                    ("reftest_rk2_ot40_outflow_evolve", orc.SRMHD, "rk2", 0.004),
+                    ("reftest_rk2_khrandom40_flow", orc.SRMHD, "rk2", 0.004),
+                    ("reftest_rk2_briowu40_periodic", orc.SRMHD, "rk2", 0.004),
+                    ("reftest_rksplit_khrandom40_periodic", orc.SRMHD, "rksplit", 0.004),
+                    ("reftest_rksplit_briowu40_flow", orc.SRMHD, "rksplit", 0.004)]
 
 
 def example_case(z, model):
@@ -402,15 +411,17 @@ def example_case(z, model):
     return orc.Case(model=model, weno_order=weno, periodic=per, gamma=gam, cfl=cfl, sigma=sig, cp=cp, **grid), steps
 
 
-@pytest.mark.parametrize("name, model, integrator", EXAMPLE_FIXTURES)
-def test_reference_examples_match_reference(golden, name, model, integrator):
+@pytest.mark.parametrize("name, model, integrator, end_time", EXAMPLE_FIXTURES + REFTEST_FIXTURES)
+def test_reference_examples_match_reference(golden, name, model, integrator, end_time):
     """The reference's serial examples (1-D Brio-Wu ideal and resistive,
-    Orszag-Tang, random KH) at their own parameters, from the reference's
+    Orszag-Tang, random KH) at their own parameters, and the set-ups of its
+    RK2 / RKSplit serial-output tests (Orszag-Tang, random KH and 2-D Brio-Wu
+    under periodic, outflow and Flow boundaries), from the reference's
     post-set state, against the reference build."""
     z = golden(name)
     case, steps = example_case(z, model)
     o = orc.Oracle(case)
-    oc, op, oa, dts, st = o.run(z["prims0"], steps, integrator=integrator)
+    oc, op, oa, dts, st = o.run(z["prims0"], steps, integrator=integrator, end_time=end_time)
     assert np.array_equal(np.array(dts), z["dts"]) and st.failures == 0
     dims = 1 if case.ny == 0 else 2
     cn = parity.SRMHD_CONS if model == orc.SRMHD else parity.SRRMHD_CONS
