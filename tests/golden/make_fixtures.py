@@ -339,6 +339,11 @@ def reference_test_cases():
     init_case("reftest_rksplit_khrandom40_periodic", g, kh, "srmhd", "rksplit", per, nkh, 4.0 / 3.0, 0.6, sigma=10.0,
               cp=1.0)
     init_case("reftest_rksplit_briowu40_flow", g, bw, "srmhd", "rksplit", flow, nbw, 5.0 / 3.0, 0.5, sigma=1e3, cp=0.1)
+    # Tests/CPU/Src/test_imex.cc SSP2FlowKHSingleFluid: resistive KH 64x16 on [-0.5,0.5]x[-1,1], sigma 0,
+    # cfl 0.5, gamma 4/3, Flow, SSP2, evolve to 0.05
+    gk = idt.Grid(64, 16, 0, -0.5, 0.5, -1.0, 1.0, 0.0, 1.0)
+    init_case("reftest_ssp2_kh64x16_flow", gk, ["init=kh", "endTime=0.05"], "srrmhd", "ssp2", flow,
+              evolve_steps(gk, 0.5, 0.05), 4.0 / 3.0, 0.5, sigma=0.0, cp=0.1)
 
 
 GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases}

@@ -21,7 +21,9 @@ CASES = [("briowu_100_rksplit", eng.SRMHD, eng.RKSPLIT, 0.4), ("briowu_100_ssp2"
          ("reftest_rk2_khrandom40_flow", eng.SRMHD, eng.RK2, 0.004),
          ("reftest_rk2_briowu40_periodic", eng.SRMHD, eng.RK2, 0.004),
          ("reftest_rksplit_khrandom40_periodic", eng.SRMHD, eng.RKSPLIT, 0.004),
-         ("reftest_rksplit_briowu40_flow", eng.SRMHD, eng.RKSPLIT, 0.004)]
+         ("reftest_rksplit_briowu40_flow", eng.SRMHD, eng.RKSPLIT, 0.004),
+         # test_imex.cc SSP2FlowKHSingleFluid: resistive KH 64x16, sigma 0, Flow, evolve to 0.05 (34 steps)
+         ("reftest_ssp2_kh64x16_flow", eng.SRRMHD, eng.SSP2, 0.05)]
 
 
 @pytest.mark.parametrize("name, model, integrator, end_time", CASES)

@@ -338,3 +338,22 @@ def test_c2p_retry_matches_oracle(quirks):
     if not quirks:
         sl = (slice(None), slice(4, -4), slice(4, -4), 0)
         assert np.allclose(p[sl], prims[sl], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("model", [eng.SRMHD, eng.SRRMHD])
+def test_flux_vector_splitting_stationary(model):
+    """FluxVectorSplittingStationary (test_srmhd.cc:33-110, test_srrmhd.cc:
+    31-113): a state at rest with rho = 0.5 and every other primitive zero on a
+    periodic 10^3 grid (gamma 5/3, sigma 1000, cp 0.5) has zero net flux, to
+    the tests' 1e-15."""
+    cfg = eng.Config(nx=10, ny=10, nz=10, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                     gamma=5.0 / 3.0, sigma=1000.0, cp=0.5, model=model, weno_order=3,
+                     bc=(eng.PERIODIC,) * 3, integrator=eng.SSP2 if model == eng.SRRMHD else eng.RK2)
+    say(f"gpu: stationary state, model {model}")
+    with eng.Engine(cfg) as e:
+        prims = e.zeros(e.nprims)
+        prims[0] = 0.5
+        e.set_prims(prims)
+        fnet = e.fnet()
+    inner = fnet[:, 4:-4, 4:-4, 4:-4]
+    assert np.abs(inner).max() <= 1e-15

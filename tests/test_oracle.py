@@ -399,7 +399,8 @@ REFTEST_FIXTURES = [("reftest_rk2_ot40_periodic", orc.SRMHD, "rk2", 0.004),
                     ("reftest_rk2_khrandom40_flow", orc.SRMHD, "rk2", 0.004),
                     ("reftest_rk2_briowu40_periodic", orc.SRMHD, "rk2", 0.004),
                     ("reftest_rksplit_khrandom40_periodic", orc.SRMHD, "rksplit", 0.004),
-                    ("reftest_rksplit_briowu40_flow", orc.SRMHD, "rksplit", 0.004)]
+                    ("reftest_rksplit_briowu40_flow", orc.SRMHD, "rksplit", 0.004),
+                    ("reftest_ssp2_kh64x16_flow", orc.SRRMHD, "ssp2", 0.05)]
 
 
 def example_case(z, model):
