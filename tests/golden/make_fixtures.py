@@ -346,7 +346,57 @@ def reference_test_cases():
               evolve_steps(gk, 0.5, 0.05), 4.0 / 3.0, 0.5, sigma=0.0, cp=0.1)
 
 
-GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases}
+FULL_PLANE_VARS = ("D", "Sx", "tau", "By")  # cons fields kept on the full-size plane
+
+
+def volume_stats(a):
+    """Per-field min, max, mean and L2 norm over the interior of a [var][i][j][k] array."""
+    out = np.zeros((a.shape[0], 4))
+    for v in range(a.shape[0]):
+        x = a[v]
+        out[v] = (x.min(), x.max(), x.mean(dtype=np.float64), np.sqrt(np.sum(x * x, dtype=np.float64)))
+    return out
+
+
+def full_size_case():
+    """KH 3-D at its benchmark size (256^3, WENO5, RK2, cfl 0.4, periodic x,z /
+    outflow y, seed 1234: the state bench.py times), 2 steps of the reference
+    build. The 4.4 GB result is kept as the dts, the x-plane through the middle
+    of four cons fields, and every cons and prim field's min, max, mean and L2
+    norm over the interior."""
+    g = idt.kh3d_grid(256)
+    prims0 = idt.kh_prims(g, seed=1234)
+    bc = ("periodic", "outflow", "periodic")
+    with tempfile.TemporaryDirectory(dir=HERE) as td:
+        pin = os.path.join(td, "prims.bin")
+        prims0.tofile(pin)
+        del prims0
+        base = _grid_args(g, 4.0 / 3.0, 0.4, 5, "rk2", bc, "srmhd")
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", "steps=2", f"prims_in={pin}", f"out={out}", "dump=2"], td)
+        log = open(out + "_log.txt").read().split("\n")
+        dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+        shape = g.shape
+        n = shape[0] * shape[1] * shape[2]
+        raw = np.memmap(f"{out}_step2.bin", dtype=np.float64, mode="r")
+        nc, np_, _ = NV["srmhd"]
+        inner = (slice(g.Ng, shape[0] - g.Ng), slice(g.Ng, shape[1] - g.Ng), slice(g.Ng, shape[2] - g.Ng))
+        cons = np.array(raw[: nc * n].reshape((nc,) + shape)[(slice(None),) + inner])
+        cs = volume_stats(cons)
+        names = ["D", "Sx", "Sy", "Sz", "tau", "Bx", "By", "Bz", "phi"]
+        plane = np.stack([cons[names.index(v), 128] for v in FULL_PLANE_VARS])
+        del cons
+        prims = np.array(raw[nc * n:(nc + np_) * n].reshape((np_,) + shape)[(slice(None),) + inner])
+        ps = volume_stats(prims)
+        del prims, raw
+    path = os.path.join(HERE, "kh3d_256_weno5_rk2_full.npz")
+    np.savez_compressed(path, dts=np.array(dts), cons_stats=cs, prims_stats=ps, plane_x128=plane,
+                        plane_vars=np.array(FULL_PLANE_VARS))
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
+
+
+GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases,
+          "full_size": full_size_case}
 
 
 def main():
@@ -382,6 +432,7 @@ def main():
     regime_cases()
     regime_rkplus_cases()
     reference_test_cases()
+    full_size_case()
 
 
 if __name__ == "__main__":

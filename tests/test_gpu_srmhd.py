@@ -357,3 +357,57 @@ def test_flux_vector_splitting_stationary(model):
         fnet = e.fnet()
     inner = fnet[:, 4:-4, 4:-4, 4:-4]
     assert np.abs(inner).max() <= 1e-15
+
+
+def _volume_stats(a):
+    # the same per-field reduction as the fixture's generator (tests/golden/make_fixtures.py)
+    out = np.zeros((a.shape[0], 4))
+    for v in range(a.shape[0]):
+        x = a[v]
+        out[v] = (x.min(), x.max(), x.mean(dtype=np.float64), np.sqrt(np.sum(x * x, dtype=np.float64)))
+    return out
+
+
+def test_full_size_two_steps_match_reference(golden):
+    """The benchmark workload at its own size (KH 3-D 256^3, WENO5, RK2, the
+    seeded state bench.py times), two steps against the reference build: dt
+    bit for bit, the middle x-plane of D, Sx, tau and By to 1e-9 of each
+    field's scale, and every cons and prim field's min, max, mean and L2 norm
+    over the 16.8 M interior cells to 1e-9 of that field's scale."""
+    z = golden("kh3d_256_weno5_rk2_full")
+    g = idt.kh3d_grid(256)
+    prims0 = idt.kh_prims(g, seed=1234)
+    say("gpu: 256^3 WENO5 RK2, 2 steps against the reference build")
+    with engine(weno=5, nx=256, ny=256, nz=256, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0,
+                zmax=1.0) as e:
+        e.set_prims(prims0)
+        del prims0
+        for s in range(2):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+            assert e.time()[1] == z["dts"][s]
+        c, p, _ = e.download()
+    inner = (slice(None), slice(4, -4), slice(4, -4), slice(4, -4))
+    c, p = c[inner], p[inner]
+    names = list(parity.SRMHD_CONS)
+    plane = np.stack([c[names.index(str(v)), 128] for v in z["plane_vars"]])
+    rows = parity.field_errors(plane, z["plane_x128"], [str(v) for v in z["plane_vars"]])
+    say(parity.report("gpu vs reference, 256^3 x-plane 128 after 2 steps", rows))
+    for n, dm, fm, rf, rg, pw in rows:
+        assert rf <= 1e-9, f"{n}: {rf:.3e}"
+    for got, ref, fields in ((_volume_stats(c), z["cons_stats"], parity.SRMHD_CONS),
+                             (_volume_stats(p), z["prims_stats"], parity.SRMHD_PRIMS)):
+        # a field's scale is its largest |value|, taken over its vector group (S, v, B + phi) as the
+        # other parity checks do, so a near-zero component such as Bx is measured on the field's scale
+        scale = {n: max(abs(ref[v, 0]), abs(ref[v, 1])) for v, n in enumerate(fields)}
+        norm = {n: ref[v, 3] for v, n in enumerate(fields)}
+        for grp in parity._GROUPS:
+            members = [n for n in grp if n in scale]
+            for n in members:
+                scale[n] = max(scale[m] for m in members)
+                norm[n] = max(norm[m] for m in members)
+        for v, name in enumerate(fields):
+            diff = np.abs(got[v] - ref[v])
+            say(f"  {name}: min/max/mean/L2 differences {diff} (scale {scale[name]:.3e})")
+            assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
+            assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name
