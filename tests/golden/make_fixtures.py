@@ -395,8 +395,72 @@ def full_size_case():
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
 
 
+def full_size_resistive_case():
+    """The resistive workload at its own size (Orszag-Tang 1024^2, sigma 1e3,
+    SSP2, WENO3, cfl 0.5, periodic: the state bench.py times), 2 steps of the
+    reference build: the dts, the middle row along each axis of every cons
+    field, and every cons and prim field's min, max, mean and L2 norm."""
+    g = idt.ot2d_grid(1024)
+    prims0 = idt.ot_prims_resistive(g)
+    per = ("periodic", "periodic", "periodic")
+    with tempfile.TemporaryDirectory(dir=HERE) as td:
+        pin = os.path.join(td, "prims.bin")
+        prims0.tofile(pin)
+        base = _grid_args(g, 4.0 / 3.0, 0.5, 3, "ssp2", per, "srrmhd", sigma=1e3, cp=0.1)
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", "steps=2", f"prims_in={pin}", f"out={out}", "dump=2"], td)
+        log = open(out + "_log.txt").read().split("\n")
+        dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+        c, p, _ = _load_state(f"{out}_step2.bin", "srrmhd", g.shape)
+        inner = (slice(None), slice(g.Ng, -g.Ng), slice(g.Ng, -g.Ng), slice(None))
+        c, p = c[inner], p[inner]
+        rec = {"dts": np.array(dts), "cons_stats": volume_stats(c), "prims_stats": volume_stats(p),
+               "row_x512": c[:, 512, :, 0].copy(), "row_y512": c[:, :, 512, 0].copy()}
+    path = os.path.join(HERE, "ot2d_1024_srrmhd_ssp2_full.npz")
+    np.savez_compressed(path, **rec)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
+
+
+def large_regime_case(n=128):
+    """The REGIME workload's set-up (KH 3-D, bz 0.1, WENO5, RKSplit with
+    REGIME, sigma 100) at 128^3, one step of the reference build: the dt, the
+    middle x-plane of four cons fields and every cons and prim field's min,
+    max, mean and L2 norm. (At 256^3 the reference's REGIME arrays alone take
+    about 60 GB, more than this machine has.)"""
+    g = idt.kh3d_grid(n)
+    prims0 = idt.kh_prims(g, seed=1234, bz=0.1)
+    bc = ("periodic", "outflow", "periodic")
+    with tempfile.TemporaryDirectory(dir=HERE) as td:
+        pin = os.path.join(td, "prims.bin")
+        prims0.tofile(pin)
+        del prims0
+        base = _grid_args(g, 4.0 / 3.0, 0.4, 5, "rksplit", bc, "srmhd", sigma=100.0)
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", "steps=1", f"prims_in={pin}", f"out={out}", "dump=1", "regime=1"], td)
+        log = open(out + "_log.txt").read().split("\n")
+        dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+        shape = g.shape
+        n = shape[0] * shape[1] * shape[2]
+        raw = np.memmap(f"{out}_step1.bin", dtype=np.float64, mode="r")
+        nc, np_, _ = NV["srmhd"]
+        inner = (slice(g.Ng, shape[0] - g.Ng), slice(g.Ng, shape[1] - g.Ng), slice(g.Ng, shape[2] - g.Ng))
+        cons = np.array(raw[: nc * n].reshape((nc,) + shape)[(slice(None),) + inner])
+        cs = volume_stats(cons)
+        names = ["D", "Sx", "Sy", "Sz", "tau", "Bx", "By", "Bz", "phi"]
+        plane = np.stack([cons[names.index(v), g.nx // 2] for v in FULL_PLANE_VARS])
+        del cons
+        prims = np.array(raw[nc * n:(nc + np_) * n].reshape((np_,) + shape)[(slice(None),) + inner])
+        ps = volume_stats(prims)
+        del prims, raw
+    path = os.path.join(HERE, f"kh3d_{g.nx}_weno5_regime_rksplit_large.npz")
+    np.savez_compressed(path, dts=np.array(dts), cons_stats=cs, prims_stats=ps, plane_mid_x=plane,
+                        plane_vars=np.array(FULL_PLANE_VARS))
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
+
+
 GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases,
-          "full_size": full_size_case}
+          "full_size": full_size_case, "full_size_resistive": full_size_resistive_case,
+          "large_regime": large_regime_case}
 
 
 def main():
@@ -433,6 +497,8 @@ def main():
     regime_rkplus_cases()
     reference_test_cases()
     full_size_case()
+    full_size_resistive_case()
+    large_regime_case()
 
 
 if __name__ == "__main__":

@@ -289,3 +289,57 @@ def test_regime_source_on_ragged_grids_matches_oracle(nx, ny, nz, order, per):
         for n, dm, fm, rf, rg, pw in rows:
             assert rf <= 1e-9, f"{n}: {rf:.3e}"
         assert np.all(got[8] == 0.0)
+
+
+def _volume_stats(a):
+    # the same per-field reduction as the fixture's generator (tests/golden/make_fixtures.py)
+    out = np.zeros((a.shape[0], 4))
+    for v in range(a.shape[0]):
+        x = a[v]
+        out[v] = (x.min(), x.max(), x.mean(dtype=np.float64), np.sqrt(np.sum(x * x, dtype=np.float64)))
+    return out
+
+
+def test_regime_large_grid_step_matches_reference(golden):
+    """The REGIME workload's set-up (KH 3-D, WENO5, RKSplit with REGIME, sigma
+    100) at 128^3, one step against the reference build: dt bit for bit, the
+    middle x-plane of D, Sx, tau and By to 1e-9 of each field's scale, and
+    every cons and prim field's min, max, mean and L2 norm over the 2.1 M
+    interior cells to 1e-9 of the field's group scale."""
+    from method_amd import initdata as idt
+
+    z = golden("kh3d_128_weno5_regime_rksplit_large")
+    g = idt.kh3d_grid(128)
+    cfg = eng.Config(nx=128, ny=128, nz=128, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                     weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), cfl=0.4, sigma=100.0,
+                     integrator=eng.RKSPLIT)
+    say("gpu: REGIME 128^3 WENO5 RKSplit, 1 step against the reference build")
+    with eng.Engine(cfg) as e:
+        e.set_prims(idt.kh_prims(g, seed=1234, bz=0.1))
+        e.set_model_extension(eng.EXT_REGIME)
+        rc, st = e.update_time(stats=True)
+        assert rc == 0 and st["failures"] == 0
+        assert e.time()[1] == z["dts"][0]
+        c, p, _ = e.download()
+    inner = (slice(None), slice(4, -4), slice(4, -4), slice(4, -4))
+    c, p = c[inner], p[inner]
+    names = list(parity.SRMHD_CONS)
+    plane = np.stack([c[names.index(str(v)), 64] for v in z["plane_vars"]])
+    rows = parity.field_errors(plane, z["plane_mid_x"], [str(v) for v in z["plane_vars"]])
+    say(parity.report("gpu vs reference, REGIME 128^3 middle x-plane after 1 step", rows))
+    for n, dm, fm, rf, rg, pw in rows:
+        assert rf <= 1e-9, f"{n}: {rf:.3e}"
+    for got, ref, fields in ((_volume_stats(c), z["cons_stats"], parity.SRMHD_CONS),
+                             (_volume_stats(p), z["prims_stats"], parity.SRMHD_PRIMS)):
+        scale = {n: max(abs(ref[v, 0]), abs(ref[v, 1])) for v, n in enumerate(fields)}
+        norm = {n: ref[v, 3] for v, n in enumerate(fields)}
+        for grp in parity._GROUPS:
+            members = [n for n in grp if n in scale]
+            for n in members:
+                scale[n] = max(scale[m] for m in members)
+                norm[n] = max(norm[m] for m in members)
+        for v, name in enumerate(fields):
+            diff = np.abs(got[v] - ref[v])
+            say(f"  {name}: min/max/mean/L2 differences {diff} (scale {scale[name]:.3e})")
+            assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
+            assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name

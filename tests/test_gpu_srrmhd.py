@@ -187,3 +187,55 @@ def test_rksplit_matches_oracle():
         rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
         say(parity.report("gpu vs oracle SRRMHD RKSplit", rows))
         check_rows(rows, 1e-9)
+
+
+def _volume_stats(a):
+    # the same per-field reduction as the fixture's generator (tests/golden/make_fixtures.py)
+    out = np.zeros((a.shape[0], 4))
+    for v in range(a.shape[0]):
+        x = a[v]
+        out[v] = (x.min(), x.max(), x.mean(dtype=np.float64), np.sqrt(np.sum(x * x, dtype=np.float64)))
+    return out
+
+
+def _group_scales(ref, fields):
+    # a field's scale is its largest |value| over its vector group, as in the other parity checks
+    scale = {n: max(abs(ref[v, 0]), abs(ref[v, 1])) for v, n in enumerate(fields)}
+    norm = {n: ref[v, 3] for v, n in enumerate(fields)}
+    for grp in parity._GROUPS:
+        members = [n for n in grp if n in scale]
+        for n in members:
+            scale[n] = max(scale[m] for m in members)
+            norm[n] = max(norm[m] for m in members)
+    return scale, norm
+
+
+def test_full_size_resistive_two_steps_match_reference(golden):
+    """The resistive workload at its own size (Orszag-Tang 1024^2, sigma 1e3,
+    SSP2, WENO3, the state bench.py times), two steps against the reference
+    build: dt bit for bit, the middle row along each axis of every cons field
+    to 1e-9 of the field's group scale, and every cons and prim field's min,
+    max, mean and L2 norm over the 1 M interior cells to 1e-9."""
+    z = golden("ot2d_1024_srrmhd_ssp2_full")
+    g = idt.ot2d_grid(1024)
+    say("gpu: resistive OT 1024^2 SSP2, 2 steps against the reference build")
+    with engine(nx=1024, ny=1024, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0) as e:  # RES: sigma 1e3, cp 0.1, cfl 0.5
+        e.set_prims(idt.ot_prims_resistive(g))
+        for s in range(2):
+            assert e.update_time() == 0
+            assert e.time()[1] == z["dts"][s]
+        c, p, _ = e.download()
+    c, p = c[:, 4:-4, 4:-4, :], p[:, 4:-4, 4:-4, :]
+    for got, ref, what in ((c[:, 512, :, 0], z["row_x512"], "row x=512"), (c[:, :, 512, 0], z["row_y512"], "row y=512")):
+        rows = parity.field_errors(got, ref, parity.SRRMHD_CONS)
+        say(parity.report(f"gpu vs reference, {what}", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-9, f"{what} {n}: {rg:.3e}"
+    for got, ref, fields in ((_volume_stats(c), z["cons_stats"], parity.SRRMHD_CONS),
+                             (_volume_stats(p), z["prims_stats"], parity.SRRMHD_PRIMS)):
+        scale, norm = _group_scales(ref, fields)
+        for v, name in enumerate(fields):
+            diff = np.abs(got[v] - ref[v])
+            say(f"  {name}: min/max/mean/L2 differences {diff} (scale {scale[name]:.3e})")
+            assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
+            assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name
