@@ -458,7 +458,38 @@ def large_regime_case(n=128):
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
 
 
-GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases,
+# (nx, ny, shear, bz, seed) of the batch member checked against the reference at its real size
+BATCH_MEMBER = (768, 384, 0.6, 0.2, 77)
+
+
+def batch_member_case():
+    """One member of the batched-ensemble workload at a real ensemble size
+    (KH 2-D 768 x 384, WENO5, RK2, cfl 0.4, periodic x / outflow y), 3 steps
+    of the reference build: the dts, the middle row along each axis of every
+    cons field, and every cons and prim field's min, max, mean and L2 norm."""
+    nx, ny, shear, bz, seed = BATCH_MEMBER
+    g = idt.Grid(nx, ny, 0, -0.5, 0.5, -1.0, 1.0)
+    prims0 = idt.kh_prims(g, seed=seed, shear=shear, bz=bz)
+    with tempfile.TemporaryDirectory(dir=HERE) as td:
+        pin = os.path.join(td, "prims.bin")
+        prims0.tofile(pin)
+        base = _grid_args(g, 4.0 / 3.0, 0.4, 5, "rk2", ("periodic", "outflow", "periodic"), "srmhd")
+        out = os.path.join(td, "out")
+        _run(base + ["mode=run", "steps=3", f"prims_in={pin}", f"out={out}", "dump=3"], td)
+        log = open(out + "_log.txt").read().split("\n")
+        dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+        c, p, _ = _load_state(f"{out}_step3.bin", "srmhd", g.shape)
+        inner = (slice(None), slice(g.Ng, -g.Ng), slice(g.Ng, -g.Ng), slice(None))
+        c, p = c[inner], p[inner]
+        rec = {"member": np.array(BATCH_MEMBER, dtype=np.float64), "dts": np.array(dts),
+               "cons_stats": volume_stats(c), "prims_stats": volume_stats(p),
+               "row_x": c[:, nx // 2, :, 0].copy(), "row_y": c[:, :, ny // 2, 0].copy()}
+    path = os.path.join(HERE, f"batch_member_{nx}x{ny}_weno5_rk2.npz")
+    np.savez_compressed(path, **rec)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
+
+
+GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases, "batch_member": batch_member_case,
           "full_size": full_size_case, "full_size_resistive": full_size_resistive_case,
           "large_regime": large_regime_case}
 
@@ -499,6 +530,7 @@ def main():
     full_size_case()
     full_size_resistive_case()
     large_regime_case()
+    batch_member_case()
 
 
 if __name__ == "__main__":

@@ -167,3 +167,41 @@ def test_full_ensemble_two_steps():
             want, wt = single_run(members[k], 2, 5)
             assert_same(b.download(k), want, f"member {k} {members[k][:2]}")
             assert b.time(k) == wt
+
+
+def test_member_at_real_size_matches_reference(golden):
+    """A member at a real ensemble size (KH 2-D 768 x 384, WENO5, RK2), run
+    inside a batch with two other members of other sizes, 3 steps against the
+    reference build: dt bit for bit, the middle row along each axis of every
+    cons field to 1e-9 of its group scale, and every cons and prim field's
+    min, max, mean and L2 norm to 1e-9."""
+    z = golden("batch_member_768x384_weno5_rk2")
+    m = tuple(int(v) if k in (0, 1, 4) else float(v) for k, v in enumerate(z["member"]))
+    members = [m, SMALL[1], SMALL[2]]
+    say(f"gpu: batch member {m[:2]} vs reference, 3 steps")
+    with make_batch(members) as b:
+        for s in range(3):
+            assert b.step(1) == 0
+            assert b.time(0)[1] == z["dts"][s]
+        c, p, _ = b.download(0)
+    nx, ny = m[:2]
+    c, p = parity.interior(c, 4, 2), parity.interior(p, 4, 2)
+    for got, ref, what in ((c[:, nx // 2, :, 0], z["row_x"], "row x"), (c[:, :, ny // 2, 0], z["row_y"], "row y")):
+        rows = parity.field_errors(got, ref, parity.SRMHD_CONS)
+        say(parity.report(f"batch member vs reference, {what}", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-9, f"{what} {n}: {rg:.3e}"
+    for got_a, ref, fields in ((c, z["cons_stats"], parity.SRMHD_CONS), (p, z["prims_stats"], parity.SRMHD_PRIMS)):
+        scale = {n: max(abs(ref[v, 0]), abs(ref[v, 1])) for v, n in enumerate(fields)}
+        norm = {n: ref[v, 3] for v, n in enumerate(fields)}
+        for grp in parity._GROUPS:
+            mem = [n for n in grp if n in scale]
+            for n in mem:
+                scale[n] = max(scale[k] for k in mem)
+                norm[n] = max(norm[k] for k in mem)
+        for v, name in enumerate(fields):
+            x = got_a[v]
+            got = np.array([x.min(), x.max(), x.mean(dtype=np.float64), np.sqrt(np.sum(x * x, dtype=np.float64))])
+            diff = np.abs(got - ref[v])
+            assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
+            assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name
