@@ -2,7 +2,10 @@
 
 One step = one RK2 step (two stages, each with the flux sweeps, the update and
 the per-cell Newton cons2prim) of the whole grid, state resident in HBM.
-Prints one JSON line. See DESIGN.md for the roofline accounting.
+Prints one JSON line. The HBM roofline counts 784 algorithmic bytes per zone
+update (BYTES_PER_ZONE_UPDATE below); the FP64 compute roofline takes its
+FLOPs and VALU instructions per zone update from the committed rocprof counter
+summary of the same workload.
 """
 from __future__ import annotations
 
