@@ -86,6 +86,11 @@ struct mgpu_engine {
   MlpDev *mlp = nullptr;              // learned guess network (device)
   unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
   int fused_yz = 0;                      // 3-D: fused z+y sweep launch (off: slower here)
+  // SRMHD RK2 and RKSplit steps keep the cons2prim root (v^2, Z) of each cell
+  // instead of writing its 13 aux values: no kernel of those steps reads aux,
+  // and what does (state download, wavespeed) forms it from the root first.
+  double *root = nullptr;
+  bool aux_stale = false;
 };
 
 namespace {
@@ -198,10 +203,10 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
   return 0;
 }
 
-int retry(mgpu_engine *e, const double *U, double *V, double *A, C2PCounters *st) {
+int retry(mgpu_engine *e, const double *U, double *V, double *A, C2PCounters *st, double *root = nullptr) {
   if (e->cfg.model != MGPU_SRMHD) return 0;  // the resistive recovery has no retry (srrmhd.cc:574-579)
   const long long last = e->cfg.quirks ? e->geo.vstride - 1 : -1;
-  launch_c2p_retry_srmhd(e->geo, U, V, A, e->cfg.gamma, e->fail_count, e->fail_list, e->fail_flag, last, st,
+  launch_c2p_retry_srmhd(e->geo, U, V, A, root, e->cfg.gamma, e->fail_count, e->fail_list, e->fail_flag, last, st,
                          e->stream);
   HIP_OK(hipGetLastError());
   return mark(e, "c2p_retry");
@@ -224,9 +229,21 @@ int rk2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   b.Vg = e->V;
   b.Uout = e->U;
   b.Vout = e->V;
-  b.Aout = e->A;
+  const bool lazy_aux = e->root != nullptr;
+  b.Aout = lazy_aux ? nullptr : e->A;
+  b.root = e->root;
   if (run_sweeps(e, b, kSweepStage2)) return -1;
-  if (retry(e, e->U, e->V, e->A, st)) return -1;
+  if (retry(e, e->U, e->V, lazy_aux ? nullptr : e->A, st, e->root)) return -1;
+  e->aux_stale = lazy_aux;
+  return 0;
+}
+
+// The aux of the resident state, formed now if the last step left it.
+int fresh_aux(mgpu_engine *e) {
+  if (!e->aux_stale) return 0;
+  launch_aux_srmhd(e->geo, e->U, e->root, e->A, e->cfg.gamma, e->stream);
+  HIP_OK(hipGetLastError());
+  e->aux_stale = false;
   return 0;
 }
 
@@ -289,14 +306,14 @@ int rksplit_step(mgpu_engine *e, double dt, C2PCounters *st) {
   if (e->cfg.model == MGPU_SRMHD) {
     const double *ext = extension_source(e, e->V);
     HIP_OK(hipGetLastError());
-    launch_split_srmhd(e->geo, e->U, e->V, e->A, dt, e->cfg.gamma, e->cfg.cp, st, e->fail_count, e->fail_list,
-                       e->fail_flag, ext, e->stream);
+    launch_split_srmhd(e->geo, e->U, e->V, e->root ? nullptr : e->A, e->root, dt, e->cfg.gamma, e->cfg.cp, st,
+                       e->fail_count, e->fail_list, e->fail_flag, ext, e->stream);
   }
   else
     launch_split_srrmhd(e->geo, e->U, e->V, e->A, dt, e->cfg.gamma, e->cfg.sigma, e->cfg.cp, st, e->stream);
   HIP_OK(hipGetLastError());
   if (mark(e, "split_source_c2p")) return -1;
-  return retry(e, e->U, e->V, e->A, st);
+  return retry(e, e->U, e->V, e->root ? nullptr : e->A, st, e->root);
 }
 
 // ---------------------------------------------------------------- RKPlus
@@ -607,6 +624,10 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
       return cleanup(fail("hipMalloc of %zu bytes failed", b.elems * sizeof(double)));
     if (hipMemset(*b.ptr, 0, b.elems * sizeof(double)) != hipSuccess) return cleanup(fail("hipMemset failed"));
   }
+  if (c.model == MGPU_SRMHD && (c.integrator == MGPU_RK2 || c.integrator == MGPU_RKSPLIT)) {
+    if (hipMalloc(&e->root, 2 * n * sizeof(double)) != hipSuccess)
+      return cleanup(fail("hipMalloc of the cons2prim root store failed"));
+  }
   if (c.integrator >= MGPU_RK2B) {
     // stage states of RK2B, RK3, RK4, RK4_10 (RKPlus.cc:56-586)
     static const int kStages[4] = {1, 2, 4, 2};
@@ -641,7 +662,8 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
 int mgpu_destroy(mgpu_engine *e) {
   if (!e) return 0;
   if (e->stream) hipStreamSynchronize(e->stream);
-  double *bufs[] = {e->U, e->V, e->A, e->U1, e->V1, e->fnet, e->staging, e->U2, e->V2, e->F1, e->S1, e->S2};
+  double *bufs[] = {e->U,  e->V,  e->A,  e->U1, e->V1, e->fnet,     e->staging,
+                    e->U2, e->V2, e->F1, e->S1, e->S2, e->root};
   for (double *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < 4; ++k) {
@@ -685,12 +707,14 @@ int mgpu_upload_state(mgpu_engine *e, const double *cons, const double *prims, c
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(e->stream));
   }
+  if (aux) e->aux_stale = false;
   return 0;
 }
 
 int mgpu_download_state(mgpu_engine *e, double *cons, double *prims, double *aux) {
   if (!e) return fail("null engine");
   if (ensure_staging(e)) return -1;
+  if (aux && fresh_aux(e)) return -1;
   double *dst[3] = {cons, prims, aux};
   const double *src[3] = {e->U, e->V, e->A};
   const int nv[3] = {e->ncons, e->nprims, e->naux};
@@ -712,6 +736,7 @@ int mgpu_prims_to_all(mgpu_engine *e) {
     launch_prims_to_all_srrmhd(e->geo, e->V, e->U, e->A, e->cfg.gamma, e->cfg.sigma, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(e->stream));
+  e->aux_stale = false;
   return 0;
 }
 
@@ -855,6 +880,7 @@ int launch_c2p(mgpu_engine *e) {
   else
     launch_c2p_srrmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->cfg.sigma, e->dstats, e->stream);
   HIP_OK(hipGetLastError());
+  e->aux_stale = false;  // every cell's aux written
   return 0;
 }
 }  // namespace
@@ -925,6 +951,7 @@ int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
     return 0;
   }
   if (!e->dspeed) HIP_OK(hipMalloc(&e->dspeed, 3 * sizeof(unsigned long long)));
+  if (fresh_aux(e)) return -1;
   HIP_OK(hipMemsetAsync(e->dspeed, 0, 3 * sizeof(unsigned long long), e->stream));
   launch_max_wavespeed_srmhd(e->geo, e->V, e->A, e->dspeed, e->stream);
   HIP_OK(hipGetLastError());

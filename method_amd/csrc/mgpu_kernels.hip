@@ -212,6 +212,9 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
   if (a.Aout) {
 #pragma unroll
     for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+  } else if (a.root) {
+    a.root[q] = v2;
+    a.root[vs + q] = Z;
   }
   cs.add(its);
   if (its < 0) {
@@ -658,6 +661,22 @@ __global__ __launch_bounds__(kThreads) void k_prims_to_all_srmhd(Geo g, const do
   for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
 }
 
+// SRMHD aux of resident cons from the cons2prim root (v^2, Z) a stage kept:
+// the recovery of srmhd.cc:509-557 on the same inputs the stage had, so the
+// aux equals what the stage would have written.
+__global__ __launch_bounds__(kThreads) void k_aux_srmhd(Geo g, const double *U, const double *root, double *A,
+                                                         double gam) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  const long long vs = g.vstride;
+  double u[9], P[8], aux[13];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
+  srmhd_recover(u, gam, root[q], root[vs + q], P, aux);
+#pragma unroll
+  for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+}
+
 // SRMHD::getPrimitiveVars on resident cons with resident prims as the guess
 // (srmhd.cc:390-562).
 // cold: start from v^2 = 0, Z = tau + D instead of the previous primitives.
@@ -706,7 +725,7 @@ __global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, 
 // neighbours (srmhd.cc:465-507). With `last_cell` >= 0 the system solved is
 // that cell's, as the reference's retry reuses the last cell's arguments.
 __global__ __launch_bounds__(kThreads) void k_c2p_retry_srmhd(Geo g, const double *U, double *V, double *A,
-                                                               double gam, const int *fail_count,
+                                                               double *root, double gam, const int *fail_count,
                                                                const long long *fail_list, unsigned char *fail_flag,
                                                                long long last_cell, C2PCounters *st) {
   const int nfail = *fail_count;
@@ -748,6 +767,10 @@ __global__ __launch_bounds__(kThreads) void k_c2p_retry_srmhd(Geo g, const doubl
     for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
     if (A)
       for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+    else if (root) {
+      root[q] = v2;
+      root[vs + q] = Z;
+    }
     if (st && its < 0) atomicAdd(&st->retry_failures, 1ull);
   }
 }
@@ -865,10 +888,10 @@ __global__ __launch_bounds__(kThreads) void k_source_srmhd(Geo g, PadGeo pg, con
 // plus the model extension's source on variables 0..7 when ext is given, then
 // cons2prim from the current primitives (srmhd.cc:390-562). Without an
 // extension only phi changes, so only phi is written back.
-__global__ __launch_bounds__(kThreads) void k_split_srmhd(Geo g, double *U, double *V, double *A, double dt,
-                                                           double gam, double cp, C2PCounters *st, int *fail_count,
-                                                           long long *fail_list, unsigned char *fail_flag,
-                                                           const double *ext) {
+__global__ __launch_bounds__(kThreads) void k_split_srmhd(Geo g, double *U, double *V, double *A, double *root,
+                                                           double dt, double gam, double cp, C2PCounters *st,
+                                                           int *fail_count, long long *fail_list,
+                                                           unsigned char *fail_flag, const double *ext) {
   const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
   int cells = 0, iters = 0, fails = 0, maxit = 0;
   if (q < g.vstride) {
@@ -898,8 +921,13 @@ __global__ __launch_bounds__(kThreads) void k_split_srmhd(Geo g, double *U, doub
     srmhd_recover(u, gam, v2, Z, P, aux);
 #pragma unroll
     for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
+    if (A) {
 #pragma unroll
-    for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+      for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+    } else {
+      root[q] = v2;
+      root[vs + q] = Z;
+    }
     cells = 1;
     if (its < 0) {
       fails = 1;
@@ -1286,17 +1314,21 @@ void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double 
   hipLaunchKernelGGL(k_prims_to_all_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, V, U, A, gam);
 }
 
+void launch_aux_srmhd(const Geo &g, const double *U, const double *root, double *A, double gam, hipStream_t s) {
+  hipLaunchKernelGGL(k_aux_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, root, A, gam);
+}
+
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
                       int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold, hipStream_t s) {
   hipLaunchKernelGGL(k_c2p_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, gam, st, fail_count,
                      fail_list, fail_flag, cold);
 }
 
-void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, const int *fail_count,
-                            const long long *fail_list, unsigned char *fail_flag, long long last_cell,
-                            C2PCounters *st, hipStream_t s) {
-  hipLaunchKernelGGL(k_c2p_retry_srmhd, dim3(64), dim3(kThreads), 0, s, g, U, V, A, gam, fail_count, fail_list,
-                     fail_flag, last_cell, st);
+void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double *root, double gam,
+                            const int *fail_count, const long long *fail_list, unsigned char *fail_flag,
+                            long long last_cell, C2PCounters *st, hipStream_t s) {
+  hipLaunchKernelGGL(k_c2p_retry_srmhd, dim3(64), dim3(kThreads), 0, s, g, U, V, A, root, gam, fail_count,
+                     fail_list, fail_flag, last_cell, st);
   hipLaunchKernelGGL(k_clear_fail_flags, dim3(64), dim3(kThreads), 0, s, fail_count, fail_list, fail_flag);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, const_cast<int *>(fail_count));
 }
@@ -1307,11 +1339,11 @@ void launch_max_wavespeed_srmhd(const Geo &g, const double *V, const double *A, 
   hipLaunchKernelGGL(k_max_wavespeed_srmhd, dim3((unsigned)blocks), dim3(kThreads), 0, s, g, V, A, out);
 }
 
-void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double cp,
-                        C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
+void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double *root, double dt, double gam,
+                        double cp, C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
                         const double *ext, hipStream_t s) {
-  hipLaunchKernelGGL(k_split_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, dt, gam, cp, st,
-                     fail_count, fail_list, fail_flag, ext);
+  hipLaunchKernelGGL(k_split_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, root, dt, gam,
+                     cp, st, fail_count, fail_list, fail_flag, ext);
 }
 
 void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double sigma,

@@ -41,6 +41,7 @@ struct SweepArgs {
   const double *Un;  // stage 2: cons at t^n
   const double *Vg;  // prims giving the cons2prim starting guess
   double *Uout, *Vout, *Aout;
+  double *root;      // SRMHD: cons2prim root (v^2, Z) [2][cell], kept when Aout is null
   double dt, gamma, sigma, cp;
   double *F1;        // SSP2: flux1 store
   const double *S1;  // SSP2: source1 (components 8..12)
@@ -74,15 +75,19 @@ void launch_source_srrmhd(const Geo &g, const PadGeo &pg, const double *U, const
 void launch_flux_vector_srrmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst,
                                int dir, int quirk, double gam, double sigma, hipStream_t s);
 void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double *A, double gam, hipStream_t s);
+// SRMHD aux from the resident cons and the cons2prim root kept by a stage,
+// through the same recovery formulas the stage would have applied.
+void launch_aux_srmhd(const Geo &g, const double *U, const double *root, double *A, double gam, hipStream_t s);
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
                       int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold, hipStream_t s);
-void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, const int *fail_count,
-                            const long long *fail_list, unsigned char *fail_flag, long long last_cell,
-                            C2PCounters *st, hipStream_t s);
+void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double *root, double gam,
+                            const int *fail_count, const long long *fail_list, unsigned char *fail_flag,
+                            long long last_cell, C2PCounters *st, hipStream_t s);
 void launch_zero_int(int *p, hipStream_t s);
 // RKSplit's source half-step and cons2prim over interior cells.
-void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double cp,
-                        C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
+// A null: the cons2prim root (v^2, Z) goes to `root` in place of the aux.
+void launch_split_srmhd(const Geo &g, double *U, double *V, double *A, double *root, double dt, double gam,
+                        double cp, C2PCounters *st, int *fail_count, long long *fail_list, unsigned char *fail_flag,
                         const double *ext, hipStream_t s);
 void launch_split_srrmhd(const Geo &g, double *U, double *V, double *A, double dt, double gam, double sigma,
                          double cp, C2PCounters *st, hipStream_t s);
@@ -110,6 +115,7 @@ struct RkStageArgs {
   const double *ext;  // SRMHD model-extension source of the rhs state (variables 0..7), or null
   const double *Vg;
   double *Uout, *Vout, *Aout;
+  double *root;      // SRMHD: cons2prim root (v^2, Z) [2][cell], kept when Aout is null
   double dt, gamma, sigma, cp;
   C2PCounters *stats;
   int *fail_count;

@@ -411,3 +411,35 @@ def test_full_size_two_steps_match_reference(golden):
             say(f"  {name}: min/max/mean/L2 differences {diff} (scale {scale[name]:.3e})")
             assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
             assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name
+
+
+@pytest.mark.parametrize("integrator", [eng.RK2, eng.RKSPLIT], ids=["rk2", "rksplit"])
+def test_aux_formed_from_kept_root_is_consistent(golden, integrator):
+    """The SRMHD RK2 and RKSplit steps keep each cell's cons2prim root (v^2, Z)
+    in place of its 13 aux values, and the aux is formed from it when read. It
+    must agree with primsToAll of the downloaded prims (srmhd.cc:573-673) to the
+    Newton tolerance, carry S.B and S^2 of the downloaded cons, and be the same
+    bits on every later read (a wavespeed reduction in between included)."""
+    z = golden("kh3d_16_weno5_rk2")
+    say(f"gpu: aux formed after {'RK2' if integrator == eng.RK2 else 'RKSplit'} steps, KH 3-D 16^3")
+    cfg = eng.Config(weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), quirks=1, integrator=integrator,
+                     **KH3D16)
+    with eng.Engine(cfg) as e:
+        e.set_prims(z["prims0"])
+        for _ in range(3):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+        c1, p1, a1 = e.download()
+        speeds = e.max_wavespeed()
+        c2, p2, a2 = e.download()
+    assert np.all(speeds[:3] > 0.0) and np.all(speeds < 1.0)
+    for x, y in ((c1, c2), (p1, p2), (a1, a2)):
+        assert np.array_equal(x, y)
+    _, _, oa = oracle_for(weno=5, **KH3D16).prims_to_all(p1)
+    a, c, p = (parity.interior(x, 4, 3) for x in (a1, c1, p1))
+    rows = parity.field_errors(a, parity.interior(oa, 4, 3), parity.SRMHD_AUX)
+    say(parity.report("aux formed from the root vs primsToAll of the prims", rows))
+    check_rows(rows, 1e-10, use_group=True)
+    S, B = c[1:4], p[5:8]
+    assert np.max(np.abs(a[10] - (B * S).sum(axis=0))) <= 1e-14 * np.abs(a[10]).max()
+    assert np.max(np.abs(a[12] - (S * S).sum(axis=0))) <= 1e-14 * np.abs(a[12]).max()
