@@ -27,8 +27,6 @@ HBM_PEAK_GBS = 8000.0
 # V^n guess, writes U^{n+1}, V^{n+1} and the cons2prim root (v^2, Z) from which
 # the aux is formed when it is read).
 BYTES_PER_ZONE_UPDATE = 696
-# The batched 2-D step writes the 13 aux values instead of the root: 98 values.
-BATCH_BYTES_PER_ZONE_UPDATE = 784
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kh3d_256_weno5_rk2_traffic.json")
 # FP64 FLOPs and VALU instructions per zone update from the rocprof
 # instruction counters of the same workload (the file names the command).
@@ -289,7 +287,7 @@ def batch_leg(steps: int, members: int):
                         "per-member dt, FP64",
             "cells": cells, "steps": steps,
             "zone_updates_per_s": batched, "ms_per_step": 1e3 * wall / steps, "event_ms_per_step": ev_ms / steps,
-            "hbm_frac_784B": batched * BATCH_BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+            "hbm_frac_696B": batched * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
             "one_at_a_time_zone_updates_per_s": one_at_a_time, "one_at_a_time_ms_per_step": single_ms / steps,
             "speedup_vs_one_at_a_time": batched / one_at_a_time,
             "tuning": best, "candidates": len(cands),

@@ -64,7 +64,8 @@ struct BatchArgs {
   const double *U, *V;   // stage input
   const double *Un;      // cons at t^n (stage two)
   const double *Vg;      // prims at t^n, the cons2prim starting guess
-  double *Uout, *Vout, *Aout;
+  double *Uout, *Vout;
+  double *root;          // stage two: cons2prim root (v^2, Z) [2][cell], from which the aux is formed
   double gamma;
   C2PCounters *stats;
   int *fail_count;
@@ -235,15 +236,15 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, int 
   const double Bsq = un[5] * un[5] + un[6] * un[6] + un[7] * un[7];
   const double Ssq = un[1] * un[1] + un[2] * un[2] + un[3] * un[3];
   const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
-  double P[kSrmhdPrims], A[kSrmhdAux];
+  double P[kSrmhdPrims], A[kSrmhdAux];  // A unused: the aux is formed from the root when read
   srmhd_recover(un, a.gamma, v2, Z, P, A);
 #pragma unroll
   for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
 #pragma unroll
   for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
   if (STAGE == 2) {
-#pragma unroll
-    for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+    a.root[q] = v2;
+    a.root[vs + q] = Z;
   }
   acc.cells += 1;
   if (its < 0) {
@@ -341,8 +342,10 @@ __device__ void batch_retry(const BatchArgs &a, bool stage2) {
     double P[8], aux[13];
     srmhd_recover(u, gam, v2, Z, P, aux);
     for (int v = 0; v < 8; ++v) a.Vout[v * vs + q] = P[v];
-    if (stage2)
-      for (int v = 0; v < 13; ++v) a.Aout[v * vs + q] = aux[v];
+    if (stage2) {
+      a.root[q] = v2;
+      a.root[vs + q] = Z;
+    }
     if (its < 0) atomicAdd(&a.stats->retry_failures, 1ull);
   }
   __syncthreads();
@@ -471,6 +474,10 @@ struct mgpu_batch {
   int ncu = 0;
   hipStream_t stream = nullptr;
   double *U = nullptr, *V = nullptr, *A = nullptr, *U1 = nullptr, *V1 = nullptr;
+  // Steps keep each cell's cons2prim root instead of writing its 13 aux
+  // values; the aux is formed from it before anything reads A.
+  double *root = nullptr;
+  bool aux_stale = false;
   double *stage_pad = nullptr, *stage_cmp = nullptr;  // per-instance host transfer staging
   double *backup = nullptr;                           // autotuner's copy of U, V, A
   InstDev *dinst = nullptr;
@@ -573,7 +580,7 @@ BatchArgs stage_args(mgpu_batch *b, int stage) {
     a.Un = b->U;
     a.Uout = b->U;
     a.Vout = b->V;
-    a.Aout = b->A;
+    a.root = b->root;
   }
   return a;
 }
@@ -621,6 +628,19 @@ int run_steps(mgpu_batch *b, int nsteps) {
       return -1;
     }
   }
+  if (nsteps > 0) b->aux_stale = true;
+  return 0;
+}
+
+// The aux of every instance, formed from the kept roots if steps left it.
+int fresh_aux(mgpu_batch *b) {
+  if (!b->aux_stale) return 0;
+  Geo g;
+  memset(&g, 0, sizeof g);
+  g.vstride = b->vs;
+  launch_aux_srmhd(g, b->U, b->root, b->A, b->cfg.gamma, b->stream);
+  BATCH_OK(hipGetLastError());
+  b->aux_stale = false;
   return 0;
 }
 
@@ -752,6 +772,7 @@ int mgpu_batch_create(const mgpu_batch_config *cfg, const mgpu_instance *inst, i
   } bufs[] = {{&b->U, 9 * n},
               {&b->V, 8 * n},
               {&b->A, 13 * n},
+              {&b->root, 2 * n},
               {&b->U1, 9 * n},
               {&b->V1, 8 * n},
               {&b->stage_pad, 30 * (size_t)b->max_pad},
@@ -788,7 +809,7 @@ int mgpu_batch_destroy(mgpu_batch *b) {
   if (!b) return 0;
   if (b->stream) hipStreamSynchronize(b->stream);
   drop_graph(b);
-  double *bufs[] = {b->U, b->V, b->A, b->U1, b->V1, b->stage_pad, b->stage_cmp, b->backup};
+  double *bufs[] = {b->U, b->V, b->A, b->root, b->U1, b->V1, b->stage_pad, b->stage_cmp, b->backup};
   for (double *p : bufs)
     if (p) hipFree(p);
   void *other[] = {b->dinst, b->dclock, b->dclock_backup, b->dtile_inst, b->dstats, b->fail_count, b->fail_list,
@@ -803,6 +824,7 @@ int mgpu_batch_destroy(mgpu_batch *b) {
 int mgpu_batch_set_prims(mgpu_batch *b, int i, const double *prims) {
   if (check_index(b, i)) return -1;
   if (!prims) return engine_fail("mgpu_batch_set_prims: null prims");
+  if (fresh_aux(b)) return -1;  // the other instances' aux first
   const Geo g = instance_geo(b, i);
   const PadGeo p = instance_pad(b, i);
   const long long n = g.vstride;
@@ -828,6 +850,7 @@ int mgpu_batch_set_prims(mgpu_batch *b, int i, const double *prims) {
 int mgpu_batch_upload_state(mgpu_batch *b, int i, const double *cons, const double *prims, const double *aux) {
   if (check_index(b, i)) return -1;
   if (!cons || !prims || !aux) return engine_fail("mgpu_batch_upload_state: null array");
+  if (fresh_aux(b)) return -1;  // the other instances' aux first
   const Geo g = instance_geo(b, i);
   const PadGeo p = instance_pad(b, i);
   const double *src[3] = {cons, prims, aux};
@@ -848,6 +871,7 @@ int mgpu_batch_download_state(mgpu_batch *b, int i, double *cons, double *prims,
   if (check_index(b, i)) return -1;
   const Geo g = instance_geo(b, i);
   const PadGeo p = instance_pad(b, i);
+  if (aux && fresh_aux(b)) return -1;
   double *dst[3] = {cons, prims, aux};
   double *src[3] = {b->U, b->V, b->A};
   const int nv[3] = {9, 8, 13};
@@ -936,6 +960,7 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best,
   const size_t n = (size_t)b->vs;
   if (!b->backup) BATCH_OK(hipMalloc(&b->backup, 30 * n * sizeof(double)));
   if (!b->dclock_backup) BATCH_OK(hipMalloc(&b->dclock_backup, b->ninst * sizeof(Clock)));
+  if (fresh_aux(b)) return -1;
   BATCH_OK(hipMemcpyAsync(b->backup, b->U, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
   BATCH_OK(hipMemcpyAsync(b->backup + 9 * n, b->V, 8 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
   BATCH_OK(hipMemcpyAsync(b->backup + 17 * n, b->A, 13 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
@@ -973,6 +998,7 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best,
   BATCH_OK(hipMemcpyAsync(b->A, b->backup + 17 * n, 13 * n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
   BATCH_OK(hipMemcpyAsync(b->dclock, b->dclock_backup, b->ninst * sizeof(Clock), hipMemcpyDeviceToDevice, b->stream));
   BATCH_OK(hipStreamSynchronize(b->stream));
+  b->aux_stale = false;  // the restored aux belongs to the restored state
   if (nall) *nall = count;
   if (best) *best = b->tune;
   return rc;
