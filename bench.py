@@ -2,8 +2,9 @@
 
 One step = one RK2 step (two stages, each with the flux sweeps, the update and
 the per-cell Newton cons2prim) of the whole grid, state resident in HBM.
-Prints one JSON line. The HBM roofline counts 696 algorithmic bytes per zone
-update (BYTES_PER_ZONE_UPDATE below); the FP64 compute roofline takes its
+Prints one JSON line. The HBM roofline counts 784 algorithmic bytes per zone
+update (BYTES_PER_ZONE_UPDATE below, the unit's definition; this engine's step
+needs 672, ENGINE_BYTES_PER_ZONE_UPDATE); the FP64 compute roofline takes its
 FLOPs and VALU instructions per zone update from the committed rocprof counter
 summary of the same workload.
 """
@@ -22,11 +23,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "zone-updates/s FP64 SRMHD 256^3; cons2prim cells/s learned vs plain guess"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes of one SRMHD RK2 zone update: 87 FP64 values
+# Algorithmic HBM bytes of one SRMHD RK2 zone update: 98 FP64 values
 # (stage 1 reads U^n, V^n and writes U1, V1; stage 2 reads U^n, U1, V1 and the
-# V^n guess, writes U^{n+1}, V^{n+1} and the cons2prim root (v^2, Z) from which
-# the aux is formed when it is read).
-BYTES_PER_ZONE_UPDATE = 696
+# V^n guess, writes U^{n+1}, V^{n+1}, A^{n+1}).
+BYTES_PER_ZONE_UPDATE = 784
+# The same count for this engine's step, which writes only rho .. p of V1 (the
+# next stage reads no more) and the cons2prim root (v^2, Z) in place of
+# A^{n+1} (the aux is formed from it when read): 84 values.
+ENGINE_BYTES_PER_ZONE_UPDATE = 672
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kh3d_256_weno5_rk2_traffic.json")
 # FP64 FLOPs and VALU instructions per zone update from the rocprof
 # instruction counters of the same workload (the file names the command).
@@ -287,7 +291,7 @@ def batch_leg(steps: int, members: int):
                         "per-member dt, FP64",
             "cells": cells, "steps": steps,
             "zone_updates_per_s": batched, "ms_per_step": 1e3 * wall / steps, "event_ms_per_step": ev_ms / steps,
-            "hbm_frac_696B": batched * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+            "hbm_frac_784B": batched * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
             "one_at_a_time_zone_updates_per_s": one_at_a_time, "one_at_a_time_ms_per_step": single_ms / steps,
             "speedup_vs_one_at_a_time": batched / one_at_a_time,
             "tuning": best, "candidates": len(cands),
@@ -386,11 +390,13 @@ def main():
                                "periodic x,z / outflow y, FP64", "cells": cells, "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
-            "kernel": "one RK2 step (the fused sweep launches of both stages), 696 B per zone update",
+            "kernel": "one RK2 step (the fused sweep launches of both stages), 784 B per zone update",
             "achieved": achieved_step,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_step / HBM_PEAK_GBS,
+            "engine_algorithmic_B_per_zone_update": ENGINE_BYTES_PER_ZONE_UPDATE,
+            "engine_frac": achieved_step * ENGINE_BYTES_PER_ZONE_UPDATE / BYTES_PER_ZONE_UPDATE / HBM_PEAK_GBS,
             "traffic": traffic,
             "event_ms_per_step": event_ms / args.steps,
             "dominant_launch": dom,

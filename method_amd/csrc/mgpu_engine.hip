@@ -220,6 +220,8 @@ int rk2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   a.Uout = e->U1;
   a.Vout = e->V1;
   a.Aout = nullptr;
+  // V1 is read only by stage two's sweeps (vx .. p), guess and retry (rho .. p)
+  if (e->cfg.model == MGPU_SRMHD) a.vout_n = 5;
   if (run_sweeps(e, a, kSweepStage1)) return -1;
   if (retry(e, e->U1, e->V1, nullptr, st)) return -1;
 

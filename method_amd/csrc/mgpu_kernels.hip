@@ -207,8 +207,10 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
   srmhd_recover(un, a.gamma, v2, Z, P, A);
 #pragma unroll
   for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+  const int nvout = a.vout_n ? a.vout_n : kSrmhdPrims;
 #pragma unroll
-  for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+  for (int v = 0; v < kSrmhdPrims; ++v)
+    if (v < nvout) a.Vout[v * vs + q] = P[v];
   if (a.Aout) {
 #pragma unroll
     for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];

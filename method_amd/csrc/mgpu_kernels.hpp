@@ -42,6 +42,8 @@ struct SweepArgs {
   const double *Vg;  // prims giving the cons2prim starting guess
   double *Uout, *Vout, *Aout;
   double *root;      // SRMHD: cons2prim root (v^2, Z) [2][cell], kept when Aout is null
+  int vout_n;        // SRMHD: prims written to Vout (0: all 8; 5: rho .. p, for a stage whose
+                     // prims are read only by the next stage's sweeps, guess and retry)
   double dt, gamma, sigma, cp;
   double *F1;        // SSP2: flux1 store
   const double *S1;  // SSP2: source1 (components 8..12)
@@ -115,7 +117,6 @@ struct RkStageArgs {
   const double *ext;  // SRMHD model-extension source of the rhs state (variables 0..7), or null
   const double *Vg;
   double *Uout, *Vout, *Aout;
-  double *root;      // SRMHD: cons2prim root (v^2, Z) [2][cell], kept when Aout is null
   double dt, gamma, sigma, cp;
   C2PCounters *stats;
   int *fail_count;
