@@ -240,8 +240,9 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, int 
   srmhd_recover(un, a.gamma, v2, Z, P, A);
 #pragma unroll
   for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+  // stage one: rho .. p of V1, all that stage two's sweeps, guess and retry read
 #pragma unroll
-  for (int v = 0; v < kSrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+  for (int v = 0; v < (STAGE == 1 ? 5 : kSrmhdPrims); ++v) a.Vout[v * vs + q] = P[v];
   if (STAGE == 2) {
     a.root[q] = v2;
     a.root[vs + q] = Z;
