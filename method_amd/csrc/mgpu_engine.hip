@@ -699,6 +699,9 @@ int mgpu_sizes(const mgpu_engine *e, int *ncons, int *nprims, int *naux) {
 int mgpu_upload_state(mgpu_engine *e, const double *cons, const double *prims, const double *aux) {
   if (!e) return fail("null engine");
   if (ensure_staging(e)) return -1;
+  // an aux left as a root belongs to the state about to be replaced: form it
+  // now, so a partial upload keeps it as the arrays it does not replace
+  if (!aux && fresh_aux(e)) return -1;
   const double *src[3] = {cons, prims, aux};
   double *dst[3] = {e->U, e->V, e->A};
   const int nv[3] = {e->ncons, e->nprims, e->naux};

@@ -443,3 +443,25 @@ def test_aux_formed_from_kept_root_is_consistent(golden, integrator):
     S, B = c[1:4], p[5:8]
     assert np.max(np.abs(a[10] - (B * S).sum(axis=0))) <= 1e-14 * np.abs(a[10]).max()
     assert np.max(np.abs(a[12] - (S * S).sum(axis=0))) <= 1e-14 * np.abs(a[12]).max()
+
+
+def test_partial_upload_keeps_the_aux_it_does_not_replace(golden):
+    """mgpu_upload_state with cons and prims but no aux leaves the device aux
+    as it was. After RK2 steps, which keep the cons2prim root in place of the
+    aux, that is the stepped state's aux, formed before the upload."""
+    z = golden("kh3d_16_weno5_rk2")
+    say("gpu: partial upload after RK2 steps, KH 3-D 16^3")
+    with engine(weno=5, **KH3D16) as e:
+        e.set_prims(z["prims0"])
+        for _ in range(2):
+            assert e.update_time() == 0
+        _, _, want = e.download()
+    with engine(weno=5, **KH3D16) as e:
+        e.set_prims(z["prims0"])
+        c0, p0, _ = e.download()
+        for _ in range(2):
+            assert e.update_time() == 0
+        e.upload(cons=c0, prims=p0)
+        c, p, a = e.download()
+    assert np.array_equal(c, c0) and np.array_equal(p, p0)
+    assert np.array_equal(a, want)
