@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--batch-members", type=int, default=64)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched-ensemble leg and print its JSON (for profiling its kernels)")
+    ap.add_argument("--sweep-plan", type=int, default=None,
+                    help="sweep plan of the headline engine (mgpu_set_sweep_plan); default: the engine's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -326,6 +328,8 @@ def main():
                      weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW, eng.PERIODIC), cfl=0.4)
     cells = cfg.interior_cells
     e = eng.Engine(cfg)
+    if args.sweep_plan is not None:
+        e.set_sweep_plan(args.sweep_plan)
     e.set_prims(prims0)
     del prims0
     print(f"bench: {n}^3 set up; warmup {args.warmup} steps", file=sys.stderr, flush=True)

@@ -123,6 +123,40 @@ int mgpu_flux_vector(mgpu_engine *e, int dir, double *f);
  * state, written for every padded cell. */
 int mgpu_source(mgpu_engine *e, double *source);
 
+/* FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56, declared at
+ * Project/CPU/Include/flux.h:72): the Lax-Friedrichs split
+ * f+- = (f +- cons)/2 (alpha = 1) and the WENO face flux at i - 1/2 along
+ * axis dir (0, 1, 2 = x, y, z), upwind on f+ plus downwind on f-
+ * (weno.cc:23-118). cons, f and frecon are host arrays of the first `vars`
+ * variables in the padded reference layout (vars < 0: all of the model's
+ * conserved variables); every padded cell is an input, as after Bcs::apply.
+ * frecon is written for those variables over every padded cell: the face
+ * flux where the index along dir lies in [shift, N - shift), shift =
+ * (order + 1)/2, and 0 elsewhere, as the reference's zero-initialised
+ * temporaries leave it. With cons and f both NULL the input is the resident
+ * state: its cons and its Model::fluxVector along dir (as mgpu_flux_vector
+ * forms them, ghost cells holding their boundary images). */
+int mgpu_flux_reconstruction(mgpu_engine *e, const double *cons, const double *f, int dir, int vars,
+                             double *frecon);
+
+/* Model::getPrimitiveVarsSingleCell (srmhd.cc:297-388, srrmhd.cc:256-312;
+ * model.h:85) for n independent cells. Host arrays hold the cells one after
+ * the other, each in the single-cell layout the reference passes
+ * (cons[Ncons], prims[Nprims], aux[Naux]). prims and aux carry the starting
+ * guess in (SRMHD: v^2 from prims, rho h / (1 - v^2) from prims[0] and
+ * aux[0]; SRRMHD: aux[10], rho h W^2) and the recovered state out. Returns
+ * the number of cells that did not converge (the reference throws for such
+ * a cell); stats may be NULL. */
+int mgpu_c2p_cells(mgpu_engine *e, long long n, const double *cons, double *prims, double *aux,
+                   mgpu_stats *stats);
+
+/* Model::sourceTermSingleCell (srmhd.cc:212-223, srrmhd.cc:194-207;
+ * model.h:55) for n independent cells in the single-cell layout. SRMHD reads
+ * cons only; SRRMHD reads cons and the current J from aux[4..6]. prims is
+ * not read and may be NULL. */
+int mgpu_source_cells(mgpu_engine *e, long long n, const double *cons, const double *prims, const double *aux,
+                      double *source);
+
 /* Model extensions (ModelExtension, modelExtension.h:14-59): MGPU_EXT_REGIME
  * is REGIME (REGIME.h:81, REGIME.cc), the resistive extension of ideal SRMHD,
  * whose source RKSplit::setSource (rkSplit.cc:15-25) and RKPlus::rhs
@@ -152,8 +186,9 @@ int mgpu_c2p(mgpu_engine *e, mgpu_stats *stats);
  * flux split take the light speed as the bound: simulation.cc:39-41,
  * fluxVectorSplitting.cc:10). out[d] = the largest |characteristic speed|
  * along axis d over interior cells of the resident state, 0 for axes the
- * grid does not have. SRMHD: the fast magnetosonic bound from the resident
- * prims and aux; SRRMHD: 1, the speed of its light modes. Synchronous. */
+ * grid does not have: the fast magnetosonic bound from the resident prims
+ * and aux. SRMHD engines only; an SRRMHD engine gets an error, since its
+ * light modes move at 1 in every cell. Synchronous. */
 int mgpu_max_wavespeed(mgpu_engine *e, double out[3]);
 
 /* Starting guess of mgpu_c2p's SRMHD Newton solve. The reference starts from
@@ -165,6 +200,15 @@ int mgpu_max_wavespeed(mgpu_engine *e, double out[3]);
  * previous primitives (counted in mgpu_stats.fallbacks). MGPU_GUESS_COLD
  * starts from v^2 = 0, Z = tau + D, for comparison. */
 int mgpu_set_c2p_guess(mgpu_engine *e, int mode, const float *params, const float *ymean_ystd);
+
+/* Newton-step histogram of the SRMHD cons2prim (no reference counterpart;
+ * the reference's hybrd keeps no per-cell counts). With on = 1 the 64 bins
+ * are cleared and every later mgpu_c2p / mgpu_timed_c2p call adds each cell
+ * of its main solve to bin min(steps, 63) (a learned-guess cell that falls
+ * back counts both solves); cells that do not converge go to bin 63. on = 0
+ * stops counting. mgpu_get_c2p_histogram copies the 64 bins out. */
+int mgpu_set_c2p_histogram(mgpu_engine *e, int on);
+int mgpu_get_c2p_histogram(mgpu_engine *e, long long *hist);
 
 /* Times the cons2prim kernel alone: reps calls of mgpu_c2p's kernel on the
  * resident cons, each starting from the resident prims as they were on entry

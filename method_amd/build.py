@@ -13,6 +13,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "method_amd")
@@ -24,7 +25,8 @@ INCLUDE = os.path.join(ROOT, "include")
 GPU_LIB = os.path.join(LIBDIR, "libmethod_gpu.so")
 HOST_LIB = os.path.join(LIBDIR, "libmethod_host.so")
 
-GPU_SOURCES = ["mgpu_kernels.hip", "mgpu_engine.hip", "mgpu_mlp.hip", "mgpu_batch.hip", "mgpu_regime.hip"]
+GPU_SOURCES = ["mgpu_kernels.hip", "mgpu_engine.hip", "mgpu_mlp.hip", "mgpu_batch.hip", "mgpu_regime.hip",
+               "mgpu_cells.hip"]
 ARCH = "gfx950"
 
 
@@ -53,11 +55,11 @@ def gpu_flags():
 
 def build_gpu() -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
-    for src in GPU_SOURCES:
-        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
-        _run([hipcc()] + gpu_flags() + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(LIBDIR, src.replace(".hip", ".o")) for src in GPU_SOURCES]
+    # one hipcc per source, run side by side (each is single-threaded)
+    with ThreadPoolExecutor(max_workers=min(len(GPU_SOURCES), os.cpu_count() or 1, 8)) as pool:
+        list(pool.map(lambda so: _run([hipcc()] + gpu_flags() + ["-c", os.path.join(CSRC, so[0]), "-o", so[1]]),
+                      zip(GPU_SOURCES, objs)))
     tmp = GPU_LIB + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, GPU_LIB)

@@ -744,6 +744,10 @@ int mgpu_batch_create(const mgpu_batch_config *cfg, const mgpu_instance *inst, i
     if (in.nx < c.Ng || in.ny < c.Ng) return engine_fail("mgpu_batch_create: instance %d has fewer than Ng cells", i);
     if (!(in.xmax > in.xmin) || !(in.ymax > in.ymin) || !(in.cfl > 0.0))
       return engine_fail("mgpu_batch_create: instance %d has a bad domain or cfl", i);
+    // An instance that could never step would keep no cons2prim root, and the
+    // aux formed from the roots after any batch step would be meaningless.
+    if (!(in.end_time > 0.0) || !std::isfinite(in.end_time))
+      return engine_fail("mgpu_batch_create: instance %d needs a finite end_time > 0", i);
   }
   BATCH_OK(hipSetDevice(c.device));
   mgpu_batch *b = new mgpu_batch();

@@ -323,6 +323,21 @@ __device__ __forceinline__ void srmhd_plain_guess(double rho, double vx, double 
   Z = rho * h / (1.0 - v2);
 }
 
+// Newton-step histogram of one block's cells: bin min(steps, 63), failed
+// cells (steps < 0) in bin 63. Every thread of the block calls it; hist null
+// (the histogram is off) makes it a no-op.
+constexpr int kHistBins = 64;
+__device__ __forceinline__ void hist_add_block(unsigned long long *hist, bool counted, int steps) {
+  if (!hist) return;
+  __shared__ unsigned s_hist[kHistBins];
+  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) s_hist[b] = 0u;
+  __syncthreads();
+  if (counted) atomicAdd(&s_hist[steps < 0 || steps >= kHistBins - 1 ? kHistBins - 1 : steps], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x)
+    if (s_hist[b]) atomicAdd(&hist[b], (unsigned long long)s_hist[b]);
+}
+
 // RK2 stage updates on one conserved value (RK2.cc:31-91), written with
 // explicit fused multiply-adds so that every kernel rounds them alike:
 // stage one U1 = U - dt F(U), stage two U = (U + U1 - dt F(U1)) / 2.

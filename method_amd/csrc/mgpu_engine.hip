@@ -86,11 +86,19 @@ struct mgpu_engine {
   MlpDev *mlp = nullptr;              // learned guess network (device)
   unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
   int fused_yz = 0;                      // 3-D: fused z+y sweep launch (off: slower here)
+  int xyz_order = 0;                     // sweeps x, y, z with the last (contiguous) one finishing
   // SRMHD RK2 and RKSplit steps keep the cons2prim root (v^2, Z) of each cell
   // instead of writing its 13 aux values: no kernel of those steps reads aux,
   // and what does (state download, wavespeed) forms it from the root first.
   double *root = nullptr;
   bool aux_stale = false;
+  // cell-list operators (single-cell model calls): device copies of the
+  // cells and their per-block counter records, grown on demand
+  unsigned long long *hist = nullptr;  // Newton-step histogram of the standalone cons2prim, when on
+  double *cells_buf = nullptr;
+  size_t cells_elems = 0;
+  CellCounters *cells_part = nullptr;
+  long long cells_part_n = 0;
 };
 
 namespace {
@@ -186,16 +194,29 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
     // the x sweep is the only one: the earlier axes' net flux it adds to is zero
     HIP_OK(hipMemsetAsync(e->fnet, 0, sizeof(double) * e->ncons * e->geo.vstride, e->stream));
   }
+  static const char *names[3][6] = {
+      {"sweep_x_write", "sweep_x_add", "sweep_x_stage1_c2p", "sweep_x_stage2_c2p", "sweep_x_imex2", "sweep_x_final"},
+      {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y_imex2", "sweep_y_final"},
+      {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z_imex2",
+       "sweep_z_final"}};
+  if (e->xyz_order) {
+    // x, then y, then z: the reference's summation order (fluxVectorSplitting.cc:79-112);
+    // the last axis, the contiguous one, applies last_mode
+    for (int dir = 0; dir < dims; ++dir) {
+      const bool last = dir == dims - 1;
+      int mode = dir == 0 ? kSweepWrite : kSweepAdd;
+      if (last && last_mode != kSweepWrite && last_mode != kSweepAdd) mode = last_mode;
+      launch_sweep(a, e->cfg.model, order, dir, mode, quirk, e->stream);
+      if (mark(e, names[dir][mode])) return -1;
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
   int first = 1;
   for (int dir = dims - 1; dir >= 0; --dir) {
     int mode = dir == 0 ? last_mode : (first ? kSweepWrite : kSweepAdd);
     if (dir == 0 && (last_mode == kSweepWrite || last_mode == kSweepAdd)) mode = first ? kSweepWrite : kSweepAdd;
     launch_sweep(a, e->cfg.model, order, dir, mode, quirk, e->stream);
-    static const char *names[3][6] = {
-        {"sweep_x_write", "sweep_x_add", "sweep_x_stage1_c2p", "sweep_x_stage2_c2p", "sweep_x_imex2", "sweep_x_final"},
-        {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y_imex2", "sweep_y_final"},
-        {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z_imex2",
-         "sweep_z_final"}};
     if (mark(e, names[dir][mode])) return -1;
     first = 0;
   }
@@ -683,6 +704,9 @@ int mgpu_destroy(mgpu_engine *e) {
   if (e->fail_list) hipFree(e->fail_list);
   if (e->fail_flag) hipFree(e->fail_flag);
   if (e->mlp) hipFree(e->mlp);
+  if (e->hist) hipFree(e->hist);
+  if (e->cells_buf) hipFree(e->cells_buf);
+  if (e->cells_part) hipFree(e->cells_part);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
   return 0;
@@ -875,13 +899,127 @@ int mgpu_source(mgpu_engine *e, double *source) {
   return 0;
 }
 
+int mgpu_flux_reconstruction(mgpu_engine *e, const double *cons, const double *f, int dir, int vars,
+                             double *frecon) {
+  if (!e || !frecon) return fail("mgpu_flux_reconstruction: null argument");
+  if (dir < 0 || dir > 2) return fail("mgpu_flux_reconstruction: dir %d out of range", dir);
+  if ((cons == nullptr) != (f == nullptr))
+    return fail("mgpu_flux_reconstruction: give both cons and f, or neither for the resident state");
+  const int nv = vars < 0 ? e->ncons : vars;
+  if (nv > e->ncons) return fail("mgpu_flux_reconstruction: vars %d exceeds the model's %d", vars, e->ncons);
+  if (ensure_staging(e)) return -1;
+  const size_t plane = (size_t)e->pad.ntot;
+  // staging: cons [ncons][padded], f [ncons][padded], frecon [ncons][padded]
+  double *sc = e->staging, *sf = sc + (size_t)e->ncons * plane, *sr = sf + (size_t)e->ncons * plane;
+  if (nv > 0) {
+    if (cons) {
+      HIP_OK(hipMemcpyAsync(sc, cons, sizeof(double) * nv * plane, hipMemcpyHostToDevice, e->stream));
+      HIP_OK(hipMemcpyAsync(sf, f, sizeof(double) * nv * plane, hipMemcpyHostToDevice, e->stream));
+    } else {
+      launch_expand(e->geo, e->pad, nv, e->U, sc, 0, e->stream);
+      if (dir < e->geo.dims) {
+        if (e->cfg.model == MGPU_SRMHD)
+          launch_flux_vector_srmhd(e->geo, e->pad, e->U, e->V, sf, dir, e->cfg.quirks, e->stream);
+        else
+          launch_flux_vector_srrmhd(e->geo, e->pad, e->U, e->V, sf, dir, e->cfg.quirks, e->cfg.gamma,
+                                    e->cfg.sigma, e->stream);
+      } else {
+        // an axis the grid does not have: every face lies outside the range
+        HIP_OK(hipMemsetAsync(sf, 0, sizeof(double) * nv * plane, e->stream));
+      }
+      HIP_OK(hipGetLastError());
+    }
+    launch_flux_recon(e->pad, e->cfg.weno_order, dir, nv, sc, sf, sr, e->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(frecon, sr, sizeof(double) * nv * plane, hipMemcpyDeviceToHost, e->stream));
+  }
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+namespace {
+// Device room for n cells of the single-cell layout and their counter records.
+int ensure_cells(mgpu_engine *e, long long n) {
+  const size_t need = (size_t)n * (e->ncons + e->nprims + e->naux + e->ncons);
+  if (e->cells_elems < need) {
+    if (e->cells_buf) HIP_OK(hipFree(e->cells_buf));
+    e->cells_buf = nullptr;
+    e->cells_elems = 0;
+    HIP_OK(hipMalloc(&e->cells_buf, need * sizeof(double)));
+    e->cells_elems = need;
+  }
+  const long long nb = cells_blocks(n) + 1;
+  if (e->cells_part_n < nb) {
+    if (e->cells_part) HIP_OK(hipFree(e->cells_part));
+    e->cells_part = nullptr;
+    e->cells_part_n = 0;
+    HIP_OK(hipMalloc(&e->cells_part, nb * sizeof(CellCounters)));
+    e->cells_part_n = nb;
+  }
+  return 0;
+}
+}  // namespace
+
+int mgpu_c2p_cells(mgpu_engine *e, long long n, const double *cons, double *prims, double *aux,
+                   mgpu_stats *stats) {
+  if (!e) return fail("null engine");
+  if (n < 0) return fail("mgpu_c2p_cells: negative cell count");
+  if (n > 0 && (!cons || !prims || !aux)) return fail("mgpu_c2p_cells: null array");
+  if (ensure_cells(e, std::max(n, 1LL))) return -1;
+  double *dc = e->cells_buf, *dp = dc + (size_t)n * e->ncons, *da = dp + (size_t)n * e->nprims;
+  const size_t bc = sizeof(double) * n * e->ncons, bp = sizeof(double) * n * e->nprims,
+               ba = sizeof(double) * n * e->naux;
+  if (n > 0) {
+    HIP_OK(hipMemcpyAsync(dc, cons, bc, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(dp, prims, bp, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(da, aux, ba, hipMemcpyHostToDevice, e->stream));
+  }
+  CellCounters *total = e->cells_part + cells_blocks(n);
+  launch_c2p_cells(e->cfg.model, n, dc, dp, da, e->cfg.gamma, e->cfg.sigma, e->cells_part, total, e->stream);
+  HIP_OK(hipGetLastError());
+  CellCounters t;
+  if (n > 0) {
+    HIP_OK(hipMemcpyAsync(prims, dp, bp, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipMemcpyAsync(aux, da, ba, hipMemcpyDeviceToHost, e->stream));
+  }
+  HIP_OK(hipMemcpyAsync(&t, total, sizeof t, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->cells = (long long)t.cells;
+    stats->iterations = (long long)t.iterations;
+    stats->failures = (long long)t.failures;
+    stats->max_iterations = (long long)t.max_iterations;
+  }
+  return t.failures > 0 ? (int)std::min<unsigned long long>(t.failures, 1ULL << 30) : 0;
+}
+
+int mgpu_source_cells(mgpu_engine *e, long long n, const double *cons, const double *prims, const double *aux,
+                      double *source) {
+  (void)prims;
+  if (!e) return fail("null engine");
+  if (n < 0) return fail("mgpu_source_cells: negative cell count");
+  const bool needs_aux = e->cfg.model != MGPU_SRMHD;
+  if (n > 0 && (!cons || !source || (needs_aux && !aux))) return fail("mgpu_source_cells: null array");
+  if (n == 0) return 0;
+  if (ensure_cells(e, n)) return -1;
+  double *dc = e->cells_buf, *da = dc + (size_t)n * e->ncons, *ds = da + (size_t)n * e->naux;
+  HIP_OK(hipMemcpyAsync(dc, cons, sizeof(double) * n * e->ncons, hipMemcpyHostToDevice, e->stream));
+  if (needs_aux) HIP_OK(hipMemcpyAsync(da, aux, sizeof(double) * n * e->naux, hipMemcpyHostToDevice, e->stream));
+  launch_source_cells(e->cfg.model, n, dc, da, e->cfg.cp, ds, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(source, ds, sizeof(double) * n * e->ncons, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
 namespace {
 int launch_c2p(mgpu_engine *e) {
   if (e->cfg.model == MGPU_SRMHD && e->guess_mode == MGPU_GUESS_LEARNED)
-    launch_c2p_learned_srmhd(e->geo, e->U, e->V, e->A, e->mlp, e->cfg.gamma, e->dstats, e->stream);
+    launch_c2p_learned_srmhd(e->geo, e->U, e->V, e->A, e->mlp, e->cfg.gamma, e->dstats, e->hist, e->stream);
   else if (e->cfg.model == MGPU_SRMHD)
     launch_c2p_srmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->dstats, e->fail_count, e->fail_list, e->fail_flag,
-                     e->guess_mode == MGPU_GUESS_COLD, e->stream);
+                     e->guess_mode == MGPU_GUESS_COLD, e->hist, e->stream);
   else
     launch_c2p_srrmhd(e->geo, e->U, e->V, e->A, e->cfg.gamma, e->cfg.sigma, e->dstats, e->stream);
   HIP_OK(hipGetLastError());
@@ -949,12 +1087,35 @@ int mgpu_set_c2p_guess(mgpu_engine *e, int mode, const float *params, const floa
   return 0;
 }
 
-int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
-  if (!e || !out) return fail("mgpu_max_wavespeed: null argument");
-  if (e->cfg.model != MGPU_SRMHD) {
-    for (int d = 0; d < 3; ++d) out[d] = d < e->geo.dims ? 1.0 : 0.0;
+int mgpu_set_c2p_histogram(mgpu_engine *e, int on) {
+  if (!e) return fail("null engine");
+  if (on && e->cfg.model != MGPU_SRMHD) return fail("mgpu_set_c2p_histogram: SRMHD engines only");
+  if (!on) {
+    if (e->hist) HIP_OK(hipFree(e->hist));
+    e->hist = nullptr;
     return 0;
   }
+  if (!e->hist) HIP_OK(hipMalloc(&e->hist, kHistBins * sizeof(unsigned long long)));
+  HIP_OK(hipMemsetAsync(e->hist, 0, kHistBins * sizeof(unsigned long long), e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int mgpu_get_c2p_histogram(mgpu_engine *e, long long *hist) {
+  if (!e || !hist) return fail("null argument");
+  if (!e->hist) return fail("mgpu_get_c2p_histogram: the histogram is off");
+  unsigned long long h[kHistBins];
+  HIP_OK(hipMemcpyAsync(h, e->hist, sizeof h, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  for (int b = 0; b < kHistBins; ++b) hist[b] = (long long)h[b];
+  return 0;
+}
+
+int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
+  if (!e || !out) return fail("mgpu_max_wavespeed: null argument");
+  if (e->cfg.model != MGPU_SRMHD)
+    return fail("mgpu_max_wavespeed: only the SRMHD fast magnetosonic bound is reduced; the resistive "
+                "model's light modes travel at 1 in every cell");
   if (!e->dspeed) HIP_OK(hipMalloc(&e->dspeed, 3 * sizeof(unsigned long long)));
   if (fresh_aux(e)) return -1;
   HIP_OK(hipMemsetAsync(e->dspeed, 0, 3 * sizeof(unsigned long long), e->stream));
@@ -969,7 +1130,9 @@ int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
 
 int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz) {
   if (!e) return fail("null engine");
-  e->fused_yz = fused_yz ? 1 : 0;
+  if (fused_yz < 0 || fused_yz > 2) return fail("mgpu_set_sweep_plan: plan %d out of range", fused_yz);
+  e->fused_yz = fused_yz == 1;
+  e->xyz_order = fused_yz == 2;
   return 0;
 }
 

@@ -684,9 +684,10 @@ __global__ __launch_bounds__(kThreads) void k_aux_srmhd(Geo g, const double *U, 
 // cold: start from v^2 = 0, Z = tau + D instead of the previous primitives.
 __global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, double *V, double *A, double gam,
                                                          C2PCounters *st, int *fail_count, long long *fail_list,
-                                                         unsigned char *fail_flag, int cold) {
+                                                         unsigned char *fail_flag, int cold,
+                                                         unsigned long long *hist) {
   const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
-  int cells = 0, iters = 0, fails = 0, maxit = 0;
+  int cells = 0, iters = 0, fails = 0, maxit = 0, steps = -1;
   if (q < g.vstride) {
     const long long vs = g.vstride;
     double u[9];
@@ -710,6 +711,7 @@ __global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, 
 #pragma unroll
     for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
     cells = 1;
+    steps = its;
     if (its < 0) {
       fails = 1;
       const int slot = atomicAdd(fail_count, 1);
@@ -721,6 +723,7 @@ __global__ __launch_bounds__(kThreads) void k_c2p_srmhd(Geo g, const double *U, 
     }
   }
   reduce_stats(st, cells, iters, fails, maxit);
+  hist_add_block(hist, cells > 0, steps);
 }
 
 // Retry of failed cells from the mean root of converged interior face
@@ -1321,9 +1324,10 @@ void launch_aux_srmhd(const Geo &g, const double *U, const double *root, double 
 }
 
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
-                      int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold, hipStream_t s) {
+                      int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold,
+                      unsigned long long *hist, hipStream_t s) {
   hipLaunchKernelGGL(k_c2p_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, V, A, gam, st, fail_count,
-                     fail_list, fail_flag, cold);
+                     fail_list, fail_flag, cold, hist);
 }
 
 void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double *root, double gam,

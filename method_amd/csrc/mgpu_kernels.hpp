@@ -80,8 +80,10 @@ void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double 
 // SRMHD aux from the resident cons and the cons2prim root kept by a stage,
 // through the same recovery formulas the stage would have applied.
 void launch_aux_srmhd(const Geo &g, const double *U, const double *root, double *A, double gam, hipStream_t s);
+// hist: null, or kHistBins Newton-step counters the cells are added to.
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
-                      int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold, hipStream_t s);
+                      int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold,
+                      unsigned long long *hist, hipStream_t s);
 void launch_c2p_retry_srmhd(const Geo &g, const double *U, double *V, double *A, double *root, double gam,
                             const int *fail_count, const long long *fail_list, unsigned char *fail_flag,
                             long long last_cell, C2PCounters *st, hipStream_t s);
@@ -151,6 +153,26 @@ void launch_compact(const Geo &g, const PadGeo &pg, int nvar, const double *src,
 void launch_expand(const Geo &g, const PadGeo &pg, int nvar, const double *src, double *dst, int ghosts_zero,
                    hipStream_t s);
 void launch_source_srmhd(const Geo &g, const PadGeo &pg, const double *U, double *dst, double cp, hipStream_t s);
+
+// ---- reference-API operators outside the time step (mgpu_cells.hip)
+// FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56) of variables
+// [0, nvars) along axis dir on padded arrays of stride pg.ntot.
+void launch_flux_recon(const PadGeo &pg, int order, int dir, int nvars, const double *cons, const double *f,
+                       double *frecon, hipStream_t s);
+// Counter record of the cell-list cons2prim: one per block, then their sum.
+struct CellCounters {
+  unsigned long long cells, iterations, failures, max_iterations;
+};
+constexpr int kCellBlock = 256;  // threads (cells) per block of the cell-list kernels
+long long cells_blocks(long long n);
+// Model::getPrimitiveVarsSingleCell for n cells in the single-cell layout
+// ([cell][var]); prims and aux carry the starting guess in and the result
+// out. partial receives cells_blocks(n) records, total their sum.
+void launch_c2p_cells(int model, long long n, const double *cons, double *prims, double *aux, double gam,
+                      double sigma, CellCounters *partial, CellCounters *total, hipStream_t s);
+// Model::sourceTermSingleCell for n cells ([cell][var]).
+void launch_source_cells(int model, long long n, const double *cons, const double *aux, double cp, double *src,
+                         hipStream_t s);
 void launch_flux_vector_srmhd(const Geo &g, const PadGeo &pg, const double *U, const double *V, double *dst, int dir,
                               int quirk, hipStream_t s);
 

@@ -241,7 +241,8 @@ __global__ __launch_bounds__(kThr) void k_mlp_predict(long long n, const double 
 // does not converge restart from the previous primitives (the reference's
 // guess, srmhd.cc:438-442) and count as fallbacks.
 __global__ __launch_bounds__(kThr) void k_c2p_learned_srmhd(Geo g, const double *U, double *V, double *A,
-                                                             const MlpDev *m, double gam, C2PCounters *st) {
+                                                             const MlpDev *m, double gam, C2PCounters *st,
+                                                             unsigned long long *hist) {
   __shared__ float lds[kThr / 64][64 * 10];
   const long long q = (long long)blockIdx.x * kThr + threadIdx.x;
   const bool live = q < g.vstride;
@@ -281,6 +282,7 @@ __global__ __launch_bounds__(kThr) void k_c2p_learned_srmhd(Geo g, const double 
     maxit = total;
     fails = its < 0;
   }
+  hist_add_block(hist, cells > 0, fails ? -1 : iters);
   if (st) {
     for (int off = 32; off > 0; off >>= 1) {
       cells += __shfl_xor(cells, off);
@@ -305,8 +307,9 @@ inline unsigned nblk(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 }  // namespace
 
 void mgpu::launch_c2p_learned_srmhd(const Geo &g, const double *U, double *V, double *A, const MlpDev *m, double gam,
-                                    C2PCounters *st, hipStream_t s) {
-  hipLaunchKernelGGL(k_c2p_learned_srmhd, dim3(nblk(g.vstride, kThr)), dim3(kThr), 0, s, g, U, V, A, m, gam, st);
+                                    C2PCounters *st, unsigned long long *hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_c2p_learned_srmhd, dim3(nblk(g.vstride, kThr)), dim3(kThr), 0, s, g, U, V, A, m, gam, st,
+                     hist);
 }
 
 // Packs FP32 parameters and output scales into the inference form.

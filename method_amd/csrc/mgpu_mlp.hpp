@@ -96,7 +96,8 @@ struct C2PCounters;
 // Packs FP32 parameters (the flat layout above) and output scales into MlpDev.
 void mlp_pack(const float *params, const float *ymean, const float *ystd, MlpDev *out);
 // SRMHD cons2prim of every interior cell with the learned starting guess.
+// hist: null, or 64 Newton-step counters (both solves of a fallback summed).
 void launch_c2p_learned_srmhd(const Geo &g, const double *U, double *V, double *A, const MlpDev *m, double gam,
-                              C2PCounters *st, hipStream_t s);
+                              C2PCounters *st, unsigned long long *hist, hipStream_t s);
 
 }  // namespace mgpu

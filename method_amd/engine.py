@@ -28,7 +28,7 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_set_regime_plan", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_flux_reconstruction", "mgpu_c2p_cells", "mgpu_source_cells", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_set_c2p_histogram", "mgpu_get_c2p_histogram", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_set_regime_plan", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
     "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_sizes", "mgpu_batch_set_prims", "mgpu_batch_upload_state",
@@ -112,6 +112,9 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_fnet.argtypes = [E, dp]
     L.mgpu_flux_vector.argtypes = [E, ctypes.c_int, dp]
     L.mgpu_source.argtypes = [E, dp]
+    L.mgpu_flux_reconstruction.argtypes = [E, dp, dp, ctypes.c_int, ctypes.c_int, dp]
+    L.mgpu_c2p_cells.argtypes = [E, ctypes.c_longlong, dp, dp, dp, sp]
+    L.mgpu_source_cells.argtypes = [E, ctypes.c_longlong, dp, dp, dp, dp]
     L.mgpu_set_model_extension.argtypes = [E, ctypes.c_int]
     L.mgpu_source_extension.argtypes = [E, dp]
     L.mgpu_regime_fields.argtypes = [E, dp, dp, dp, dp, dp]
@@ -123,6 +126,8 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_set_sweep_plan.argtypes = [E, ctypes.c_int]
     L.mgpu_set_regime_plan.argtypes = [E, ctypes.c_int]
     L.mgpu_timed_c2p.argtypes = [E, ctypes.c_int, dp, sp]
+    L.mgpu_set_c2p_histogram.argtypes = [E, ctypes.c_int]
+    L.mgpu_get_c2p_histogram.argtypes = [E, ctypes.POINTER(ctypes.c_longlong)]
     L.mgpu_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(E)]
     L.mgpu_mlp_destroy.argtypes = [E]
     L.mgpu_mlp_set_data.argtypes = [E, dp, dp, ctypes.c_longlong]
@@ -290,9 +295,10 @@ class Engine:
     def sync(self):
         self._check(self.L.mgpu_sync(self._h), "mgpu_sync")
 
-    def set_sweep_plan(self, fused_yz: bool):
-        """3-D grids: separate z and y sweeps (default) or one fused launch."""
-        self._check(self.L.mgpu_set_sweep_plan(self._h, 1 if fused_yz else 0), "mgpu_set_sweep_plan")
+    def set_sweep_plan(self, plan):
+        """0: sweeps z, y, x (x finishes); 1 (or True): fused z+y launch, then x;
+        2: sweeps x, y, z (the contiguous axis finishes)."""
+        self._check(self.L.mgpu_set_sweep_plan(self._h, int(plan)), "mgpu_set_sweep_plan")
 
     def set_regime_plan(self, fused: bool):
         """REGIME source: D formed on chip in the differencing launch (default)
@@ -313,6 +319,40 @@ class Engine:
     def source(self) -> np.ndarray:
         out = self.zeros(self.ncons)
         self._check(self.L.mgpu_source(self._h, _dp(out)), "mgpu_source")
+        return out
+
+    def flux_reconstruction(self, dir: int, cons=None, f=None, vars: int = -1) -> np.ndarray:
+        """FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56): face fluxes at
+        i - 1/2 along `dir` of given padded (cons, f), or of the resident state
+        and its flux vector when both are None."""
+        nv = self.ncons if vars < 0 else vars
+        out = self.zeros(self.ncons)
+        c = None if cons is None else np.ascontiguousarray(cons, dtype=np.float64)
+        ff = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        self._check(self.L.mgpu_flux_reconstruction(self._h, _dp(c), _dp(ff), dir, vars, _dp(out)),
+                    "mgpu_flux_reconstruction")
+        return out[:nv]
+
+    def c2p_cells(self, cons: np.ndarray, prims: np.ndarray, aux: np.ndarray):
+        """Model::getPrimitiveVarsSingleCell over n cells ([cell][var] arrays);
+        prims and aux are the starting guess. Returns (rc, prims, aux, stats)."""
+        c = np.ascontiguousarray(cons, dtype=np.float64)
+        p = np.array(prims, dtype=np.float64, order="C", copy=True)
+        a = np.array(aux, dtype=np.float64, order="C", copy=True)
+        n = c.shape[0]
+        assert c.shape == (n, self.ncons) and p.shape == (n, self.nprims) and a.shape == (n, self.naux)
+        st = MgpuStats()
+        rc = self._check(self.L.mgpu_c2p_cells(self._h, n, _dp(c), _dp(p), _dp(a), ctypes.byref(st)),
+                         "mgpu_c2p_cells")
+        return rc, p, a, st.as_dict()
+
+    def source_cells(self, cons: np.ndarray, aux=None) -> np.ndarray:
+        """Model::sourceTermSingleCell over n cells ([cell][var] arrays)."""
+        c = np.ascontiguousarray(cons, dtype=np.float64)
+        a = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+        out = np.zeros_like(c)
+        self._check(self.L.mgpu_source_cells(self._h, c.shape[0], _dp(c), None, _dp(a), _dp(out)),
+                    "mgpu_source_cells")
         return out
 
     def set_model_extension(self, kind: int):
@@ -356,6 +396,18 @@ class Engine:
         st = MgpuStats()
         rc = self._check(self.L.mgpu_timed_c2p(self._h, reps, ctypes.byref(ms), ctypes.byref(st)), "mgpu_timed_c2p")
         return ms.value, st.as_dict(), rc
+
+    def set_c2p_histogram(self, on: bool):
+        """Start (clearing the 64 bins) or stop counting Newton steps per cell
+        in c2p() / timed_c2p()."""
+        self._check(self.L.mgpu_set_c2p_histogram(self._h, 1 if on else 0), "mgpu_set_c2p_histogram")
+
+    def c2p_histogram(self) -> np.ndarray:
+        """Cells per Newton-step count (bin 63: 63 or more, or not converged)."""
+        h = np.zeros(64, dtype=np.int64)
+        self._check(self.L.mgpu_get_c2p_histogram(self._h, h.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))),
+                    "mgpu_get_c2p_histogram")
+        return h
 
     def set_c2p_guess(self, mode: int, params=None, ymean_ystd=None):
         """Starting guess of c2p(): GUESS_PLAIN (previous primitives, the

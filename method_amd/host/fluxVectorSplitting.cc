@@ -19,8 +19,15 @@ void WenoBase::checkSufficientGhostZones() {
   }
 }
 
-void FVS::fluxReconstruction(double *, double *, double *, double *, double *, int, int) {
-  throw std::logic_error("FVS::fluxReconstruction: the engine reconstructs inside its flux sweeps; call FVS::F");
+// The face fluxes of the given arrays (fluxVectorSplitting.cc:4-56), formed on
+// the device from copies of cons and f; the resident state is not touched.
+void FVS::fluxReconstruction(double *cons, double *prims, double *aux, double *f, double *frecon, int dir,
+                             int vars) {
+  (void)prims;
+  (void)aux;
+  if (weno) data->gpu.wenoOrder = weno->order;
+  mgpu_engine *e = method_host::engine(data);
+  method_host::check(mgpu_flux_reconstruction(e, cons, f, dir, vars, frecon), "FVS::fluxReconstruction");
 }
 
 void FVS::F(double *cons, double *prims, double *aux, double *f, double *fnet) {

@@ -38,8 +38,9 @@ void SRMHD::fluxVector(double *cons, double *prims, double *aux, double *f, cons
   data->gpu.deviceCurrent = false;
 }
 
-void SRMHD::sourceTermSingleCell(double *, double *, double *, double *, int, int, int) {
-  throw std::logic_error("SRMHD::sourceTermSingleCell: single-cell model calls run inside the engine's kernels");
+// One cell through the engine's cell-list kernels (srmhd.cc:212-223).
+void SRMHD::sourceTermSingleCell(double *cons, double *prims, double *aux, double *source, int, int, int) {
+  check(mgpu_source_cells(engine(data), 1, cons, prims, aux, source), "SRMHD::sourceTermSingleCell");
 }
 
 void SRMHD::sourceTerm(double *cons, double *prims, double *aux, double *source) {
@@ -51,8 +52,12 @@ void SRMHD::sourceTerm(double *cons, double *prims, double *aux, double *source)
   data->gpu.deviceCurrent = false;
 }
 
-void SRMHD::getPrimitiveVarsSingleCell(double *, double *, double *, int, int, int) {
-  throw std::logic_error("SRMHD::getPrimitiveVarsSingleCell: single-cell model calls run inside the engine's kernels");
+// One cell through the engine's cell-list kernels (srmhd.cc:297-388): the
+// start comes from the cell's prims and aux h, and a cell that does not
+// converge throws as the reference's does.
+void SRMHD::getPrimitiveVarsSingleCell(double *cons, double *prims, double *aux, int, int, int) {
+  if (check(mgpu_c2p_cells(engine(data), 1, cons, prims, aux, nullptr), "SRMHD::getPrimitiveVarsSingleCell") > 0)
+    throw std::runtime_error("C2P could not converge.\n");
 }
 
 // Failures are reported on stdout as the reference's retry path does

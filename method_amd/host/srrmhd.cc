@@ -38,8 +38,10 @@ void SRRMHD::fluxVector(double *cons, double *prims, double *aux, double *f, con
   data->gpu.deviceCurrent = false;
 }
 
-void SRRMHD::sourceTermSingleCell(double *, double *, double *, double *, int, int, int) {
-  throw std::logic_error("SRRMHD::sourceTermSingleCell: single-cell model calls run inside the engine's kernels");
+// One cell through the engine's cell-list kernels (srrmhd.cc:194-207): the
+// current comes from the cell's aux.
+void SRRMHD::sourceTermSingleCell(double *cons, double *prims, double *aux, double *source, int, int, int) {
+  check(mgpu_source_cells(engine(data), 1, cons, prims, aux, source), "SRRMHD::sourceTermSingleCell");
 }
 
 // The stiff source needs the current J, formed from cons, prims and sigma.
@@ -51,9 +53,14 @@ void SRRMHD::sourceTerm(double *cons, double *prims, double *aux, double *source
   data->gpu.deviceCurrent = false;
 }
 
-void SRRMHD::getPrimitiveVarsSingleCell(double *, double *, double *, int, int, int) {
-  throw std::logic_error(
-      "SRRMHD::getPrimitiveVarsSingleCell: single-cell model calls run inside the engine's kernels");
+// One cell through the engine's cell-list kernels (srrmhd.cc:256-312): the
+// secant starts from the cell's aux rhohWsq; a cell that does not converge
+// is reported and throws as the reference's secant does (srrmhd.cc:574-579).
+void SRRMHD::getPrimitiveVarsSingleCell(double *cons, double *prims, double *aux, int i, int j, int k) {
+  if (check(mgpu_c2p_cells(engine(data), 1, cons, prims, aux, nullptr), "SRRMHD::getPrimitiveVarsSingleCell") > 0) {
+    printf("Original C2P could not converge in cell (%d, %d, %d)\n", i, j, k);
+    throw std::runtime_error("C2P could not converge.\n");
+  }
 }
 
 void SRRMHD::getPrimitiveVars(double *cons, double *prims, double *aux) {

@@ -172,6 +172,13 @@ class Oracle:
         self.L.oc_flux_vector(ctypes.byref(self.c), _p(cons), _p(prims), _p(aux), _p(f), dir)
         return f
 
+    def flux_reconstruction(self, cons, f, dir: int):
+        """FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56) of padded cons and f."""
+        out = np.zeros_like(np.ascontiguousarray(cons))
+        self.L.oc_flux_reconstruction(ctypes.byref(self.c), _p(np.ascontiguousarray(cons)),
+                                      _p(np.ascontiguousarray(f)), _p(out), dir)
+        return out
+
     def fnet(self, cons, prims, aux):
         out = np.zeros_like(cons)
         self.L.oc_fvs_F(ctypes.byref(self.c), _p(cons), _p(prims), _p(aux), _p(out))

@@ -12,6 +12,9 @@
 //                on the initial state; dumps fnet.
 //   mode=regime: Simulation::set then one REGIME::sourceExtension
 //                (REGIME.cc:83-138) of the initial state; dumps it.
+//   mode=frecon: Simulation::set then, per axis, Model::fluxVector and
+//                FVS::fluxReconstruction (fluxVectorSplitting.cc:4-56) of the
+//                set-up state; dumps f and frecon of each axis.
 //   mode=c2p   : reads cons, prims and aux (the C2P guess) and calls
 //                Model::getPrimitiveVars once (srmhd.cc:390-562,
 //                srrmhd.cc:314-356); dumps prims and aux.
@@ -331,6 +334,21 @@ int main(int argc, char *argv[]) {
     FILE *f = fopen((out + "_regime.bin").c_str(), "wb");
     write_doubles(f, data.sourceExtension, ntot * data.Ncons);
     fclose(f);
+    dump_state(out + "_step0.bin", &data);
+    return 0;
+  }
+
+  if (mode == "frecon") {
+    // the calls of test_fvs.cc:56-170: fluxVector then fluxReconstruction
+    std::vector<double> frecon(ntot * data.Ncons, 0.0);
+    for (int dir = 0; dir < 3; dir++) {
+      model->fluxVector(data.cons, data.prims, data.aux, data.f, dir);
+      fvs.fluxReconstruction(data.cons, data.prims, data.aux, data.f, frecon.data(), dir);
+      FILE *f = fopen((out + "_frecon" + std::to_string(dir) + ".bin").c_str(), "wb");
+      write_doubles(f, data.f, ntot * data.Ncons);
+      write_doubles(f, frecon.data(), frecon.size());
+      fclose(f);
+    }
     dump_state(out + "_step0.bin", &data);
     return 0;
   }

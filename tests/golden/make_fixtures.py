@@ -346,7 +346,8 @@ def reference_test_cases():
               evolve_steps(gk, 0.5, 0.05), 4.0 / 3.0, 0.5, sigma=0.0, cp=0.1)
 
 
-FULL_PLANE_VARS = ("D", "Sx", "tau", "By")  # cons fields kept on the full-size plane
+FULL_PLANE_VARS = ("D", "Sx", "tau", "By")  # cons fields kept on the REGIME workload's plane
+FULL_STEPS = 10  # reference steps of the full-size fixtures
 
 
 def volume_stats(a):
@@ -358,12 +359,12 @@ def volume_stats(a):
     return out
 
 
-def full_size_case():
+def full_size_case(steps=FULL_STEPS):
     """KH 3-D at its benchmark size (256^3, WENO5, RK2, cfl 0.4, periodic x,z /
-    outflow y, seed 1234: the state bench.py times), 2 steps of the reference
-    build. The 4.4 GB result is kept as the dts, the x-plane through the middle
-    of four cons fields, and every cons and prim field's min, max, mean and L2
-    norm over the interior."""
+    outflow y, seed 1234: the state bench.py times), `steps` steps of the
+    reference build. The 4.4 GB result is kept as the dts, the x-plane through
+    the middle of every cons and prim field, and every cons and prim field's
+    min, max, mean and L2 norm over the interior."""
     g = idt.kh3d_grid(256)
     prims0 = idt.kh_prims(g, seed=1234)
     bc = ("periodic", "outflow", "periodic")
@@ -373,33 +374,33 @@ def full_size_case():
         del prims0
         base = _grid_args(g, 4.0 / 3.0, 0.4, 5, "rk2", bc, "srmhd")
         out = os.path.join(td, "out")
-        _run(base + ["mode=run", "steps=2", f"prims_in={pin}", f"out={out}", "dump=2"], td)
+        _run(base + ["mode=run", f"steps={steps}", f"prims_in={pin}", f"out={out}", f"dump={steps}"], td)
         log = open(out + "_log.txt").read().split("\n")
         dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
         shape = g.shape
         n = shape[0] * shape[1] * shape[2]
-        raw = np.memmap(f"{out}_step2.bin", dtype=np.float64, mode="r")
+        raw = np.memmap(f"{out}_step{steps}.bin", dtype=np.float64, mode="r")
         nc, np_, _ = NV["srmhd"]
         inner = (slice(g.Ng, shape[0] - g.Ng), slice(g.Ng, shape[1] - g.Ng), slice(g.Ng, shape[2] - g.Ng))
         cons = np.array(raw[: nc * n].reshape((nc,) + shape)[(slice(None),) + inner])
         cs = volume_stats(cons)
-        names = ["D", "Sx", "Sy", "Sz", "tau", "Bx", "By", "Bz", "phi"]
-        plane = np.stack([cons[names.index(v), 128] for v in FULL_PLANE_VARS])
+        cplane = cons[:, 128].copy()
         del cons
         prims = np.array(raw[nc * n:(nc + np_) * n].reshape((np_,) + shape)[(slice(None),) + inner])
         ps = volume_stats(prims)
+        pplane = prims[:, 128].copy()
         del prims, raw
     path = os.path.join(HERE, "kh3d_256_weno5_rk2_full.npz")
-    np.savez_compressed(path, dts=np.array(dts), cons_stats=cs, prims_stats=ps, plane_x128=plane,
-                        plane_vars=np.array(FULL_PLANE_VARS))
+    np.savez_compressed(path, steps=np.array(steps), dts=np.array(dts), cons_stats=cs, prims_stats=ps,
+                        cons_plane_x128=cplane, prims_plane_x128=pplane)
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
 
 
-def full_size_resistive_case():
+def full_size_resistive_case(steps=FULL_STEPS):
     """The resistive workload at its own size (Orszag-Tang 1024^2, sigma 1e3,
-    SSP2, WENO3, cfl 0.5, periodic: the state bench.py times), 2 steps of the
-    reference build: the dts, the middle row along each axis of every cons
-    field, and every cons and prim field's min, max, mean and L2 norm."""
+    SSP2, WENO3, cfl 0.5, periodic: the state bench.py times), `steps` steps of
+    the reference build: the dts, the middle row along each axis of every cons
+    and prim field, and every cons and prim field's min, max, mean and L2 norm."""
     g = idt.ot2d_grid(1024)
     prims0 = idt.ot_prims_resistive(g)
     per = ("periodic", "periodic", "periodic")
@@ -408,14 +409,16 @@ def full_size_resistive_case():
         prims0.tofile(pin)
         base = _grid_args(g, 4.0 / 3.0, 0.5, 3, "ssp2", per, "srrmhd", sigma=1e3, cp=0.1)
         out = os.path.join(td, "out")
-        _run(base + ["mode=run", "steps=2", f"prims_in={pin}", f"out={out}", "dump=2"], td)
+        _run(base + ["mode=run", f"steps={steps}", f"prims_in={pin}", f"out={out}", f"dump={steps}"], td)
         log = open(out + "_log.txt").read().split("\n")
         dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
-        c, p, _ = _load_state(f"{out}_step2.bin", "srrmhd", g.shape)
+        c, p, _ = _load_state(f"{out}_step{steps}.bin", "srrmhd", g.shape)
         inner = (slice(None), slice(g.Ng, -g.Ng), slice(g.Ng, -g.Ng), slice(None))
         c, p = c[inner], p[inner]
-        rec = {"dts": np.array(dts), "cons_stats": volume_stats(c), "prims_stats": volume_stats(p),
-               "row_x512": c[:, 512, :, 0].copy(), "row_y512": c[:, :, 512, 0].copy()}
+        rec = {"steps": np.array(steps), "dts": np.array(dts), "cons_stats": volume_stats(c),
+               "prims_stats": volume_stats(p),
+               "row_x512": c[:, 512, :, 0].copy(), "row_y512": c[:, :, 512, 0].copy(),
+               "prow_x512": p[:, 512, :, 0].copy(), "prow_y512": p[:, :, 512, 0].copy()}
     path = os.path.join(HERE, "ot2d_1024_srrmhd_ssp2_full.npz")
     np.savez_compressed(path, **rec)
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
@@ -489,7 +492,87 @@ def batch_member_case():
     print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
 
 
-GROUPS = {"regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases, "batch_member": batch_member_case,
+def frecon_case(name, g, init_args, weno, bc, gamma, cfl):
+    """FVS::fluxReconstruction of a reference set-up state along every axis,
+    as Tests/CPU/Src/test_fvs.cc:56-170 calls it after Model::fluxVector: the
+    post-set prims and cons, and per axis the flux vector f and frecon, over
+    the whole padded grid."""
+    with tempfile.TemporaryDirectory() as td:
+        base = _grid_args(g, gamma, cfl, weno, "rk2", bc, "srmhd")
+        out = os.path.join(td, "out")
+        _run(base + ["mode=frecon", f"out={out}"] + init_args, td)
+        c, p, _ = _load_state(f"{out}_step0.bin", "srmhd", g.shape)
+        n = c.size
+        rec = {"prims0": p, "cons0": c}
+        for d in range(3):
+            raw = np.fromfile(f"{out}_frecon{d}.bin")
+            rec[f"f{d}"] = raw[:n].reshape(c.shape)
+            rec[f"frecon{d}"] = raw[n:].reshape(c.shape)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **rec)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
+def frecon_cases():
+    # test_fvs.cc's set-up (OTVortexSingleFluid, SRMHD, Weno3, Outflow, the
+    # Data defaults gamma 5/3 and cfl 0.5) on a small ragged 3-D grid with Ng 3
+    g = idt.Grid(10, 12, 8, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, Ng=3)
+    frecon_case("frecon_ot3d_weno3", g, ["init=ot"], 3, ("outflow", "outflow", "outflow"), 5.0 / 3.0, 0.5)
+    # WENO5 on the reference's KH example state, 2-D, Flow
+    g2 = idt.Grid(16, 12, 0, -0.5, 0.5, -1.0, 1.0)
+    frecon_case("frecon_kh2d_weno5", g2, ["init=kh", "mag=1"], 5, ("periodic", "outflow", "periodic"), 4.0 / 3.0,
+                0.4)
+
+
+def srrmhd3d_prims(g: idt.Grid, bc):
+    """The resistive Orszag-Tang profile in x-y with vz = 0.05 sin 2 pi z and
+    Bz = 0.1 added and E = -v x B of the whole 3-D state (the ideal field, as
+    the 2-D profile has), ghosts holding the images the boundaries give."""
+    from oracle import oracle as orc
+
+    p = idt.ot_prims_resistive(g)
+    _, _, z = g.centres()
+    p[3] = 0.05 * np.sin(2 * np.pi * z)[None, None, :]
+    p[7] = 0.1
+    v, B = p[1:4], p[5:8]
+    p[8:11] = -np.cross(v, B, axis=0)
+    periodic = tuple(1 if b == "periodic" else 0 for b in bc)
+    o = orc.Oracle(orc.Case(nx=g.nx, ny=g.ny, nz=g.nz, xmin=g.xmin, xmax=g.xmax, ymin=g.ymin, ymax=g.ymax,
+                            zmin=g.zmin, zmax=g.zmax, Ng=g.Ng, model=orc.SRRMHD, periodic=periodic))
+    p = np.ascontiguousarray(p)
+    o.apply_bcs(p)
+    return p
+
+
+def srrmhd3d_cases():
+    """3-D resistive runs, so the z-direction flux of srrmhd.cc:163-171 (Sy
+    written twice, Sz left from the y call) is checked against the reference
+    itself: SSP2 (sigma 50) and RKSplit (sigma 10), periodic and Flow, on a
+    ragged 12 x 14 x 10 grid, 3 steps. Interior cells are stored."""
+    g = idt.Grid(12, 14, 10, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    per, flow = ("periodic",) * 3, ("periodic", "outflow", "outflow")
+    for integ, sigma, weno in (("ssp2", 50.0, 3), ("rksplit", 10.0, 5)):
+        for bc, bname in ((per, "periodic"), (flow, "flow")):
+            prims0 = srrmhd3d_prims(g, bc)
+            name = f"srrmhd3d_12x14x10_{integ}_{bname}"
+            with tempfile.TemporaryDirectory() as td:
+                pin = os.path.join(td, "prims.bin")
+                prims0.tofile(pin)
+                base = _grid_args(g, 4.0 / 3.0, 0.5, weno, integ, bc, "srrmhd", sigma=sigma, cp=0.1)
+                out = os.path.join(td, "out")
+                _run(base + ["mode=run", "steps=3", f"prims_in={pin}", f"out={out}", "dump=3"], td)
+                log = open(out + "_log.txt").read().split("\n")
+                dts = [float(l.split()[5]) for l in log if l.startswith("step ")]
+                c, p, _ = _load_state(f"{out}_step3.bin", "srrmhd", g.shape)
+            inner = (slice(None), slice(g.Ng, -g.Ng), slice(g.Ng, -g.Ng), slice(g.Ng, -g.Ng))
+            rec = {"prims0": prims0[inner], "cons3": c[inner], "prims3": p[inner], "dts": np.array(dts),
+                   "sigma": np.array(sigma), "weno": np.array(weno), "periodic": np.array([b == "periodic" for b in bc])}
+            path = os.path.join(HERE, f"{name}.npz")
+            np.savez_compressed(path, **rec)
+            print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB), dts {dts}")
+
+
+GROUPS = {"frecon": frecon_cases, "srrmhd3d": srrmhd3d_cases, "regime_rkplus": regime_rkplus_cases, "reference_tests": reference_test_cases, "batch_member": batch_member_case,
           "full_size": full_size_case, "full_size_resistive": full_size_resistive_case,
           "large_regime": large_regime_case}
 
@@ -531,6 +614,8 @@ def main():
     full_size_resistive_case()
     large_regime_case()
     batch_member_case()
+    frecon_cases()
+    srrmhd3d_cases()
 
 
 if __name__ == "__main__":

@@ -22,8 +22,9 @@ def interior(a, Ng, dims):
     return a[tuple(sl)]
 
 
-def field_errors(got, ref, names):
-    """Rows of (name, max|diff|, max|ref|, diff/max|ref|, diff/group scale, max pointwise rel)."""
+def group_scales(ref, names):
+    """(field max |ref|, group scale) per field name: a field's group scale is
+    the largest max |ref| over the vector group it belongs to."""
     scales = {n: float(np.abs(ref[i]).max()) for i, n in enumerate(names)}
     gscale = dict(scales)
     for grp in _GROUPS:
@@ -31,6 +32,12 @@ def field_errors(got, ref, names):
         for n in grp:
             if n in gscale:
                 gscale[n] = m
+    return scales, gscale
+
+
+def field_errors(got, ref, names):
+    """Rows of (name, max|diff|, max|ref|, diff/max|ref|, diff/group scale, max pointwise rel)."""
+    scales, gscale = group_scales(ref, names)
     rows = []
     for i, n in enumerate(names):
         d = np.abs(got[i] - ref[i])
@@ -42,6 +49,18 @@ def field_errors(got, ref, names):
             pw = np.where(np.abs(ref[i]) > 0, d / np.abs(ref[i]), np.where(d == 0, 0.0, np.inf))
         rows.append((n, dm, fm, rel_f, rel_g, float(pw.max())))
     return rows
+
+
+def pointwise_floor(got, ref, names, floor_frac):
+    """Per field, max |got - ref| / max(|ref|, floor_frac * group scale): the
+    pointwise relative error, with values below the floor measured against it
+    (near a zero crossing a pure pointwise ratio says nothing about parity)."""
+    _, gscale = group_scales(ref, names)
+    out = []
+    for i, n in enumerate(names):
+        den = np.maximum(np.abs(ref[i]), max(floor_frac * gscale[n], 1e-300))
+        out.append(float((np.abs(got[i] - ref[i]) / den).max()))
+    return out
 
 
 def report(title, rows):
@@ -74,4 +93,42 @@ def c2p_retry_case(oracle):
     for i, j in C2P_RETRY_CELLS:
         gp[1, i, j, 0] = 1.0
         gp[2, i, j, 0] = 0.0
+    return cons, prims, aux, gp, ga
+
+
+# The 3-D variant: KH 12^3 with four bad guesses, two of them face neighbours
+# along z (so each is the other's failed neighbour, which the retry skips)
+# and one next to the z edge of the interior.
+C2P_RETRY_GRID_3D = dict(nx=12, ny=12, nz=12, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0)
+C2P_RETRY_CELLS_3D = ((6, 7, 7), (6, 7, 8), (11, 9, 5), (9, 13, 15))
+
+
+def c2p_retry_case_3d(oracle):
+    from method_amd import initdata as idt
+
+    p0 = idt.kh_prims(idt.kh3d_grid(12), seed=6, shear=0.7, bz=0.5)
+    cons, prims, aux = oracle.set(p0)
+    gp, ga = prims.copy(), aux.copy()
+    for i, j, k in C2P_RETRY_CELLS_3D:
+        gp[1, i, j, k] = 0.0
+        gp[3, i, j, k] = 1.0
+        gp[2, i, j, k] = 0.0
+    return cons, prims, aux, gp, ga
+
+
+def c2p_step_retry_case(oracle):
+    """The 16 x 16 KH state of c2p_retry_case, consistent, except that at the
+    three cells the stored density is the subnormal 1e-310 and the stored h is
+    infinite: the flux (which reads v and p) is untouched, but the starting
+    root Z = rho h / (1 - v^2) (srmhd.cc:438-442, h from p / rho or from the
+    aux) is infinite, so the cons2prim of both RK2 stages fails there (the
+    stage-two guess is the step's input prims, RK2.cc:81-91) and retries."""
+    from method_amd import initdata as idt
+
+    p0 = idt.kh_prims(idt.kh2d_grid(16), seed=5, shear=0.7, bz=0.5)
+    cons, prims, aux = oracle.set(p0)
+    gp, ga = prims.copy(), aux.copy()
+    for i, j in C2P_RETRY_CELLS:
+        gp[0, i, j, 0] = 1e-310
+        ga[0, i, j, 0] = float("inf")
     return cons, prims, aux, gp, ga

@@ -205,3 +205,40 @@ def test_member_at_real_size_matches_reference(golden):
             diff = np.abs(got - ref[v])
             assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
             assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name
+
+
+def test_batch_retry_keeps_root_for_the_aux():
+    """The batch's retry (run by the last workgroup of each stage launch), on a
+    member whose step fails in both stages at three cells: the stage-two retry
+    keeps the root, and the aux formed from it on download, with the prims and
+    cons, must match the oracle's step; the other member is untouched by it.
+    Without the last-cell quirk, for the reason test_gpu_srmhd's step test
+    gives."""
+    quirks = 0
+    say(f"gpu: batch step with failing cells, quirks={quirks}")
+    g = parity.C2P_RETRY_GRID
+    o = orc.Oracle(orc.Case(weno_order=5, periodic=(1, 0, 1), quirks=quirks, **g))
+    cons, prims, aux, gp, ga = parity.c2p_step_retry_case(o)
+    dt = o.timestep(0.0, 0)
+    oc, op, oa = cons.copy(), gp.copy(), ga.copy()
+    _, ost = o.step(oc, op, oa, dt, "rk2")
+    other = SMALL[0]
+    inst = [eng.Instance(g["nx"], g["ny"]), eng.Instance(other[0], other[1])]
+    with eng.Batch(inst, weno_order=5, bc=BC, quirks=quirks) as b:
+        b.upload(0, cons, gp, ga)
+        b.set_prims(1, prims_of(*other))
+        rc, st = b.step(1, stats=True)
+        assert b.time(0)[1] == dt
+        c, p, a = b.download(0)
+        c1, p1, a1 = b.download(1)
+    say(f"  batch failures {st['failures']} retry failures {st['fallbacks']}, oracle failures {ost.failures}")
+    assert st["failures"] == ost.failures == 2 * len(parity.C2P_RETRY_CELLS)
+    assert st["fallbacks"] == 0
+    for got, ref, names in ((c, oc, parity.SRMHD_CONS), (p, op, parity.SRMHD_PRIMS), (a, oa, parity.SRMHD_AUX)):
+        rows = parity.field_errors(parity.interior(got, 4, 2), parity.interior(ref, 4, 2), names)
+        say(parity.report(f"batch step with retries, quirks={quirks}", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-10, f"{n}: {rg:.3e}"
+    (sc, sp, sa), _ = single_run(other, 1, 5)
+    for x, y in ((c1, sc), (p1, sp), (a1, sa)):
+        assert np.array_equal(x, y)

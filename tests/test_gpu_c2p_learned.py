@@ -110,3 +110,51 @@ def test_learned_guess_gives_the_same_roots_with_fewer_steps_than_cold(trained):
     it = {m: out[m][1]["iterations"] / out[m][1]["cells"] for m in out}
     assert it[eng.GUESS_LEARNED] < it[eng.GUESS_COLD]
     assert out[eng.GUESS_LEARNED][1]["fallbacks"] <= 1e-3 * out[eng.GUESS_LEARNED][1]["cells"]
+
+
+def _p99(hist):
+    c = np.cumsum(hist)
+    return int(np.searchsorted(c, 0.99 * c[-1]))
+
+
+def test_config3_at_its_own_size(trained):
+    """The learned-guess workload at its own size: SRMHD cons2prim of every
+    cell of the 256^3 KH state bench.py times (seed 1234), the guess being the
+    state one step older, with the plain (srmhd.cc:438-442), learned and cold
+    starts. Every cell converges; the learned and cold roots equal the plain
+    ones to 1e-10 of each field's scale; fallbacks stay within 1e-3 of the
+    cells. Mean and p99 Newton steps per cell are reported from the kernel's
+    step histogram."""
+    p, n, _ = trained
+    g = idt.kh3d_grid(256)
+    cfg = eng.Config(nx=256, ny=256, nz=256, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                     weno_order=5)
+    out = {}
+    with eng.Engine(cfg) as e:
+        e.set_prims(idt.kh_prims(g, seed=1234))
+        e.update_time()
+        _, p_prev, _ = e.download()
+        e.update_time()
+        for mode, name in ((eng.GUESS_PLAIN, "plain"), (eng.GUESS_LEARNED, "learned"), (eng.GUESS_COLD, "cold")):
+            e.upload(prims=p_prev)
+            e.set_c2p_guess(mode, p if mode == eng.GUESS_LEARNED else None, n if mode == eng.GUESS_LEARNED else None)
+            e.set_c2p_histogram(True)
+            rc, st = e.c2p()
+            h = e.c2p_histogram()
+            e.set_c2p_histogram(False)
+            _, prims, _ = e.download()
+            assert h.sum() == st["cells"] == 256 ** 3
+            mean = st["iterations"] / st["cells"]
+            assert abs(mean - np.dot(np.arange(64), h) / h.sum()) < 1e-9
+            say(f"{name}: mean {mean:.3f} Newton steps/cell, p99 {_p99(h)}, max {st['max_iterations']}, "
+                f"fallbacks {st['fallbacks']}, failures {st['failures']}")
+            assert rc == 0 and st["failures"] == 0 and h[63] == 0
+            out[name] = (prims[:, 4:-4, 4:-4, 4:-4], st)
+    ref = out["plain"][0]
+    scale = np.abs(ref).reshape(8, -1).max(axis=1)
+    for name in ("learned", "cold"):
+        d = np.abs(out[name][0] - ref).reshape(8, -1).max(axis=1) / np.maximum(scale, 1e-300)
+        say(f"{name}: max rel diff to plain by field {d.max():.2e}")
+        assert d.max() <= 1e-10
+    assert out["learned"][1]["fallbacks"] <= 1e-3 * out["learned"][1]["cells"]
+    assert out["learned"][1]["iterations"] < out["cold"][1]["iterations"]

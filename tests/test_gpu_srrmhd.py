@@ -210,27 +210,35 @@ def _group_scales(ref, fields):
     return scale, norm
 
 
-def test_full_size_resistive_two_steps_match_reference(golden):
+def test_full_size_resistive_ten_steps_match_reference(golden):
     """The resistive workload at its own size (Orszag-Tang 1024^2, sigma 1e3,
-    SSP2, WENO3, the state bench.py times), two steps against the reference
-    build: dt bit for bit, the middle row along each axis of every cons field
-    to 1e-9 of the field's group scale, and every cons and prim field's min,
-    max, mean and L2 norm over the 1 M interior cells to 1e-9."""
+    SSP2, WENO3, the state bench.py times), ten steps against the reference
+    build: dt bit for bit; the middle row along each axis of every cons and
+    prim field to 1e-9 of the field's group scale, and pointwise to 1e-8
+    relative wherever |value| is at least 1e-3 of that scale; and every cons
+    and prim field's min, max, mean and L2 norm over the 1 M interior cells to
+    1e-9."""
     z = golden("ot2d_1024_srrmhd_ssp2_full")
+    steps = int(z["steps"])
     g = idt.ot2d_grid(1024)
-    say("gpu: resistive OT 1024^2 SSP2, 2 steps against the reference build")
+    say(f"gpu: resistive OT 1024^2 SSP2, {steps} steps against the reference build")
     with engine(nx=1024, ny=1024, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0) as e:  # RES: sigma 1e3, cp 0.1, cfl 0.5
         e.set_prims(idt.ot_prims_resistive(g))
-        for s in range(2):
+        for s in range(steps):
             assert e.update_time() == 0
             assert e.time()[1] == z["dts"][s]
         c, p, _ = e.download()
     c, p = c[:, 4:-4, 4:-4, :], p[:, 4:-4, 4:-4, :]
-    for got, ref, what in ((c[:, 512, :, 0], z["row_x512"], "row x=512"), (c[:, :, 512, 0], z["row_y512"], "row y=512")):
-        rows = parity.field_errors(got, ref, parity.SRRMHD_CONS)
+    for got, ref, names, what in ((c[:, 512, :, 0], z["row_x512"], parity.SRRMHD_CONS, "cons row x=512"),
+                                  (c[:, :, 512, 0], z["row_y512"], parity.SRRMHD_CONS, "cons row y=512"),
+                                  (p[:, 512, :, 0], z["prow_x512"], parity.SRRMHD_PRIMS, "prims row x=512"),
+                                  (p[:, :, 512, 0], z["prow_y512"], parity.SRRMHD_PRIMS, "prims row y=512")):
+        rows = parity.field_errors(got, ref, names)
         say(parity.report(f"gpu vs reference, {what}", rows))
-        for n, dm, fm, rf, rg, pw in rows:
+        pw = parity.pointwise_floor(got, ref, names, 1e-3)
+        for (n, dm, fm, rf, rg, _), e_pw in zip(rows, pw):
             assert rg <= 1e-9, f"{what} {n}: {rg:.3e}"
+            assert e_pw <= 1e-8, f"{what} {n}: pointwise {e_pw:.3e}"
     for got, ref, fields in ((_volume_stats(c), z["cons_stats"], parity.SRRMHD_CONS),
                              (_volume_stats(p), z["prims_stats"], parity.SRRMHD_PRIMS)):
         scale, norm = _group_scales(ref, fields)
@@ -239,3 +247,32 @@ def test_full_size_resistive_two_steps_match_reference(golden):
             say(f"  {name}: min/max/mean/L2 differences {diff} (scale {scale[name]:.3e})")
             assert np.all(diff[:3] <= 1e-9 * max(scale[name], 1e-300)), name
             assert diff[3] <= 1e-9 * max(norm[name], 1e-300), name
+
+
+@pytest.mark.parametrize("name", ["srrmhd3d_12x14x10_ssp2_periodic", "srrmhd3d_12x14x10_ssp2_flow",
+                                  "srrmhd3d_12x14x10_rksplit_periodic", "srrmhd3d_12x14x10_rksplit_flow"])
+def test_3d_matches_reference(golden, name):
+    """3-D resistive runs against the reference build itself (SSP2 sigma 50,
+    RKSplit sigma 10; periodic and Flow; ragged 12 x 14 x 10; 3 steps): the
+    z-direction flux quirk of srrmhd.cc:163-171 and the 3-D stencils to 1e-9
+    of each field's group scale, dt bit for bit."""
+    z = golden(name)
+    grid = dict(nx=12, ny=14, nz=10, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    integ = eng.SSP2 if "ssp2" in name else eng.RKSPLIT
+    bc = tuple(eng.PERIODIC if b else eng.OUTFLOW for b in z["periodic"])
+    say(f"gpu: SRRMHD 3-D {name} vs reference")
+    p0 = np.zeros((11, 20, 22, 18))
+    p0[:, 4:-4, 4:-4, 4:-4] = z["prims0"]
+    cfg = eng.Config(model=eng.SRRMHD, integrator=integ, weno_order=int(z["weno"]), bc=bc, quirks=1,
+                     sigma=float(z["sigma"]), cp=0.1, cfl=0.5, **grid)
+    with eng.Engine(cfg) as e:
+        e.set_prims(p0)
+        for s in range(3):
+            rc, st = e.update_time(stats=True)
+            assert rc == 0 and st["failures"] == 0
+            assert e.time()[1] == z["dts"][s]
+        c, p, _ = e.download()
+    for got, ref, names in ((c, z["cons3"], parity.SRRMHD_CONS), (p, z["prims3"], parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 3), ref, names)
+        say(parity.report(f"gpu vs reference, {name}", rows))
+        check_rows(rows, 1e-9)

@@ -691,3 +691,71 @@ def test_regime_sheet_rotations_agree():
     b = np.transpose(cb[7, 4:-4, 4:-4, 4:-4], (1, 2, 0))   # dB[7, k, i, j] at (i, j, k)
     c = np.transpose(cc[5, 4:-4, 4:-4, 4:-4], (2, 0, 1))   # dC[5, j, k, i] at (i, j, k)
     assert np.abs(a - b).max() <= 1e-15 and np.abs(a - c).max() <= 1e-15
+
+
+# ------------------------------------------- FVS::fluxReconstruction dumps
+@pytest.mark.parametrize("name, grid, order, periodic", [
+    ("frecon_ot3d_weno3", dict(nx=10, ny=12, nz=8, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0, Ng=3,
+                               gamma=5.0 / 3.0, cfl=0.5), 3, (0, 0, 0)),
+    ("frecon_kh2d_weno5", dict(nx=16, ny=12, nz=0, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0), 5, (1, 0, 1)),
+])
+def test_flux_reconstruction_matches_reference(golden, name, grid, order, periodic):
+    """The oracle's FVS::fluxReconstruction of the reference's own f and cons
+    equals the reference's frecon bit for bit on every axis (test_fvs.cc's
+    calls), including the zeros outside [shift, N - shift)."""
+    z = golden(name)
+    o = orc.Oracle(orc.Case(weno_order=order, periodic=periodic, **grid))
+    for d in range(3):
+        got = o.flux_reconstruction(z["cons0"], z[f"f{d}"], d)
+        assert np.array_equal(got, z[f"frecon{d}"]), d
+    # and the fixture's f is the oracle's flux vector of the post-set state
+    for d in range(3 if grid["nz"] else 2):
+        c, p, a = o.set(z["prims0"])
+        f = o.flux_vector(c, p, a, d)
+        assert np.allclose(f, z[f"f{d}"], rtol=0, atol=1e-14 * np.abs(z[f"f{d}"]).max()), d
+
+
+# ----------------------------------------------- 3-D resistive trajectories
+SRRMHD3D = dict(nx=12, ny=14, nz=10, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
+
+
+def srrmhd3d_case(z, name):
+    periodic = tuple(int(b) for b in z["periodic"])
+    return orc.Case(model=orc.SRRMHD, weno_order=int(z["weno"]), periodic=periodic, sigma=float(z["sigma"]), cp=0.1,
+                    cfl=0.5, **SRRMHD3D)
+
+
+def srrmhd3d_prims0(case, z):
+    """The fixture's interior primitives with ghosts holding their boundary images."""
+    _, p, _ = case.zeros()
+    p[:, 4:-4, 4:-4, 4:-4] = z["prims0"]
+    orc.Oracle(case).apply_bcs(p)
+    return p
+
+
+@pytest.mark.parametrize("name", ["srrmhd3d_12x14x10_ssp2_periodic", "srrmhd3d_12x14x10_ssp2_flow",
+                                  "srrmhd3d_12x14x10_rksplit_periodic", "srrmhd3d_12x14x10_rksplit_flow"])
+def test_srrmhd_3d_matches_reference(golden, name):
+    """3-D resistive runs of the reference build (SSP2 at sigma 50 and RKSplit
+    at sigma 10, periodic and Flow, 3 steps): the oracle's z-direction flux
+    quirk (srrmhd.cc:163-171) and its 3-D stencils agree with the reference to
+    1e-9 of each field's group scale, dt bit for bit. The same runs without
+    the quirk move away from the reference, so the quirk is exercised."""
+    z = golden(name)
+    case = srrmhd3d_case(z, name)
+    integ = "ssp2" if "ssp2" in name else "rksplit"
+    p0 = srrmhd3d_prims0(case, z)
+    c, p, a, dts, st = orc.Oracle(case).run(p0, 3, integrator=integ)
+    assert np.array_equal(np.array(dts), z["dts"])
+    worst = 0.0
+    for got, ref, names in ((c, z["cons3"], parity.SRRMHD_CONS), (p, z["prims3"], parity.SRRMHD_PRIMS)):
+        rows = parity.field_errors(parity.interior(got, 4, 3), ref, names)
+        print(parity.report(f"oracle vs reference, {name}", rows))
+        for n, dm, fm, rf, rg, pw in rows:
+            assert rg <= 1e-9, n
+            worst = max(worst, rg)
+    case0 = srrmhd3d_case(z, name)
+    case0.quirks = 0
+    c0, _, _, _, _ = orc.Oracle(case0).run(p0, 3, integrator=integ)
+    rows = parity.field_errors(parity.interior(c0, 4, 3), z["cons3"], parity.SRRMHD_CONS)
+    assert max(r[4] for r in rows) > 1e3 * max(worst, 1e-16)
