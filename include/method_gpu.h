@@ -150,6 +150,20 @@ int mgpu_flux_reconstruction(mgpu_engine *e, const double *cons, const double *f
 int mgpu_c2p_cells(mgpu_engine *e, long long n, const double *cons, double *prims, double *aux,
                    mgpu_stats *stats);
 
+/* No counterpart in the reference. mgpu_c2p_cells for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): cons, prims and aux are DEVICE arrays
+ * in the same single-cell layout, read and written on the engine's stream
+ * (order earlier writes to them before the call). The call allocates
+ * nothing: its per-block counters and their total live in a static device
+ * array of each GPU, sized for at most 4,194,304 cells per call (larger n is
+ * an error). Calls on one GPU share that array, so calls that overlap on one
+ * GPU (from several threads or streams) race. With stats NULL the call only
+ * launches and returns 0 (mgpu_sync before reading the results); otherwise
+ * it waits, fills stats and returns the failure count as mgpu_c2p_cells. */
+int mgpu_c2p_cells_noalloc(mgpu_engine *e, long long n, const double *cons, double *prims, double *aux,
+                           mgpu_stats *stats);
+
 /* Model::sourceTermSingleCell (srmhd.cc:212-223, srrmhd.cc:194-207;
  * model.h:55) for n independent cells in the single-cell layout. SRMHD reads
  * cons only; SRRMHD reads cons and the current J from aux[4..6]. prims is

@@ -14,6 +14,8 @@
 // buffer.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "method_gpu.h"
 #include "mgpu_kernels.hpp"
 
@@ -22,6 +24,18 @@ namespace mgpu {
 namespace {
 
 constexpr int kCellThreads = kCellBlock;
+
+// Counter records of the allocation-free cell-list cons2prim: one per block
+// of kNoAllocMaxCells cells, then the total. Every GPU holds its own copy for
+// the life of the process; the host reaches it only through the address that
+// noalloc_scratch_setup looked up on that GPU.
+constexpr long long kNoAllocRecords = kNoAllocMaxCells / kCellBlock + 1;
+static_assert(kNoAllocMaxCells % kCellBlock == 0, "whole blocks");
+static __device__ __align__(256) CellCounters scratch[kNoAllocRecords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+CellCounters *g_scratch_ptr[kMaxDevices] = {};
 
 // Sum over a block of one cell counter set; thread 0 gets the totals.
 __device__ __forceinline__ void block_counts(int &cells, int &iters, int &fails, int &maxit) {
@@ -238,6 +252,24 @@ void launch_flux_recon_o(const PadGeo &pg, int dir, int nvars, const double *con
 }  // namespace
 
 long long cells_blocks(long long n) { return (n + kCellThreads - 1) / kCellThreads; }
+
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return 0;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return -1;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+  g_scratch_ptr[device] = static_cast<CellCounters *>(p);
+  return 0;
+}
+
+CellCounters *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_ptr[device];
+}
 
 void launch_flux_recon(const PadGeo &pg, int order, int dir, int nvars, const double *cons, const double *f,
                        double *frecon, hipStream_t s) {

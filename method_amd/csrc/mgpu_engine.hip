@@ -594,6 +594,9 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   if (c.nx < c.Ng || (c.ny > 0 && c.ny < c.Ng) || (c.nz > 0 && c.nz < c.Ng))
     return fail("mgpu_create: every active axis needs at least Ng interior cells");
   HIP_OK(hipSetDevice(c.device));
+  // the allocation-free cell-list cons2prim's scratch address on this GPU
+  if (noalloc_scratch_setup(c.device))
+    return fail("mgpu_create: hipGetSymbolAddress of the cell-list scratch failed on device %d", c.device);
 
   mgpu_engine *e = new mgpu_engine();
   e->cfg = c;
@@ -991,6 +994,31 @@ int mgpu_c2p_cells(mgpu_engine *e, long long n, const double *cons, double *prim
     stats->failures = (long long)t.failures;
     stats->max_iterations = (long long)t.max_iterations;
   }
+  return t.failures > 0 ? (int)std::min<unsigned long long>(t.failures, 1ULL << 30) : 0;
+}
+
+int mgpu_c2p_cells_noalloc(mgpu_engine *e, long long n, const double *cons, double *prims, double *aux,
+                           mgpu_stats *stats) {
+  if (!e) return fail("null engine");
+  if (n < 0) return fail("mgpu_c2p_cells_noalloc: negative cell count");
+  if (n > kNoAllocMaxCells)
+    return fail("mgpu_c2p_cells_noalloc: %lld cells exceed the maximum of %lld per call", n, kNoAllocMaxCells);
+  if (n > 0 && (!cons || !prims || !aux)) return fail("mgpu_c2p_cells_noalloc: null array");
+  CellCounters *part = noalloc_scratch(e->cfg.device);
+  if (!part) return fail("mgpu_c2p_cells_noalloc: no scratch address for device %d", e->cfg.device);
+  // every record the launch reads is written by it first: nothing to clear
+  CellCounters *total = part + cells_blocks(n);
+  launch_c2p_cells(e->cfg.model, n, cons, prims, aux, e->cfg.gamma, e->cfg.sigma, part, total, e->stream);
+  HIP_OK(hipGetLastError());
+  if (!stats) return 0;
+  CellCounters t;
+  HIP_OK(hipMemcpyAsync(&t, total, sizeof t, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  memset(stats, 0, sizeof *stats);
+  stats->cells = (long long)t.cells;
+  stats->iterations = (long long)t.iterations;
+  stats->failures = (long long)t.failures;
+  stats->max_iterations = (long long)t.max_iterations;
   return t.failures > 0 ? (int)std::min<unsigned long long>(t.failures, 1ULL << 30) : 0;
 }
 

@@ -170,6 +170,14 @@ long long cells_blocks(long long n);
 // out. partial receives cells_blocks(n) records, total their sum.
 void launch_c2p_cells(int model, long long n, const double *cons, double *prims, double *aux, double gam,
                       double sigma, CellCounters *partial, CellCounters *total, hipStream_t s);
+// The allocation-free cell-list cons2prim keeps its per-block records and
+// their total in a static device array sized for this many cells per call.
+constexpr long long kNoAllocMaxCells = 1LL << 22;
+// noalloc_scratch_setup looks up that array's address on `device` (which
+// must be current) once per device and returns nonzero on failure;
+// noalloc_scratch returns the address found, or null before set-up.
+int noalloc_scratch_setup(int device);
+CellCounters *noalloc_scratch(int device);
 // Model::sourceTermSingleCell for n cells ([cell][var]).
 void launch_source_cells(int model, long long n, const double *cons, const double *aux, double cp, double *src,
                          hipStream_t s);

@@ -28,7 +28,7 @@ OUTFLOW, PERIODIC = 0, 1
 EXPORTED = [
     "mgpu_last_error", "mgpu_create", "mgpu_destroy", "mgpu_sizes", "mgpu_upload_state", "mgpu_download_state",
     "mgpu_prims_to_all", "mgpu_step", "mgpu_update_time", "mgpu_get_time", "mgpu_set_time", "mgpu_fnet",
-    "mgpu_flux_vector", "mgpu_source", "mgpu_flux_reconstruction", "mgpu_c2p_cells", "mgpu_source_cells", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_set_c2p_histogram", "mgpu_get_c2p_histogram", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_set_regime_plan", "mgpu_sync", "mgpu_timed_steps",
+    "mgpu_flux_vector", "mgpu_source", "mgpu_flux_reconstruction", "mgpu_c2p_cells", "mgpu_c2p_cells_noalloc", "mgpu_source_cells", "mgpu_set_model_extension", "mgpu_source_extension", "mgpu_regime_fields", "mgpu_c2p", "mgpu_max_wavespeed", "mgpu_set_c2p_guess", "mgpu_set_c2p_histogram", "mgpu_get_c2p_histogram", "mgpu_timed_c2p", "mgpu_set_sweep_plan", "mgpu_set_regime_plan", "mgpu_sync", "mgpu_timed_steps",
     "mgpu_kernel_slot_name", "mgpu_mlp_create", "mgpu_mlp_destroy", "mgpu_mlp_set_data", "mgpu_mlp_train",
     "mgpu_mlp_gradients", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_predict",
     "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_sizes", "mgpu_batch_set_prims", "mgpu_batch_upload_state",
@@ -114,6 +114,8 @@ def load_library(path: str = LIB_PATH):
     L.mgpu_source.argtypes = [E, dp]
     L.mgpu_flux_reconstruction.argtypes = [E, dp, dp, ctypes.c_int, ctypes.c_int, dp]
     L.mgpu_c2p_cells.argtypes = [E, ctypes.c_longlong, dp, dp, dp, sp]
+    vp = ctypes.c_void_p
+    L.mgpu_c2p_cells_noalloc.argtypes = [E, ctypes.c_longlong, vp, vp, vp, sp]
     L.mgpu_source_cells.argtypes = [E, ctypes.c_longlong, dp, dp, dp, dp]
     L.mgpu_set_model_extension.argtypes = [E, ctypes.c_int]
     L.mgpu_source_extension.argtypes = [E, dp]
@@ -345,6 +347,17 @@ class Engine:
         rc = self._check(self.L.mgpu_c2p_cells(self._h, n, _dp(c), _dp(p), _dp(a), ctypes.byref(st)),
                          "mgpu_c2p_cells")
         return rc, p, a, st.as_dict()
+
+    def c2p_cells_noalloc(self, n: int, cons_ptr: int, prims_ptr: int, aux_ptr: int, stats: bool = True):
+        """mgpu_c2p_cells on DEVICE arrays (addresses of [cell][var] arrays on
+        this engine's GPU), allocating nothing. Returns (rc, stats or None);
+        without stats the call only launches (sync before reading)."""
+        st = MgpuStats() if stats else None
+        rc = self._check(self.L.mgpu_c2p_cells_noalloc(self._h, int(n), ctypes.c_void_p(cons_ptr),
+                                                       ctypes.c_void_p(prims_ptr), ctypes.c_void_p(aux_ptr),
+                                                       ctypes.byref(st) if stats else None),
+                         "mgpu_c2p_cells_noalloc")
+        return rc, (st.as_dict() if stats else None)
 
     def source_cells(self, cons: np.ndarray, aux=None) -> np.ndarray:
         """Model::sourceTermSingleCell over n cells ([cell][var] arrays)."""
