@@ -89,18 +89,48 @@ def calibrate_copy():
 
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "method_ref")
+REF_OMP_BIN = os.path.join(ROOT, "oracle", "_ref", "method_ref_omp")
+
+
+def host_cpu():
+    """(model name, threads this process may run on) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    # the box grants this process a share of its cores through OMP_NUM_THREADS
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        threads = min(threads, int(env))
+    return model, max(threads, 1)
 
 
 def cpu_baseline(n: int, steps: int):
-    """The reference's own serial CPU build (oracle/_ref/method_ref, compiled from
-    the reference sources) when it is present, else the C oracle, on one host
-    core, over a bounded sample of the workload: KH 3-D n^3, WENO5, RK2."""
+    """The reference's own CPU builds (oracle/_ref, compiled from the reference
+    sources) when they are present, else the C oracle, over a bounded sample of
+    the workload: KH 3-D n^3, WENO5, RK2. The reported figure is the OpenMP
+    build on the host threads this process may use; the serial build's
+    figure sits beside it."""
+    model, threads = host_cpu()
     if os.path.exists(REF_BIN):
-        return reference_baseline(n, steps)
-    return port_baseline(n, steps)
+        serial = reference_baseline(REF_BIN, n, steps, 1, model)
+        if os.path.exists(REF_OMP_BIN) and threads > 1:
+            out = reference_baseline(REF_OMP_BIN, n, steps, threads, model)
+            out["serial"] = serial
+            return out
+        return serial
+    out = port_baseline(n, steps)
+    out["cpu_model"] = model
+    return out
 
 
-def reference_baseline(n: int, steps: int):
+def reference_baseline(binary: str, n: int, steps: int, threads: int, model: str):
     import subprocess
     import tempfile
 
@@ -108,6 +138,7 @@ def reference_baseline(n: int, steps: int):
 
     g = idt.kh3d_grid(n)
     prims0 = idt.kh_prims(g, seed=1234)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     with tempfile.TemporaryDirectory() as td:
         pin = os.path.join(td, "prims.bin")
         np.ascontiguousarray(prims0).tofile(pin)
@@ -115,14 +146,16 @@ def reference_baseline(n: int, steps: int):
                 "zmin=0.0", "zmax=1.0", "Ng=4", f"gamma={4.0 / 3.0!r}", "cfl=0.4", "weno=5", "integrator=rk2",
                 "bcx=periodic", "bcy=outflow", "bcz=periodic", "mode=run", f"steps={steps}", f"prims_in={pin}",
                 f"out={os.path.join(td, 'out')}", "dump=-1"]
-        r = subprocess.run([REF_BIN] + args, capture_output=True, text=True, cwd=td)
+        r = subprocess.run([binary] + args, capture_output=True, text=True, cwd=td, env=env)
     if r.returncode != 0:
         raise RuntimeError("reference CPU build failed: " + r.stderr[-2000:])
     secs = float(r.stdout.split("step_seconds=")[1].split()[0])
-    return {"value": n ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-            "sample": f"reference METHOD serial CPU build (Simulation::updateTime with RK2, FVS, WENO5, SRMHD "
+    build = "OpenMP (-fopenmp; FVS and WENO loops parallel, cons2prim serial)" if threads > 1 else "serial"
+    return {"value": n ** 3 * steps / secs, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
+            "cpu_model": model,
+            "sample": f"reference METHOD {build} CPU build (Simulation::updateTime with RK2, FVS, WENO5, SRMHD "
                       f"with cminpack hybrd C2P), KH 3-D {n}^3, {steps} steps, {secs:.1f} s inside updateTime "
-                      f"on 1 host core"}
+                      f"on {threads} host thread{'s' if threads > 1 else ''} of {model}"}
 
 
 def port_baseline(n: int, steps: int):
