@@ -207,7 +207,17 @@ def c2p_comparison(e, dt, args):
         out[name] = {"cells_per_s": cells / (ms * 1e-3), "kernel_ms": ms,
                      "newton_steps_per_cell": st["iterations"] / max(cells, 1), "max_newton_steps": st["max_iterations"],
                      "fallback_rate": st["fallbacks"] / max(cells, 1), "failures": st["failures"]}
+        # Newton steps per cell over every cell (a separate, untimed call)
+        e.upload(prims=p_prev)
+        e.set_c2p_histogram(True)
+        e.c2p()
+        hist = e.c2p_histogram()
+        e.set_c2p_histogram(False)
+        c = np.cumsum(hist)
+        out[name]["newton_steps_p50"] = int(np.searchsorted(c, 0.50 * c[-1]))
+        out[name]["newton_steps_p99"] = int(np.searchsorted(c, 0.99 * c[-1]))
     e.set_c2p_guess(eng.GUESS_PLAIN)
+    out["learned"]["guess_error"] = learned_guess_error(e, params, scales, prims["plain"])
     ref = prims["plain"]
     scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
     for name in ("learned", "cold"):
@@ -216,6 +226,32 @@ def c2p_comparison(e, dt, args):
     out["learned_vs_plain_speedup"] = out["learned"]["cells_per_s"] / out["plain"]["cells_per_s"]
     out["learned_vs_cold_speedup"] = out["learned"]["cells_per_s"] / out["cold"]["cells_per_s"]
     return out
+
+
+def learned_guess_error(e, params, scales, prims):
+    """Relative error of the network's decoded guess (W, p) against the
+    converged primitives on every 16th interior cell of the evaluation state:
+    median and 90th percentile."""
+    from method_amd import engine as eng
+
+    cons, _, _ = e.download()
+    ng = 4
+    c8 = cons[:8, ng:-ng, ng:-ng, ng:-ng].reshape(8, -1)[:, ::16].T.copy()
+    pr = prims[:, ng:-ng, ng:-ng, ng:-ng].reshape(prims.shape[0], -1)[:, ::16]
+    m = eng.Mlp(seed=0)
+    try:
+        m.set_params(params, scales)
+        y = m.predict(c8).astype(np.float64)
+    finally:
+        m.close()
+    ym, ys = scales[:2].astype(np.float64), scales[2:].astype(np.float64)
+    W_g = np.exp(np.clip(y[:, 0] * ys[0] + ym[0], 0.0, 6.0))
+    p_g = (c8[:, 4] + c8[:, 0]) * np.exp(y[:, 1] * ys[1] + ym[1])
+    W = 1.0 / np.sqrt(1.0 - np.sum(pr[1:4] ** 2, axis=0))
+    eW, ep = np.abs(W_g / W - 1.0), np.abs(p_g / pr[4] - 1.0)
+    return {"cells": int(c8.shape[0]), "W_rel_err_p50": float(np.median(eW)),
+            "W_rel_err_p90": float(np.quantile(eW, 0.9)), "p_rel_err_p50": float(np.median(ep)),
+            "p_rel_err_p90": float(np.quantile(ep, 0.9))}
 
 
 def srrmhd_leg(steps: int):

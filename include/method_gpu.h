@@ -315,13 +315,14 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best,
 int mgpu_batch_timed_steps(mgpu_batch *b, int nsteps, double *total_ms);
 int mgpu_batch_sync(mgpu_batch *b);
 
-/* Sweep plan of 3-D grids: 0 (default) runs the z and y terms of FVS::F
- * (fluxVectorSplitting.cc:92-112) as two launches, 1 as one fused launch.
- * The results are identical bit for bit; only the launch count, the HBM
- * traffic of the net flux and the register pressure differ (on MI355X the
- * fused launch holds more registers and is slower). 2-D grids are not
+/* No counterpart in the reference. Sweep plan of 3-D grids: 0 (default)
+ * runs the z, y and x terms of FVS::F (fluxVectorSplitting.cc:79-112) as one
+ * launch each, the x launch finishing the stage; 1 fuses the z and y
+ * launches; 2 runs x, y, z with the contiguous z launch finishing. Plans 0
+ * and 1 give identical bits; plan 2 sums the axes in the other order. Only
+ * launch count, HBM traffic and register pressure differ. 2-D grids are not
  * affected. */
-int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz);
+int mgpu_set_sweep_plan(mgpu_engine *e, int plan);
 
 /* Plan of the REGIME source (no reference counterpart; the reference forms
  * D = M K per axis in arrays, REGIME.cc:514-774, and differences them,

@@ -28,6 +28,10 @@ HOST_LIB = os.path.join(LIBDIR, "libmethod_host.so")
 GPU_SOURCES = ["mgpu_kernels.hip", "mgpu_engine.hip", "mgpu_mlp.hip", "mgpu_batch.hip", "mgpu_regime.hip",
                "mgpu_cells.hip"]
 ARCH = "gfx950"
+# Per-source extra flags. The MLP's MFMA accumulators stay in ordinary VGPRs,
+# so the activations read them without a copy out of the accumulation
+# registers.
+SOURCE_FLAGS = {"mgpu_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def hipcc() -> str:
@@ -58,7 +62,8 @@ def build_gpu() -> str:
     objs = [os.path.join(LIBDIR, src.replace(".hip", ".o")) for src in GPU_SOURCES]
     # one hipcc per source, run side by side (each is single-threaded)
     with ThreadPoolExecutor(max_workers=min(len(GPU_SOURCES), os.cpu_count() or 1, 8)) as pool:
-        list(pool.map(lambda so: _run([hipcc()] + gpu_flags() + ["-c", os.path.join(CSRC, so[0]), "-o", so[1]]),
+        list(pool.map(lambda so: _run([hipcc()] + gpu_flags() + SOURCE_FLAGS.get(so[0], [])
+                                      + ["-c", os.path.join(CSRC, so[0]), "-o", so[1]]),
                       zip(GPU_SOURCES, objs)))
     tmp = GPU_LIB + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)

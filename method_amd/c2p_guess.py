@@ -44,8 +44,8 @@ def workload_prims(n_kh: int, n_ot: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed + 1)
     out = []
     if n_kh:
-        g = idt.kh3d_grid(48)
-        with eng.Engine(eng.Config(nx=48, ny=48, nz=48, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0,
+        g = idt.kh3d_grid(64)
+        with eng.Engine(eng.Config(nx=64, ny=64, nz=64, xmin=-0.5, xmax=0.5, ymin=-1.0, ymax=1.0, zmin=0.0,
                                    zmax=1.0, weno_order=5)) as e:
             e.set_prims(idt.kh_prims(g, seed=seed))
             for _ in range(5):
@@ -87,10 +87,12 @@ def training_set(n_synth: int, n_kh: int, n_ot: int, seed: int):
     return cons[:, :8].copy(), np.stack([W, prims[:, 4]], axis=1)
 
 
-def train_guess(n_synth: int = 1 << 19, n_kh: int = 1 << 16, n_ot: int = 1 << 15, steps: int = 12000,
+def train_guess(n_synth: int = 1 << 18, n_kh: int = 1 << 18, n_ot: int = 1 << 17, steps: int = 12000,
                 batch: int = 8192, lr: float = 5e-3, seed: int = 7):
     """Trains the network on the GPU with a step-decayed Adam learning rate;
-    returns (params, output scales, final loss)."""
+    returns (params, output scales, final loss). By default 40 % of the
+    samples are Kelvin-Helmholtz states, 20 % Orszag-Tang states and 40 %
+    synthetic states over the full ranges."""
     cons8, wp = training_set(n_synth, n_kh, n_ot, seed)
     m = eng.Mlp(seed=seed)
     m.set_data(cons8, wp)

@@ -35,10 +35,17 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 bits, round to nearest even: the hardware conversion on the
-// device, the same rounding in integer arithmetic on the host.
-__device__ __forceinline__ short bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+// Four FP32 values as one BF16 MFMA operand, rounded to nearest even by the
+// hardware conversion, two values per instruction.
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x4 bf4(float a, float b, float c, float d) {
+  const bf16x2v lo = __builtin_convertvector((floatx2{a, b}), bf16x2v);
+  const bf16x2v hi = __builtin_convertvector((floatx2{c, d}), bf16x2v);
+  return __builtin_bit_cast(bf16x4, (uint2{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)}));
+}
 
+// FP32 -> BF16 bits, round to nearest even, in integer arithmetic (host).
 __host__ __device__ inline uint16_t f2bf(float x) {
   uint32_t u;
   memcpy(&u, &x, 4);
@@ -149,88 +156,113 @@ __global__ void k_normalise_targets(long long n, float *T, float m0, float s0, f
 
 // ------------------------------------------------------------ inference
 
+// The network's constants as a block stages them in LDS: the 24 weight
+// fragments in lane order (a wave reads fragment f as one contiguous row,
+// free of bank conflicts), the biases and the output scales.
+struct alignas(16) MlpShared {
+  uint2 frag[kMlpFrags][64];
+  float b1[kMlpHid], b2[kMlpHid];
+  float b3[kMlpOut], ymean[kMlpOut], ystd[kMlpOut];
+};
+static_assert(sizeof(MlpDev::frag) == sizeof(MlpShared::frag), "fragment layout");
+
+__device__ __forceinline__ void mlp_stage_shared(const MlpDev &m, MlpShared &s) {
+  const int t = threadIdx.x;
+  const uint4 *src = reinterpret_cast<const uint4 *>(&m.frag[0][0][0]);
+  uint4 *dst = reinterpret_cast<uint4 *>(&s.frag[0][0]);
+  for (int k = t; k < kMlpFrags * 64 / 2; k += blockDim.x) dst[k] = src[k];
+  if (t < kMlpHid) s.b1[t] = m.b1[t];
+  else if (t < 2 * kMlpHid) s.b2[t - kMlpHid] = m.b2[t - kMlpHid];
+  else if (t < 2 * kMlpHid + kMlpOut) {
+    const int k = t - 2 * kMlpHid;
+    s.b3[k] = m.b3[k];
+    s.ymean[k] = m.ymean[k];
+    s.ystd[k] = m.ystd[k];
+  }
+}
+
+__device__ __forceinline__ bf16x4 frag(const MlpShared &sh, int f, int lane) {
+  return __builtin_bit_cast(bf16x4, sh.frag[f][lane]);
+}
+
 // Runs the network for the 64 cells of one wave (one per lane, `feat` their
-// inputs) with BF16 weights on MFMA 16x16x16 and FP32 accumulation, and
-// returns this lane's two outputs. The weights are the A operand (16
-// outputs x 16 inputs per tile) and the activations the B operand (16 inputs
-// x 16 cells), so a layer's accumulator tile (lane l: rows 4(l/16)..+3 of
-// column l%16) is already the next layer's B operand. lds: 64*8 + 64*2 floats
-// private to the wave.
-__device__ __forceinline__ void mlp_wave(const MlpDev &m, const float feat[8], float *lds, float out[2]) {
+// inputs) on MFMA 16x16x16 BF16 with FP32 accumulation, and returns this
+// lane's two outputs. The weights are the A operand (16 outputs x 16 inputs
+// per tile) and the activations the B operand (16 inputs x 16 cells), so a
+// layer's accumulator tile (lane l: rows 4(l/16)..+3 of column l%16) is
+// already the next layer's B operand; each accumulator starts from the bias
+// of its rows. lds: 64*8 + 64*2 floats private to the wave.
+__device__ __forceinline__ void mlp_wave(const MlpShared &sh, const float feat[8], float *lds, float out[2]) {
   const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lq = lane >> 4;
   float *sfeat = lds, *sout = lds + 64 * 8;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) sfeat[lane * 8 + k] = feat[k];
-  // A operands: weights, [out][in] rows lr, inputs 4 lq .. 4 lq + 3
-  bf16x4 a1[4], a2[4][4], a3[4];
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    const uint16_t *w = m.w1t + (16 * mt + lr) * 16 + 4 * lq;
-    a1[mt] = bf16x4{(short)w[0], (short)w[1], (short)w[2], (short)w[3]};
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) {
-      const uint16_t *w2 = m.w2t + (16 * mt + lr) * kMlpHid + 16 * kt + 4 * lq;
-      a2[mt][kt] = bf16x4{(short)w2[0], (short)w2[1], (short)w2[2], (short)w2[3]};
-    }
-    const uint16_t *w3 = m.w3t + lr * kMlpHid + 16 * mt + 4 * lq;
-    a3[mt] = bf16x4{(short)w3[0], (short)w3[1], (short)w3[2], (short)w3[3]};
-  }
+  *reinterpret_cast<floatx4 *>(sfeat + lane * 8) = floatx4{feat[0], feat[1], feat[2], feat[3]};
+  *reinterpret_cast<floatx4 *>(sfeat + lane * 8 + 4) = floatx4{feat[4], feat[5], feat[6], feat[7]};
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     // B operand of layer 1: inputs 4 lq .. 4 lq + 3 of cell 16 t + lr (zero past 8)
-    bf16x4 x;
-    {
-      const float *f = sfeat + (16 * t + lr) * 8 + 4 * lq;
-      const bool in = lq < 2;
-      x = bf16x4{bf(in ? f[0] : 0.f), bf(in ? f[1] : 0.f), bf(in ? f[2] : 0.f), bf(in ? f[3] : 0.f)};
+    bf16x4 x = {0, 0, 0, 0};
+    if (lq < 2) {
+      const floatx4 f = *reinterpret_cast<const floatx4 *>(sfeat + (16 * t + lr) * 8 + 4 * lq);
+      x = bf4(f[0], f[1], f[2], f[3]);
     }
     bf16x4 h1[4], h2[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      // the accumulator starts from the bias of its rows
-      floatx4 acc = {m.b1[16 * mt + 4 * lq], m.b1[16 * mt + 4 * lq + 1], m.b1[16 * mt + 4 * lq + 2],
-                     m.b1[16 * mt + 4 * lq + 3]};
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[mt], x, acc, 0, 0, 0);
-      h1[mt] = bf16x4{bf(gelu(acc[0])), bf(gelu(acc[1])), bf(gelu(acc[2])), bf(gelu(acc[3]))};
+      floatx4 acc = *reinterpret_cast<const floatx4 *>(sh.b1 + 16 * mt + 4 * lq);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(sh, mt, lane), x, acc, 0, 0, 0);
+      h1[mt] = bf4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
     }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      floatx4 acc = {m.b2[16 * mt + 4 * lq], m.b2[16 * mt + 4 * lq + 1], m.b2[16 * mt + 4 * lq + 2],
-                     m.b2[16 * mt + 4 * lq + 3]};
+      floatx4 acc = *reinterpret_cast<const floatx4 *>(sh.b2 + 16 * mt + 4 * lq);
 #pragma unroll
-      for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[mt][kt], h1[kt], acc, 0, 0, 0);
-      h2[mt] = bf16x4{bf(gelu(acc[0])), bf(gelu(acc[1])), bf(gelu(acc[2])), bf(gelu(acc[3]))};
+      for (int kt = 0; kt < 4; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(sh, 4 + 4 * mt + kt, lane), h1[kt], acc, 0, 0, 0);
+      h2[mt] = bf4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
     }
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[kt], h2[kt], acc, 0, 0, 0);
+    for (int kt = 0; kt < 4; ++kt)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(sh, 20 + kt, lane), h2[kt], acc, 0, 0, 0);
     if (lq == 0) {  // rows 0 and 1 are the outputs of cell 16 t + lr
-      sout[(16 * t + lr) * 2] = acc[0] + m.b3[0];
-      sout[(16 * t + lr) * 2 + 1] = acc[1] + m.b3[1];
+      sout[(16 * t + lr) * 2] = acc[0] + sh.b3[0];
+      sout[(16 * t + lr) * 2 + 1] = acc[1] + sh.b3[1];
     }
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_s_waitcnt(0);
   out[0] = sout[lane * 2];
   out[1] = sout[lane * 2 + 1];
+  // the next group's feature writes must not overtake this group's reads
+  __builtin_amdgcn_wave_barrier();
 }
 
 constexpr int kThr = 256;
+// 256-cell groups per block of the fused cons2prim: the constants a block
+// stages serve four groups of its cells.
+constexpr int kLearnedGroups = 4;
 
-// Network outputs (normalised) for n cells given as (D, Sx, Sy, Sz, tau, Bx, By, Bz).
+// Network outputs (normalised) for n cells given as (D, Sx, Sy, Sz, tau, Bx,
+// By, Bz), with the inputs the fused cons2prim forms (mlp_features_fast).
 __global__ __launch_bounds__(kThr) void k_mlp_predict(long long n, const double *cons8, const MlpDev *m, float *y) {
-  __shared__ float lds[kThr / 64][64 * 10];
+  __shared__ __align__(16) float lds[kThr / 64][64 * 10];
+  __shared__ MlpShared sh;
+  mlp_stage_shared(*m, sh);
+  __syncthreads();
   const long long i = (long long)blockIdx.x * kThr + threadIdx.x;
   float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (i < n) {
-    const double *u = cons8 + 8 * i;
-    mlp_features(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], f);
+    double u[9];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) u[v] = cons8[8 * i + v];
+    u[8] = 0.0;
+    mlp_features_fast(u, f);
   }
   float out[2];
-  mlp_wave(*m, f, lds[threadIdx.x >> 6], out);
+  mlp_wave(sh, f, lds[threadIdx.x >> 6], out);
   if (i < n) {
     y[2 * i] = out[0];
     y[2 * i + 1] = out[1];
@@ -239,50 +271,67 @@ __global__ __launch_bounds__(kThr) void k_mlp_predict(long long n, const double 
 
 // SRMHD cons2prim with the learned starting guess; cells whose Newton solve
 // does not converge restart from the previous primitives (the reference's
-// guess, srmhd.cc:438-442) and count as fallbacks.
+// guess, srmhd.cc:438-442) and count as fallbacks. A block takes
+// kLearnedGroups consecutive runs of 256 cells and stages the network's
+// constants in LDS once for all of them.
 __global__ __launch_bounds__(kThr) void k_c2p_learned_srmhd(Geo g, const double *U, double *V, double *A,
                                                              const MlpDev *m, double gam, C2PCounters *st,
                                                              unsigned long long *hist) {
-  __shared__ float lds[kThr / 64][64 * 10];
-  const long long q = (long long)blockIdx.x * kThr + threadIdx.x;
-  const bool live = q < g.vstride;
+  __shared__ __align__(16) float lds[kThr / 64][64 * 10];
+  __shared__ MlpShared sh;
+  __shared__ unsigned s_hist[kHistBins];
+  mlp_stage_shared(*m, sh);
+  if (hist)
+    for (int b = threadIdx.x; b < kHistBins; b += kThr) s_hist[b] = 0u;
+  __syncthreads();
   const long long vs = g.vstride;
-  double u[9];
-  float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (live) {
-#pragma unroll
-    for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
-    mlp_features(u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], f);
-  }
-  float y[2];
-  mlp_wave(*m, f, lds[threadIdx.x >> 6], y);
+  const float gfac = (float)(gam / (gam - 1.0));
+  const float ym[2] = {sh.ymean[0], sh.ymean[1]}, ys[2] = {sh.ystd[0], sh.ystd[1]};
   int cells = 0, iters = 0, fails = 0, maxit = 0, fallback = 0;
-  if (live) {
-    double v2, Z;
-    mlp_decode_guess(y[0], y[1], *m, u[0], u[4], gam, v2, Z);
-    const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
-    const double Bsq = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
-    const double Ssq = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
-    int its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
-    int total = its < 0 ? kNewtonMaxIt : its;
-    if (its < 0) {
-      fallback = 1;
-      srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
-      its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
-      total += its < 0 ? kNewtonMaxIt : its;
+  for (int grp = 0; grp < kLearnedGroups; ++grp) {
+    const long long q = ((long long)blockIdx.x * kLearnedGroups + grp) * kThr + threadIdx.x;
+    const bool live = q < vs;
+    double u[9];
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (live) {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) u[v] = U[v * vs + q];
+      mlp_features_fast(u, f);
     }
-    double P[8], aux[13];
-    srmhd_recover(u, gam, v2, Z, P, aux);
+    float y[2];
+    mlp_wave(sh, f, lds[threadIdx.x >> 6], y);
+    if (live) {
+      double v2, Z;
+      mlp_decode_guess_fast(y[0], y[1], ym, ys, u[0], u[4], gfac, v2, Z);
+      const double BS = u[5] * u[1] + u[6] * u[2] + u[7] * u[3];
+      const double Bsq = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+      const double Ssq = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
+      int its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
+      int total = its < 0 ? kNewtonMaxIt : its;
+      if (its < 0) {
+        ++fallback;
+        srmhd_plain_guess(V[q], V[vs + q], V[2 * vs + q], V[3 * vs + q], V[4 * vs + q], gam, v2, Z);
+        its = srmhd_newton(u[0], u[4], Bsq, Ssq, BS, gam, v2, Z);
+        total += its < 0 ? kNewtonMaxIt : its;
+      }
+      double P[8], aux[13];
+      srmhd_recover(u, gam, v2, Z, P, aux);
 #pragma unroll
-    for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
+      for (int v = 0; v < 8; ++v) V[v * vs + q] = P[v];
 #pragma unroll
-    for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
-    cells = 1;
-    iters = total;
-    maxit = total;
-    fails = its < 0;
+      for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
+      ++cells;
+      iters += total;
+      maxit = max(maxit, total);
+      fails += its < 0;
+      if (hist) atomicAdd(&s_hist[its < 0 || total >= kHistBins - 1 ? kHistBins - 1 : total], 1u);
+    }
   }
-  hist_add_block(hist, cells > 0, fails ? -1 : iters);
+  if (hist) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < kHistBins; b += kThr)
+      if (s_hist[b]) atomicAdd(&hist[b], (unsigned long long)s_hist[b]);
+  }
   if (st) {
     for (int off = 32; off > 0; off >>= 1) {
       cells += __shfl_xor(cells, off);
@@ -308,19 +357,28 @@ inline unsigned nblk(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 void mgpu::launch_c2p_learned_srmhd(const Geo &g, const double *U, double *V, double *A, const MlpDev *m, double gam,
                                     C2PCounters *st, unsigned long long *hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_c2p_learned_srmhd, dim3(nblk(g.vstride, kThr)), dim3(kThr), 0, s, g, U, V, A, m, gam, st,
-                     hist);
+  hipLaunchKernelGGL(k_c2p_learned_srmhd, dim3(nblk(g.vstride, kThr * kLearnedGroups)), dim3(kThr), 0, s, g, U, V, A,
+                     m, gam, st, hist);
 }
 
 // Packs FP32 parameters and output scales into the inference form.
 void mgpu::mlp_pack(const float *params, const float *ymean, const float *ystd, MlpDev *out) {
   memset(out, 0, sizeof(MlpDev));
-  for (int o = 0; o < kMlpHid; ++o)
-    for (int i = 0; i < kMlpIn; ++i) out->w1t[o * 16 + i] = f2bf(params[kMlpW1 + i * kMlpHid + o]);
-  for (int o = 0; o < kMlpHid; ++o)
-    for (int i = 0; i < kMlpHid; ++i) out->w2t[o * kMlpHid + i] = f2bf(params[kMlpW2 + i * kMlpHid + o]);
-  for (int o = 0; o < kMlpOut; ++o)
-    for (int i = 0; i < kMlpHid; ++i) out->w3t[o * kMlpHid + i] = f2bf(params[kMlpW3 + i * kMlpOut + o]);
+  // transposed weights [out][in]: W1 padded to 16 inputs, W3 to 16 outputs
+  auto w1t = [&](int o, int i) { return i < kMlpIn ? params[kMlpW1 + i * kMlpHid + o] : 0.0f; };
+  auto w2t = [&](int o, int i) { return params[kMlpW2 + i * kMlpHid + o]; };
+  auto w3t = [&](int o, int i) { return o < kMlpOut ? params[kMlpW3 + i * kMlpOut + o] : 0.0f; };
+  for (int lane = 0; lane < 64; ++lane) {
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int j = 0; j < 4; ++j) {
+      for (int mt = 0; mt < 4; ++mt) {
+        out->frag[mt][lane][j] = f2bf(w1t(16 * mt + lr, 4 * lq + j));
+        for (int kt = 0; kt < 4; ++kt)
+          out->frag[4 + 4 * mt + kt][lane][j] = f2bf(w2t(16 * mt + lr, 16 * kt + 4 * lq + j));
+      }
+      for (int kt = 0; kt < 4; ++kt) out->frag[20 + kt][lane][j] = f2bf(w3t(lr, 16 * kt + 4 * lq + j));
+    }
+  }
   for (int k = 0; k < kMlpHid; ++k) {
     out->b1[k] = params[kMlpB1 + k];
     out->b2[k] = params[kMlpB2 + k];

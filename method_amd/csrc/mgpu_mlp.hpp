@@ -19,13 +19,18 @@ constexpr int kMlpB3 = kMlpW3 + kMlpHid * kMlpOut;
 constexpr int kMlpParams = kMlpB3 + kMlpOut;  // 4866
 constexpr int kMlpWeights = kMlpIn * kMlpHid + kMlpHid * kMlpHid + kMlpHid * kMlpOut;  // 4736
 
-// Inference form in device memory: weights transposed to [out][in] in BF16 (W1
-// padded to 16 inputs, W3 to 16 outputs) as MFMA A operands, FP32 biases and
-// the output de-normalisation.
+// Inference form in device memory. The BF16 weights are stored as the 24 MFMA
+// 16x16x16 A-operand fragments the inference reads, each in lane order (lane
+// l holds 4 consecutive inputs 4 (l / 16) .. + 3 of output row l % 16 of its
+// 16 x 16 tile of the transposed weights; W1 padded to 16 inputs, W3 to 16
+// outputs), so a wave reads a fragment as one contiguous 512-byte row:
+//   fragment mt       W1 tile (outputs 16 mt ..),
+//   fragment 4+4mt+kt W2 tile (outputs 16 mt .., inputs 16 kt ..),
+//   fragment 20+kt    W3 tile (inputs 16 kt ..).
+// Then the FP32 biases and the output de-normalisation.
+constexpr int kMlpFrags = 24;
 struct MlpDev {
-  uint16_t w1t[kMlpHid * 16];
-  uint16_t w2t[kMlpHid * kMlpHid];
-  uint16_t w3t[16 * kMlpHid];
+  uint16_t frag[kMlpFrags][64][4];
   float b1[kMlpHid], b2[kMlpHid], b3[kMlpOut];
   float ymean[kMlpOut], ystd[kMlpOut];
 };
@@ -54,41 +59,76 @@ __device__ __host__ inline void mlp_targets(double W, double p, double D, double
   t[1] = (float)log(p / (tau + D));
 }
 
-// tanh-form GELU written through the logistic function:
-// 0.5 x (1 + tanh(c (x + 0.044715 x^3))) = x sigmoid(2 c (x + 0.044715 x^3)),
-// with the logistic's exp taken as a base-2 exponential (log2 e folded into
-// the constants) and its reciprocal as the hardware reciprocal: one exp2, one
-// rcp and five FP32 operations, for training and inference alike.
-constexpr float kGeluK = 1.5957691216057308f;  // 2 sqrt(2/pi)
-constexpr float kGeluA = 0.044715f;
-constexpr float kLog2e = 1.4426950408889634f;
-constexpr float kGeluK2 = -kGeluK * kLog2e;           // exp(-u) = exp2(x (kGeluK2 + kGeluKA2 x^2))
-constexpr float kGeluKA2 = -kGeluK * kGeluA * kLog2e;
+// GELU, x Phi(x), with the normal distribution function Phi taken as a
+// clamped odd polynomial: Phi(x) ~ 1/2 + t P(t^2), t = clamp(x, -c, c),
+// P of degree 4, its coefficients a minimax fit of |x (Phi~(x) - Phi(x))|
+// over the real line under the constraint Phi~(c) = 1 (largest error
+// 1.22e-3, below the BF16 rounding of the activation it feeds; x beyond c
+// gives x or 0). Eight FP32 operations and no transcendental; the training
+// kernels use the same function and its exact derivative.
+constexpr float kGeluC = 3.38253573f;
+constexpr float kGeluP0 = 3.93638938e-01f, kGeluP1 = -5.91898904e-02f, kGeluP2 = 6.53467361e-03f,
+                kGeluP3 = -3.95613815e-04f, kGeluP4 = 9.83280758e-06f;
 
-__device__ __forceinline__ float gelu_sigmoid(float x) {
-  const float t = x * fmaf(kGeluKA2, x * x, kGeluK2);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
+__device__ __forceinline__ float gelu_poly(float t2) {
+  float p = fmaf(t2, kGeluP4, kGeluP3);
+  p = fmaf(t2, p, kGeluP2);
+  p = fmaf(t2, p, kGeluP1);
+  return fmaf(t2, p, kGeluP0);
 }
 
-__device__ __forceinline__ float gelu(float x) { return x * gelu_sigmoid(x); }
+__device__ __forceinline__ float gelu(float x) {
+  const float t = __builtin_amdgcn_fmed3f(x, -kGeluC, kGeluC);
+  return x * fmaf(t, gelu_poly(t * t), 0.5f);
+}
 
+// d/dx of gelu: Phi~(x) + x Phi~'(x), Phi~' = P(t^2) + 2 t^2 P'(t^2) inside the
+// clamp and 0 outside.
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float s = gelu_sigmoid(x);
-  return s + x * s * (1.0f - s) * kGeluK * (1.0f + 3.0f * kGeluA * x * x);
+  const float t = __builtin_amdgcn_fmed3f(x, -kGeluC, kGeluC);
+  const float t2 = t * t;
+  const float P = gelu_poly(t2);
+  float dP = fmaf(t2, 4.0f * kGeluP4, 3.0f * kGeluP3);
+  dP = fmaf(t2, dP, 2.0f * kGeluP2);
+  dP = fmaf(t2, dP, kGeluP1);
+  const float dphi = fabsf(x) < kGeluC ? fmaf(2.0f * t2, dP, P) : 0.0f;
+  return fmaf(x, dphi, fmaf(t, P, 0.5f));
 }
 
-// Decoded guess (v^2, Z = rho h W^2) for the Newton solve from the two outputs.
-__device__ __forceinline__ void mlp_decode_guess(float y0, float y1, const MlpDev &m, double D, double tau, double gam,
-                                                 double &v2, double &Z) {
-  double lw = (double)(y0 * m.ystd[0] + m.ymean[0]);
-  if (lw < 0.0) lw = 0.0;
-  if (lw > 6.0) lw = 6.0;
-  const double W = exp(lw);
-  const double p = (tau + D) * exp((double)(y1 * m.ystd[1] + m.ymean[1]));
-  v2 = 1.0 - 1.0 / (W * W);
-  const double rho = D / W;
-  const double h = 1.0 + p / rho * (gam / (gam - 1.0));
-  Z = rho * h * W * W;
+// The network inputs of mlp_features in FP32 for inference, from the same
+// conserved values: hardware log2, reciprocal and reciprocal square root, and
+// log(E / D) as log(1 + tau / D) so that it keeps its relative accuracy when
+// tau << D.
+constexpr float kLn2 = 0.6931471805599453f;
+__device__ __forceinline__ void mlp_features_fast(const double u[9], float f[8]) {
+  const float D = (float)u[0], tau = (float)u[4];
+  const float E = D + tau;
+  const float iE = __builtin_amdgcn_rcpf(E), isE = __builtin_amdgcn_rsqf(E);
+  f[0] = __builtin_amdgcn_logf(D) * (0.2f * kLn2);
+  f[1] = (float)u[1] * iE;
+  f[2] = (float)u[2] * iE;
+  f[3] = (float)u[3] * iE;
+  f[4] = __builtin_amdgcn_logf(1.0f + tau * __builtin_amdgcn_rcpf(D)) * (0.25f * kLn2);
+  f[5] = (float)u[5] * isE;
+  f[6] = (float)u[6] * isE;
+  f[7] = (float)u[7] * isE;
+}
+
+// Decoded guess (v^2, Z = rho h W^2) for the Newton solve from the two
+// normalised outputs, in FP32 (the guess only seeds the FP64 Newton solve):
+// W = exp(log W) clamped to [1, e^6], p = E exp(log(p / E)),
+// v^2 = 1 - 1 / W^2 and Z = rho h W^2 = D W h with h = 1 + p W / D gam/(gam-1).
+constexpr float kLog2e = 1.4426950408889634f;
+__device__ __forceinline__ void mlp_decode_guess_fast(float y0, float y1, const float ym[2], const float ys[2],
+                                                      double D, double tau, float gfac, double &v2, double &Z) {
+  const float lw = fminf(fmaxf(fmaf(y0, ys[0], ym[0]), 0.0f), 6.0f);
+  const float W = __builtin_amdgcn_exp2f(lw * kLog2e);
+  const float iW = __builtin_amdgcn_rcpf(W);
+  const float Df = (float)D, Ef = (float)(D + tau);
+  const float p = Ef * __builtin_amdgcn_exp2f(fmaf(y1, ys[1], ym[1]) * kLog2e);
+  const float h = fmaf(p * W * __builtin_amdgcn_rcpf(Df), gfac, 1.0f);
+  v2 = (double)fmaf(-iW, iW, 1.0f);
+  Z = (double)(Df * W * h);
 }
 
 struct Geo;

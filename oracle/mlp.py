@@ -14,8 +14,12 @@ import numpy as np
 IN, HID, OUT = 8, 64, 2
 SIZES = [("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)), ("W3", (HID, OUT)), ("b3", (OUT,))]
 NPARAMS = sum(int(np.prod(s)) for _, s in SIZES)
-GELU_K = 1.5957691216057308
-GELU_A = 0.044715
+# GELU as the engine evaluates it: x Phi~(x) with Phi~(x) = 1/2 + t P(t^2),
+# t = clip(x, -c, c), P of degree 4 (the FP32 constants of mgpu_mlp.hpp, here
+# in float64); within 1.22e-3 of the exact x Phi(x) over |x| <= 40.
+GELU_C = float(np.float32(3.38253573))
+GELU_P = [float(np.float32(v)) for v in (3.93638938e-01, -5.91898904e-02, 6.53467361e-03, -3.95613815e-04,
+                                         9.83280758e-06)]
 
 
 def unpack(p):
@@ -44,15 +48,33 @@ def targets(W, p, D, tau):
     return np.stack([np.log(W), np.log(p / (tau + D))], axis=1)
 
 
+def _poly(t2):
+    p = np.zeros_like(t2)
+    for c in reversed(GELU_P):
+        p = p * t2 + c
+    return p
+
+
+def _dpoly(t2):
+    p = np.zeros_like(t2)
+    for k in reversed(range(1, len(GELU_P))):
+        p = p * t2 + k * GELU_P[k]
+    return p
+
+
 def gelu(x):
-    u = GELU_K * (x + GELU_A * x ** 3)
-    return x / (1.0 + np.exp(-u))
+    x = np.asarray(x, dtype=np.float64)
+    t = np.clip(x, -GELU_C, GELU_C)
+    return x * (0.5 + t * _poly(t * t))
 
 
 def gelu_grad(x):
-    u = GELU_K * (x + GELU_A * x ** 3)
-    s = 1.0 / (1.0 + np.exp(-u))
-    return s + x * s * (1.0 - s) * GELU_K * (1.0 + 3.0 * GELU_A * x ** 2)
+    x = np.asarray(x, dtype=np.float64)
+    t = np.clip(x, -GELU_C, GELU_C)
+    t2 = t * t
+    P = _poly(t2)
+    dphi = np.where(np.abs(x) < GELU_C, P + 2.0 * t2 * _dpoly(t2), 0.0)
+    return 0.5 + t * P + x * dphi
 
 
 def forward(p, x):
