@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--srrmhd-only", action="store_true",
                     help="run only the SRRMHD SSP2 leg and print its JSON (for profiling its kernels)")
     ap.add_argument("--mlp-steps", type=int, default=12000)
+    ap.add_argument("--mlp-params", default=None,
+                    help="npz of trained guess-network parameters: loaded if it exists, else written after training")
+    ap.add_argument("--c2p-only", action="store_true",
+                    help="run only the cons2prim guess comparison and print its JSON (for profiling its kernels)")
     ap.add_argument("--no-regime", action="store_true", help="skip the SRMHD + REGIME RKSplit leg")
     ap.add_argument("--regime-n", type=int, default=256)
     ap.add_argument("--regime-steps", type=int, default=10)
@@ -186,9 +190,15 @@ def c2p_comparison(e, dt, args):
     from method_amd import c2p_guess
     from method_amd import engine as eng
 
-    print("bench: training the cons2prim guess network", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    params, scales, loss = c2p_guess.train_guess(steps=args.mlp_steps)
+    if args.mlp_params and os.path.exists(args.mlp_params):
+        with np.load(args.mlp_params, allow_pickle=False) as z:
+            params, scales, loss = z["params"], z["scales"], float(z["loss"])
+    else:
+        print("bench: training the cons2prim guess network", file=sys.stderr, flush=True)
+        params, scales, loss = c2p_guess.train_guess(steps=args.mlp_steps)
+        if args.mlp_params:
+            np.savez(args.mlp_params, params=params, scales=scales, loss=np.float64(loss))
     train_s = time.perf_counter() - t0
     _, p_prev, _ = e.download()
     _, st_step = e.step(dt, stats=True)
@@ -405,6 +415,11 @@ def main():
     for _ in range(args.warmup):
         e.update_time()
     e.sync()
+    if args.c2p_only:
+        t, dt, _ = e.time()
+        print(json.dumps({"c2p": c2p_comparison(e, dt, args)}))
+        e.close()
+        return
 
     # dt sequence of the next K updateTime calls (slow start changes dt only)
     t, _, iters = e.time()
