@@ -78,7 +78,9 @@ def parse():
 
 def calibrate_copy():
     """A plain copy kernel of known size: 2^27 doubles read and written per launch.
-    Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those counters."""
+    Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those counters;
+    its timed copies (HIP events on the copy's stream) give the practical HBM
+    rate of a pure streaming kernel on this GPU."""
     import torch
 
     n = 1 << 27
@@ -88,8 +90,16 @@ def calibrate_copy():
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
+    reps = 20
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
     print(json.dumps({"calibration": "copy", "bytes_read_per_launch": 8 * n, "bytes_written_per_launch": 8 * n,
-                      "launches": 3}))
+                      "launches": 3 + reps, "ms_per_copy": ms, "copy_GBs": 16 * n / (ms * 1e-3) / 1e9}))
 
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "method_ref")
@@ -333,7 +343,9 @@ def regime_leg(n: int, steps: int):
 def batch_leg(steps: int, members: int):
     """64 independent 2-D ideal SRMHD KH runs of uneven size, WENO5, RK2, per-member
     dt: the batched engine (autotuned persistent stage kernels under a hipGraph)
-    against the same members launched one at a time on the single-instance engine."""
+    against the same members launched one at a time on the single-instance
+    engine, and one at a time as one-member batches (the fused stage kernels
+    without the batching), which separates the two effects."""
     from method_amd import engine as eng
     from method_amd import initdata as idt
 
@@ -368,6 +380,20 @@ def batch_leg(steps: int, members: int):
             ms, _ = e.timed_steps(dts)
             single_ms += ms
     one_at_a_time = cells * steps / (single_ms * 1e-3)
+
+    # fusion without batching: each member alone as a one-member batch (the
+    # same fused stage kernels and hipGraph, autotuned for that member),
+    # launched one member after another
+    fused_ms = 0.0
+    for m in ms_list:
+        with eng.Batch([eng.Instance(m[0], m[1])], weno_order=5, bc=(eng.PERIODIC, eng.OUTFLOW)) as b1:
+            b1.set_prims(0, idt.kh_prims(idt.Grid(m[0], m[1], 0, -0.5, 0.5, -1.0, 1.0), seed=m[4], shear=m[2],
+                                         bz=m[3]))
+            b1.autotune(trial_steps=3)
+            b1.step(2)
+            b1.sync()
+            fused_ms += b1.timed_steps(steps)
+    fused_one_at_a_time = cells * steps / (fused_ms * 1e-3)
     return {"workload": f"{members} independent ideal SRMHD 2-D KH runs, nx, ny from {{128..768}}, WENO5, RK2, "
                         "per-member dt, FP64",
             "cells": cells, "steps": steps,
@@ -375,6 +401,10 @@ def batch_leg(steps: int, members: int):
             "hbm_frac_784B": batched * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
             "one_at_a_time_zone_updates_per_s": one_at_a_time, "one_at_a_time_ms_per_step": single_ms / steps,
             "speedup_vs_one_at_a_time": batched / one_at_a_time,
+            "fused_one_member_per_launch_zone_updates_per_s": fused_one_at_a_time,
+            "fused_one_member_per_launch_ms_per_step": fused_ms / steps,
+            "speedup_from_fusion": fused_one_at_a_time / one_at_a_time,
+            "speedup_from_batching": batched / fused_one_at_a_time,
             "tuning": best, "candidates": len(cands),
             "newton_steps_per_cell": st["iterations"] / max(st["cells"], 1), "failures": st["failures"]}
 

@@ -315,13 +315,13 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best,
 int mgpu_batch_timed_steps(mgpu_batch *b, int nsteps, double *total_ms);
 int mgpu_batch_sync(mgpu_batch *b);
 
-/* No counterpart in the reference. Sweep plan of 3-D grids: 0 (default)
- * runs the z, y and x terms of FVS::F (fluxVectorSplitting.cc:79-112) as one
- * launch each, the x launch finishing the stage; 1 fuses the z and y
- * launches; 2 runs x, y, z with the contiguous z launch finishing. Plans 0
- * and 1 give identical bits; plan 2 sums the axes in the other order. Only
- * launch count, HBM traffic and register pressure differ. 2-D grids are not
- * affected. */
+/* No counterpart in the reference. Sweep plan of 3-D grids: 2 (default)
+ * runs the x, y and z terms of FVS::F (fluxVectorSplitting.cc:79-112) as one
+ * launch each in the reference's order, the contiguous z launch finishing
+ * the stage; 0 runs z, y, x with the x launch finishing; 1 is plan 0 with
+ * the z and y launches fused. Plans 0 and 1 give identical bits; plan 2 sums
+ * the axes in the other order. Only launch count, HBM traffic and register
+ * pressure differ. 1-D and 2-D grids always sweep as plan 0. */
 int mgpu_set_sweep_plan(mgpu_engine *e, int plan);
 
 /* Plan of the REGIME source (no reference counterpart; the reference forms

@@ -86,7 +86,7 @@ struct mgpu_engine {
   MlpDev *mlp = nullptr;              // learned guess network (device)
   unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
   int fused_yz = 0;                      // 3-D: fused z+y sweep launch (off: slower here)
-  int xyz_order = 0;                     // sweeps x, y, z with the last (contiguous) one finishing
+  int xyz_order = 1;                     // 3-D: sweeps x, y, z with the last (contiguous) one finishing
   // SRMHD RK2 and RKSplit steps keep the cons2prim root (v^2, Z) of each cell
   // instead of writing its 13 aux values: no kernel of those steps reads aux,
   // and what does (state download, wavespeed) forms it from the root first.
@@ -173,9 +173,11 @@ SweepArgs sweep_args(mgpu_engine *e, const double *U, const double *V, C2PCounte
   return a;
 }
 
-// Sweeps in the order z, y, x; the x sweep applies `last_mode`. On 3-D grids
-// the z and y sweeps can run as one fused launch instead; both plans give the
-// same bits.
+// The directional sweeps of one stage. 3-D grids sweep x, y, z by default,
+// the reference's summation order (fluxVectorSplitting.cc:79-112), and the
+// contiguous z sweep applies `last_mode`; 1-D and 2-D grids, and 3-D grids
+// under sweep plan 0 or 1, sweep z, y, x with the x sweep applying it (plan 1
+// fuses the z and y sweeps into one launch with the same bits as plan 0).
 int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
   const int dims = e->geo.dims;
   const int order = e->cfg.weno_order, quirk = e->cfg.quirks;
@@ -199,7 +201,7 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
       {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y_imex2", "sweep_y_final"},
       {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z_imex2",
        "sweep_z_final"}};
-  if (e->xyz_order) {
+  if (e->xyz_order && dims == 3) {
     // x, then y, then z: the reference's summation order (fluxVectorSplitting.cc:79-112);
     // the last axis, the contiguous one, applies last_mode
     for (int dir = 0; dir < dims; ++dir) {
@@ -837,6 +839,9 @@ int mgpu_set_model_extension(mgpu_engine *e, int kind) {
   if (kind != MGPU_EXT_NONE && kind != MGPU_EXT_REGIME) return fail("mgpu_set_model_extension: unknown kind %d", kind);
   if (kind == MGPU_EXT_REGIME && e->cfg.model != MGPU_SRMHD)
     return fail("mgpu_set_model_extension: REGIME extends the ideal SRMHD model only");
+  // the REGIME kernels index the padded grid with 32-bit cell numbers
+  if (kind == MGPU_EXT_REGIME && e->pad.ntot >= (1LL << 31))
+    return fail("mgpu_set_model_extension: REGIME needs fewer than 2^31 padded cells (have %lld)", e->pad.ntot);
   if (kind == MGPU_EXT_REGIME && !e->rg_work) {
     const size_t nw = regime_work_doubles(e->pad), ns = (size_t)8 * e->geo.vstride;
     if (hipMalloc(&e->rg_work, nw * sizeof(double)) != hipSuccess ||

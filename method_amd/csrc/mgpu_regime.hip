@@ -40,10 +40,43 @@ struct Spacing {
   double ih2[3];
 };
 
-__device__ __forceinline__ void padded_pos(const PadGeo &pg, long long p, int pos[3]) {
-  pos[0] = (int)(p / ((long long)pg.N[1] * pg.N[2]));
-  pos[1] = (int)((p / pg.N[2]) % pg.N[1]);
-  pos[2] = (int)(p % pg.N[2]);
+// n / d for 0 <= n < 2^31 by a multiply-high and a shift: with l the
+// smallest integer with d <= 2^l and m = ceil(2^(31+l) / d) (< 2^32 for
+// d >= 2), n m = 2^(31+l) (n/d + delta) with 0 <= delta < 2^-l <= 1/d, so
+// (n m) >> (31 + l) is floor(n / d) exactly (Granlund & Montgomery 1994,
+// sec. 4). d == 1 passes n through.
+struct FastDiv {
+  unsigned d, m, s;  // s = l - 1
+};
+
+FastDiv make_fastdiv(unsigned d) {
+  FastDiv f = {d, 0u, 0u};
+  if (d < 2) return f;
+  unsigned l = 0;
+  while ((1ull << l) < d) ++l;
+  f.m = (unsigned)(((1ull << (31 + l)) + d - 1) / d);
+  f.s = l - 1;
+  return f;
+}
+
+__device__ __forceinline__ unsigned fdiv(unsigned n, const FastDiv &f) {
+  return f.d < 2 ? n : __umulhi(n, f.m) >> f.s;
+}
+
+// Divisors of the padded index: one plane (N1 N2) and one row (N2).
+struct PadDiv {
+  FastDiv plane, row;
+};
+
+// (i, j, k) of padded cell p (the padded grid holds fewer than 2^31 cells).
+__device__ __forceinline__ void padded_pos(const PadDiv &dv, long long p, int pos[3]) {
+  const unsigned u = (unsigned)p;
+  const unsigned i = fdiv(u, dv.plane);
+  const unsigned r = u - i * dv.plane.d;
+  const unsigned j = fdiv(r, dv.row);
+  pos[0] = (int)i;
+  pos[1] = (int)j;
+  pos[2] = (int)(r - j * dv.row.d);
 }
 
 // interior cell whose values Bcs::apply copies into padded position pos
@@ -62,11 +95,11 @@ __device__ __forceinline__ void load_prims(const double *V, long long vs, long l
   for (int v = 0; v < 8; ++v) P[v] = V[v * vs + q];
 }
 
-__global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, const double *V, double *E) {
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, PadDiv dv, const double *V, double *E) {
   const long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
-  padded_pos(pg, p, pos);
+  padded_pos(dv, p, pos);
   double P[8];
   load_prims(V, g.vstride, image_of(g, pg, pos), P);
   E[p] = -(P[2] * P[7] - P[3] * P[6]);
@@ -74,12 +107,12 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, c
   E[2 * pg.ntot + p] = -(P[1] * P[6] - P[2] * P[5]);
 }
 
-__global__ __launch_bounds__(kRegimeThreads) void k_regime_q(Geo g, PadGeo pg, Spacing sp, const double *E,
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_q(Geo g, PadGeo pg, PadDiv dv, Spacing sp, const double *E,
                                                              double *Q, int run) {
   const long long p = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
-  padded_pos(pg, p, pos);
+  padded_pos(dv, p, pos);
   bool in = true;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax)
@@ -120,13 +153,13 @@ __device__ __forceinline__ void stiff_flux(int ax, const double P[8], double fb[
 // each padded cell with [shift, N - shift) along AX, zero elsewhere. The axis
 // is a template parameter so every per-axis array index is a constant.
 template <int ORDER, int AX>
-__global__ __launch_bounds__(kRegimeThreads) void k_regime_flux(Geo g, PadGeo pg, const double *V, const double *E,
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_flux(Geo g, PadGeo pg, PadDiv dv, const double *V, const double *E,
                                                                 double *F, int run) {
   constexpr int R = Weno<ORDER>::R;
   const long long p = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
-  padded_pos(pg, p, pos);
+  padded_pos(dv, p, pos);
   if (pos[AX] < R || pos[AX] >= pg.N[AX] - R) {
     for (int v = 0; v < 3; ++v) F[v * pg.ntot + p] = 0.0;
     return;
@@ -286,7 +319,7 @@ __device__ __forceinline__ void regime_d_axis(const RegimeCell &c, const double 
 
 // K (REGIME.cc:293-370), then D = M K per active axis, into the work arrays
 // (the fields the C ABI reads back, and the two-launch source).
-__global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, Spacing sp, const double *V,
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, PadDiv dv, Spacing sp, const double *V,
                                                              const double *E, const double *Q, const double *F,
                                                              double *D, double *Kout, double *Aout, double gam,
                                                              double sig, int run) {
@@ -294,7 +327,7 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, S
   if (p >= pg.ntot) return;
   const long long nt = pg.ntot;
   int pos[3];
-  padded_pos(pg, p, pos);
+  padded_pos(dv, p, pos);
   double K[3];
   regime_k(g, pg, sp, F, p, pos, K);
   RegimeCell c;
@@ -466,14 +499,14 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_source(Geo g, PadGeo 
 
 // dw/ds_b of every padded cell into dw[12*3][cell] (REGIME.cc:226-290), from
 // the fields k_regime_d left (q, alpha).
-__global__ __launch_bounds__(kRegimeThreads) void k_regime_dwdsb(Geo g, PadGeo pg, const double *V, const double *E,
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_dwdsb(Geo g, PadGeo pg, PadDiv dv, const double *V, const double *E,
                                                                  const double *Q, const double *A, double sig,
                                                                  double *out) {
   const long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   const long long nt = pg.ntot;
   int pos[3];
-  padded_pos(pg, p, pos);
+  padded_pos(dv, p, pos);
   double P[8];
   load_prims(V, g.vstride, image_of(g, pg, pos), P);
   const double Ec[3] = {E[p], E[nt + p], E[2 * nt + p]};
@@ -484,6 +517,8 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_dwdsb(Geo g, PadGeo p
 }
 
 unsigned blocks(long long n) { return (unsigned)((n + kRegimeThreads - 1) / kRegimeThreads); }
+
+PadDiv pad_div(const PadGeo &pg) { return {make_fastdiv((unsigned)(pg.N[1] * pg.N[2])), make_fastdiv((unsigned)pg.N[2])}; }
 
 }  // namespace
 
@@ -505,7 +540,8 @@ RegimeFields regime_fields_of(const PadGeo &pg, double *work) {
 void launch_regime_dwdsb(const Geo &g, const PadGeo &pg, const double *V, const double *work, double sigma,
                          double *out, hipStream_t s) {
   const RegimeFields f = regime_fields_of(pg, const_cast<double *>(work));
-  hipLaunchKernelGGL(k_regime_dwdsb, dim3(blocks(pg.ntot)), dim3(kRegimeThreads), 0, s, g, pg, V, f.E, f.q, f.alpha,
+  const PadDiv dv = pad_div(pg);
+  hipLaunchKernelGGL(k_regime_dwdsb, dim3(blocks(pg.ntot)), dim3(kRegimeThreads), 0, s, g, pg, dv, V, f.E, f.q, f.alpha,
                      sigma, out);
 }
 
@@ -515,22 +551,23 @@ void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], co
   const Spacing sp = {{dx[0], dx[1], dx[2]}, {1.0 / (2 * dx[0]), 1.0 / (2 * dx[1]), 1.0 / (2 * dx[2])}};
   const RegimeFields fl = regime_fields_of(pg, work);
   double *E = fl.E, *Q = fl.q, *F = fl.F, *D = fl.D;
-  hipLaunchKernelGGL(k_regime_e, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, V, E);
+  const PadDiv dv = pad_div(pg);
+  hipLaunchKernelGGL(k_regime_e, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, V, E);
   // an eighth of the blocks of one padded (stencil kernels) or interior
   // plane: each XCD then keeps the same rows of every plane, so both its
   // x and its y neighbours stay in its L2
   const int run_pad = (int)std::max<long long>(1, (long long)pg.N[1] * pg.N[2] / kRegimeThreads / 8);
   const int run_int = (int)std::max<long long>(1, (long long)g.n[1] * g.n[2] / kRegimeThreads / 8);
-  hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, E, Q, run_pad);
+  hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, sp, E, Q, run_pad);
   const dim3 grid(blocks(nt)), block(kRegimeThreads);
   if (order == 3) {
-    hipLaunchKernelGGL((k_regime_flux<3, 0>), grid, block, 0, s, g, pg, V, E, F, run_pad);
-    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<3, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt, run_pad);
-    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<3, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt, run_pad);
+    hipLaunchKernelGGL((k_regime_flux<3, 0>), grid, block, 0, s, g, pg, dv, V, E, F, run_pad);
+    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<3, 1>), grid, block, 0, s, g, pg, dv, V, E, F + 3 * nt, run_pad);
+    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<3, 2>), grid, block, 0, s, g, pg, dv, V, E, F + 6 * nt, run_pad);
   } else {
-    hipLaunchKernelGGL((k_regime_flux<5, 0>), grid, block, 0, s, g, pg, V, E, F, run_pad);
-    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<5, 1>), grid, block, 0, s, g, pg, V, E, F + 3 * nt, run_pad);
-    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<5, 2>), grid, block, 0, s, g, pg, V, E, F + 6 * nt, run_pad);
+    hipLaunchKernelGGL((k_regime_flux<5, 0>), grid, block, 0, s, g, pg, dv, V, E, F, run_pad);
+    if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<5, 1>), grid, block, 0, s, g, pg, dv, V, E, F + 3 * nt, run_pad);
+    if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<5, 2>), grid, block, 0, s, g, pg, dv, V, E, F + 6 * nt, run_pad);
   }
   if (fused) {
     if (g.dims == 3) {
@@ -551,7 +588,7 @@ void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], co
     }
     return;
   }
-  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, sp, V, E, Q, F, D, fl.K,
+  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, sp, V, E, Q, F, D, fl.K,
                      fl.alpha, gam, sigma, run_pad);
   hipLaunchKernelGGL(k_regime_source, dim3(blocks(g.vstride)), dim3(kRegimeThreads), 0, s, g, pg, sp, D, src,
                      run_int);

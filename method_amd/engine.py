@@ -298,8 +298,8 @@ class Engine:
         self._check(self.L.mgpu_sync(self._h), "mgpu_sync")
 
     def set_sweep_plan(self, plan):
-        """0: sweeps z, y, x (x finishes); 1 (or True): fused z+y launch, then x;
-        2: sweeps x, y, z (the contiguous axis finishes)."""
+        """3-D grids: 2 (default): sweeps x, y, z (the contiguous axis finishes);
+        0: sweeps z, y, x (x finishes); 1 (or True): fused z+y launch, then x."""
         self._check(self.L.mgpu_set_sweep_plan(self._h, int(plan)), "mgpu_set_sweep_plan")
 
     def set_regime_plan(self, fused: bool):
