@@ -15,8 +15,9 @@ class FVS : public FluxMethod {
     if (weno) data->gpu.wenoOrder = weno->order;
   }
   virtual ~FVS() {}
-  // Reconstructed face fluxes are never formed as a separate array by the
-  // engine; this throws std::logic_error.
+  // The face fluxes at i - 1/2 along dir of the given host arrays
+  // (fluxVectorSplitting.cc:4-56), formed on the device by
+  // mgpu_flux_reconstruction; the resident state is not touched.
   void fluxReconstruction(double *cons, double *prims, double *aux, double *f, double *frecon, int dir,
                           int vars = -1);
   void F(double *cons, double *prims, double *aux, double *f, double *fnet);
