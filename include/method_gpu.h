@@ -315,13 +315,15 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, mgpu_batch_tuning *best,
 int mgpu_batch_timed_steps(mgpu_batch *b, int nsteps, double *total_ms);
 int mgpu_batch_sync(mgpu_batch *b);
 
-/* No counterpart in the reference. Sweep plan of 3-D grids: 2 (default)
- * runs the x, y and z terms of FVS::F (fluxVectorSplitting.cc:79-112) as one
- * launch each in the reference's order, the contiguous z launch finishing
- * the stage; 0 runs z, y, x with the x launch finishing; 1 is plan 0 with
- * the z and y launches fused. Plans 0 and 1 give identical bits; plan 2 sums
- * the axes in the other order. Only launch count, HBM traffic and register
- * pressure differ. 1-D and 2-D grids always sweep as plan 0. */
+/* No counterpart in the reference. Sweep plan of 3-D grids: 3 (default)
+ * sums the x, y and z terms of FVS::F (fluxVectorSplitting.cc:79-112) in the
+ * reference's order with an x launch and then one launch of the y and z
+ * terms that finishes the stage; 2 runs the same as three launches, the
+ * contiguous z launch finishing; 0 runs z, y, x with the x launch finishing;
+ * 1 is plan 0 with the z and y launches fused. Plans 2 and 3 give identical
+ * bits, and so do plans 0 and 1; the two pairs sum the axes in opposite
+ * orders. Only launch count, HBM traffic and register pressure differ. 1-D
+ * and 2-D grids always sweep as plan 0. */
 int mgpu_set_sweep_plan(mgpu_engine *e, int plan);
 
 /* Plan of the REGIME source (no reference counterpart; the reference forms

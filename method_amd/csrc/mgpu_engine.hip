@@ -87,6 +87,7 @@ struct mgpu_engine {
   unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
   int fused_yz = 0;                      // 3-D: fused z+y sweep launch (off: slower here)
   int xyz_order = 1;                     // 3-D: sweeps x, y, z with the last (contiguous) one finishing
+  int fused_yzf = 1;                     // 3-D: x sweep, then the y and z sweeps as one launch
   // SRMHD RK2 and RKSplit steps keep the cons2prim root (v^2, Z) of each cell
   // instead of writing its 13 aux values: no kernel of those steps reads aux,
   // and what does (state download, wavespeed) forms it from the root first.
@@ -173,11 +174,13 @@ SweepArgs sweep_args(mgpu_engine *e, const double *U, const double *V, C2PCounte
   return a;
 }
 
-// The directional sweeps of one stage. 3-D grids sweep x, y, z by default,
-// the reference's summation order (fluxVectorSplitting.cc:79-112), and the
-// contiguous z sweep applies `last_mode`; 1-D and 2-D grids, and 3-D grids
-// under sweep plan 0 or 1, sweep z, y, x with the x sweep applying it (plan 1
-// fuses the z and y sweeps into one launch with the same bits as plan 0).
+// The directional sweeps of one stage. 3-D grids sum the axes in the
+// reference's order, x, y, z (fluxVectorSplitting.cc:79-112): by default an
+// x sweep, then one launch of the y and z sweeps that applies `last_mode`
+// (plan 3), or, under plan 2, three launches with the contiguous z sweep
+// applying it, with the same bits. 1-D and 2-D grids, and 3-D grids under
+// plan 0 or 1, sweep z, y, x with the x sweep applying it (plan 1 fuses the z
+// and y sweeps into one launch with the same bits as plan 0).
 int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
   const int dims = e->geo.dims;
   const int order = e->cfg.weno_order, quirk = e->cfg.quirks;
@@ -201,6 +204,17 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
       {"sweep_y_write", "sweep_y_add", "sweep_y_stage1_c2p", "sweep_y_stage2_c2p", "sweep_y_imex2", "sweep_y_final"},
       {"sweep_z_write", "sweep_z_add", "sweep_z_stage1_c2p", "sweep_z_stage2_c2p", "sweep_z_imex2",
        "sweep_z_final"}};
+  if (dims == 3 && e->fused_yzf) {
+    launch_sweep(a, e->cfg.model, order, 0, kSweepWrite, quirk, e->stream);
+    if (mark(e, names[0][kSweepWrite])) return -1;
+    const int mode = (last_mode == kSweepWrite || last_mode == kSweepAdd) ? kSweepAdd : last_mode;
+    launch_sweep_yzf(a, e->cfg.model, order, mode, quirk, e->stream);
+    static const char *yzf_names[6] = {"sweep_yz_add", "sweep_yz_add", "sweep_yz_stage1_c2p",
+                                       "sweep_yz_stage2_c2p", "sweep_yz_imex2", "sweep_yz_final"};
+    if (mark(e, yzf_names[mode])) return -1;
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
   if (e->xyz_order && dims == 3) {
     // x, then y, then z: the reference's summation order (fluxVectorSplitting.cc:79-112);
     // the last axis, the contiguous one, applies last_mode
@@ -1163,9 +1177,10 @@ int mgpu_max_wavespeed(mgpu_engine *e, double out[3]) {
 
 int mgpu_set_sweep_plan(mgpu_engine *e, int fused_yz) {
   if (!e) return fail("null engine");
-  if (fused_yz < 0 || fused_yz > 2) return fail("mgpu_set_sweep_plan: plan %d out of range", fused_yz);
+  if (fused_yz < 0 || fused_yz > 3) return fail("mgpu_set_sweep_plan: plan %d out of range", fused_yz);
   e->fused_yz = fused_yz == 1;
   e->xyz_order = fused_yz == 2;
+  e->fused_yzf = fused_yz == 3;
   return 0;
 }
 

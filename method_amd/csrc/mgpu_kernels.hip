@@ -142,11 +142,11 @@ __global__ __launch_bounds__(kThreads) void k_flux_stored(Geo g, int model, cons
   }
 }
 
-template <class M, int ORDER, int DAX, int CAX, bool DFAST>
+template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_ = (DFAST ? 32 : 16), int TC_ = (DFAST ? 8 : 16)>
 struct SweepShape {
   static constexpr int R = Weno<ORDER>::R;
-  static constexpr int TL = DFAST ? 32 : 16;  // cells along the sweep axis
-  static constexpr int TC = DFAST ? 8 : 16;   // pencils along the column axis
+  static constexpr int TL = TL_;  // cells along the sweep axis
+  static constexpr int TC = TC_;  // pencils along the column axis
   static constexpr int TLH = TL + 2 * R;      // with halo
   static constexpr int EAX = 3 - DAX - CAX;
   static constexpr int kFaces = (TL + 1) * TC;
@@ -255,6 +255,231 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
 // Stored-flux sweeps only accumulate the net flux (write and add modes).
 template <class M>
 __device__ __forceinline__ void finish_rk2(const SweepArgs &, long long, const double *, StoredM<M>, CellC2P &) {}
+
+// What a sweep does with a cell's net flux fn once its own axis is added
+// (the sweep's mode): keep it for the next sweep, or finish the stage there.
+template <class M, int MODE>
+__device__ __forceinline__ void sweep_apply(const SweepArgs &a, long long q, const double *fn, CellC2P &cs) {
+  constexpr int NV = M::NC;
+  const long long vs = a.geo.vstride;
+  if (MODE == kSweepWrite || MODE == kSweepAdd) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a.fnet[v * vs + q] = fn[v];
+  } else if (MODE == kSweepStage1 || MODE == kSweepStage2) {
+    double un[NV];
+    if (MODE == kSweepStage1) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) un[v] = rk2_stage1(a.U[v * vs + q], fn[v], a.dt);
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) un[v] = rk2_stage2(a.Un[v * vs + q], a.U[v * vs + q], fn[v], a.dt);
+    }
+    finish_rk2(a, q, un, M(), cs);
+  } else if (MODE == kSweepImex2) {
+    // flux1 complete: keep it, then solve stage two from
+    // c = cons - dt (flux1 - om2gam source1) (SSP2.cc:112-148)
+    double c[kSrrmhdCons], x[kSrrmhdCons], A[kSrrmhdAux];
+#pragma unroll
+    for (int v = 0; v < kSrrmhdCons; ++v) {
+      a.F1[v * vs + q] = fn[v];
+      c[v] = a.Un[v * vs + q] - a.dt * fn[v];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) c[8 + m] += a.dt * kSsp2Om2Gam * a.S1[m * vs + q];
+    const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                                     a.Vg[4 * vs + q], a.gamma);
+    const int its = imex_stage(c, a.dt, a.gamma, a.sigma, a.cp, Zg, x);
+    c2p_store_srrmhd(a, q, x, A, cs);
+    if (its < 0) cs.fails = 1;
+    double s5[5];
+    srrmhd_source5(x, A, a.cp, s5);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) a.S2[m * vs + q] = s5[m];
+  } else if (MODE == kSweepImexFinal) {
+    // cons += -dt/2 (flux1 + flux2 - source1 - source2) (SSP2.cc:156-166)
+    double un[kSrrmhdCons], A[kSrrmhdAux];
+#pragma unroll
+    for (int v = 0; v < kSrrmhdCons; ++v) un[v] = a.Un[v * vs + q] - 0.5 * a.dt * (a.F1[v * vs + q] + fn[v]);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) un[8 + m] += 0.5 * a.dt * (a.S1[m * vs + q] + a.S2[m * vs + q]);
+    c2p_store_srrmhd(a, q, un, A, cs);
+  }
+}
+
+// Phases 1 and 2 of a directional sweep on one tile: the split fluxes of the
+// tile's cells and their halo along DAX into LDS, then the WENO value of
+// every face of the tile. On return (after a barrier) s_f holds the face
+// values in S::gidx order: face lf of pencil cc is the left face of tile
+// cell lf. d0 and c0 are the tile's first cell along DAX and CAX, base_e the
+// offset of its plane along the third axis, nd and nc the grid's extents.
+template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_, int TC_, bool QUIRK, int NLDS>
+__device__ __forceinline__ void sweep_tile_faces(const SweepArgs &a, double (&s_f)[NLDS], int d0, int c0,
+                                                 long long base_e, int nd, int nc) {
+  using S = SweepShape<M, ORDER, DAX, CAX, DFAST, TL_, TC_>;
+  constexpr int R = S::R, TC = S::TC, TLH = S::TLH;
+  constexpr int NV = M::NC;
+  const Geo &g = a.geo;
+  const long long vs = g.vstride;
+  const unsigned tid = threadIdx.x;
+
+  if constexpr (S::kSplit) {
+    // Phase 1: split fluxes of every tile cell and its halo along DAX. f+ goes
+    // to LDS now; each thread keeps its cells' f- in registers until the upwind
+    // half of every face is done, then f- takes f+'s place. LDS holds one of
+    // the two at a time, which halves it and lets more blocks share a CU.
+    constexpr int kCells = TLH * TC;
+    constexpr int kCellPasses = (kCells + kThreads - 1) / kThreads;
+    double fm_keep[kCellPasses][NV];
+  #pragma unroll
+    for (int cp = 0; cp < kCellPasses; ++cp) {
+      const unsigned idx = tid + cp * kThreads;
+      if (idx < kCells) {
+        const int l = DFAST ? idx % TLH : idx / TC;
+        const int cc = DFAST ? idx / TLH : idx % TC;
+        const int gc = c0 + cc;
+        double fp[NV];
+        if (gc < nc) {
+          // cells past nd + R - 1 only feed faces outside the grid; clamping
+          // them keeps the periodic image inside [0, nd) for any tile overhang
+          const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
+          const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
+          double u[NV], pl[M::NPL];
+  #pragma unroll
+          for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
+  #pragma unroll
+          for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
+          double f[NV];
+          M::template flux<DAX, QUIRK>(u, pl, a, q, f);
+  #pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            fp[v] = 0.5 * (f[v] + u[v]);
+            fm_keep[cp][v] = 0.5 * (f[v] - u[v]);
+          }
+        } else {
+  #pragma unroll
+          for (int v = 0; v < NV; ++v) fp[v] = fm_keep[cp][v] = 0.0;
+        }
+  #pragma unroll
+        for (int v = 0; v < NV; ++v) s_f[S::fidx(v, l, cc)] = fp[v];
+      }
+    }
+    // likewise the kept f- values are formed before the barrier
+  #pragma unroll
+    for (int cp = 0; cp < kCellPasses; ++cp)
+  #pragma unroll
+      for (int v = 0; v < NV; ++v) asm volatile("" : "+v"(fm_keep[cp][v]));
+    __syncthreads();
+
+    // Phase 2: WENO face values, face lf being the left face of tile cell lf.
+    // The (variable, face) pairs are dealt out one per thread per pass, so the
+    // faces beyond one per thread cost a fraction of a pass spread over the
+    // waves instead of a full pass of all variables on one wave. Upwind halves
+    // from f+ first, then downwind halves from f-.
+    constexpr int kItems = NV * S::kFaces;
+    constexpr int kPasses = (kItems + kThreads - 1) / kThreads;
+    double nu[kPasses], du[kPasses];
+  #pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const unsigned item = tid + k * kThreads;
+      if (item < kItems) {
+        const double *fpv = s_f + S::item_fidx(item);
+        Weno<ORDER>::up([&](int m) { return fpv[m * S::kStrideL]; }, nu[k], du[k]);
+      }
+    }
+    // the upwind halves are formed here, before the barrier after which f-
+    // overwrites f+: left to itself the compiler keeps the loaded f+ stencils
+    // alive across it and forms them later, which doubles the registers
+#pragma unroll
+    for (int k = 0; k < kPasses; ++k) asm volatile("" : "+v"(nu[k]), "+v"(du[k]));
+    __syncthreads();
+  #pragma unroll
+    for (int cp = 0; cp < kCellPasses; ++cp) {
+      const unsigned idx = tid + cp * kThreads;
+      if (idx < kCells) {
+        const int l = DFAST ? idx % TLH : idx / TC;
+        const int cc = DFAST ? idx / TLH : idx % TC;
+  #pragma unroll
+        for (int v = 0; v < NV; ++v) s_f[S::fidx(v, l, cc)] = fm_keep[cp][v];
+      }
+    }
+    __syncthreads();
+  #pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const unsigned item = tid + k * kThreads;
+      if (item < kItems) {
+        const double *fmv = s_f + S::item_fidx(item);
+        double nd, dd;
+        Weno<ORDER>::down([&](int m) { return fmv[m * S::kStrideL]; }, nd, dd);
+        nu[k] = weno_combine(nu[k], du[k], nd, dd);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPasses; ++k) asm volatile("" : "+v"(nu[k]));
+    __syncthreads();
+    // item = v * kFaces + idx, idx enumerating (lf, cc) in gidx order for both
+    // layouts, so the face of item goes to s_f[item]
+  #pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const unsigned item = tid + k * kThreads;
+      if (item < kItems) s_f[item] = nu[k];
+    }
+    __syncthreads();
+  } else {
+    // Phase 1: f+ and f- of every tile cell and its halo side by side in LDS.
+    for (unsigned idx = tid; idx < TLH * TC; idx += kThreads) {
+      const int l = DFAST ? idx % TLH : idx / TC;
+      const int cc = DFAST ? idx / TLH : idx % TC;
+      const int gc = c0 + cc;
+      double fp[NV], fm[NV];
+      if (gc < nc) {
+        const int gd = remap(min(d0 + l - R, nd - 1 + R), nd, g.periodic[DAX]);
+        const long long q = base_e + (long long)gd * g.stride[DAX] + (long long)gc * g.stride[CAX];
+        double u[NV], pl[M::NPL];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[v] = a.U[v * vs + q];
+#pragma unroll
+        for (int v = 0; v < M::NPL; ++v) pl[v] = a.V[(M::P0 + v) * vs + q];
+        double f[NV];
+        M::template flux<DAX, QUIRK>(u, pl, a, q, f);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          fp[v] = 0.5 * (f[v] + u[v]);
+          fm[v] = 0.5 * (f[v] - u[v]);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        s_f[S::fidx(v, l, cc)] = fp[v];
+        s_f[S::fidx(NV + v, l, cc)] = fm[v];
+      }
+    }
+    __syncthreads();
+    constexpr int kItems = NV * S::kFaces;
+    constexpr int kPasses = (kItems + kThreads - 1) / kThreads;
+    double face[kPasses];
+#pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const unsigned item = tid + k * kThreads;
+      if (item < kItems) {
+        const double *fpv = s_f + S::item_fidx(item);
+        const double *fmv = fpv + NV * TLH * TC;
+        face[k] = Weno<ORDER>::face([&](int m) { return fpv[m * S::kStrideL]; },
+                                    [&](int m) { return fmv[m * S::kStrideL]; });
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+      const unsigned item = tid + k * kThreads;
+      if (item < kItems) s_f[item] = face[k];
+    }
+    __syncthreads();
+  }
+
+}
 
 // One directional sweep of the flux divergence of model M.
 template <class M, int ORDER, int DAX, int CAX, bool DFAST, int MODE, bool QUIRK>
@@ -487,6 +712,62 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
         c2p_store_srrmhd(a, q, un, A, cs);
       }
     }
+  }
+  if (MODE >= kSweepStage1) reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
+}
+
+// The y and z terms of a 3-D stage in one launch, after an x sweep has
+// written the x term (sweep plan 3): a 16 x 16 (y, z) tile of one x plane
+// reconstructs its y faces, then its z faces, through the same LDS, each
+// phase exactly as a separate sweep does it. Each thread keeps its cell's net
+// flux with the y term added in registers between the two phases, and the z
+// phase applies the sweep's mode. The sums are formed in the separate sweeps'
+// order, so plan 3 gives plan 2's bits, and the y-added net flux never makes
+// an HBM round trip.
+template <class M, int ORDER, int MODE, bool QUIRK>
+__global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
+  using SY = SweepShape<M, ORDER, 1, 2, false, 16, 16>;
+  using SZ = SweepShape<M, ORDER, 2, 1, true, 16, 16>;
+  constexpr int NV = M::NC;
+  constexpr int kLds = SY::kLds > SZ::kLds ? SY::kLds : SZ::kLds;
+  __shared__ double s_f[kLds];
+  __shared__ double s_acc[NV][kThreads];  // each cell's net flux with the y term added
+  const Geo &g = a.geo;
+  const int ny = g.n[1], nz = g.n[2];
+  const int tiles_y = (ny + 15) / 16, tiles_z = (nz + 15) / 16;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int tz = lb % tiles_z;
+  const int rest = lb / tiles_z;
+  const int ty = rest % tiles_y;
+  const int i = rest / tiles_y;
+  const int y0 = ty * 16, z0 = tz * 16;
+  const long long base_e = (long long)i * g.stride[0];
+  const long long vs = g.vstride;
+  const unsigned tid = threadIdx.x;
+
+  // y phase: faces along y, pencils along z; the thread's cell is (y0 + r, z0 + c)
+  sweep_tile_faces<M, ORDER, 1, 2, false, 16, 16, false>(a, s_f, y0, z0, base_e, ny, nz);
+  const int r = tid / 16, c = tid % 16;
+  const bool live = y0 + r < ny && z0 + c < nz;
+  const long long q = base_e + (long long)(y0 + r) * g.stride[1] + (z0 + c);
+  if (live) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      s_acc[v][tid] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1], a.fnet[v * vs + q]);
+  }
+  __syncthreads();  // every y face is read before the z phase refills LDS
+  // the z phase's loads stay after the y phase, so its registers are free
+  __builtin_amdgcn_sched_barrier(0);
+
+  // z phase: faces along z, pencils along y; the same cell is (tile cell c, pencil r)
+  sweep_tile_faces<M, ORDER, 2, 1, true, 16, 16, QUIRK>(a, s_f, z0, y0, base_e, nz, ny);
+  CellC2P cs;
+  if (live) {
+    double fn[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      fn[v] = fma(s_f[SZ::gidx(v, c + 1, r)] - s_f[SZ::gidx(v, c, r)], g.inv_d[2], s_acc[v][tid]);
+    sweep_apply<M, MODE>(a, q, fn, cs);
   }
   if (MODE >= kSweepStage1) reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
 }
@@ -1288,6 +1569,40 @@ void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStr
   } else {
     if (order == 3) launch_sweep_yz_m<SrrmhdM, 3>(a, quirk, s);
     else launch_sweep_yz_m<SrrmhdM, 5>(a, quirk, s);
+  }
+}
+
+template <class M, int ORDER, int MODE>
+void launch_sweep_yzf_o(const SweepArgs &a, int quirk, hipStream_t s) {
+  const int tiles = ((a.geo.n[1] + 15) / 16) * ((a.geo.n[2] + 15) / 16) * a.geo.n[0];
+  if (quirk)
+    hipLaunchKernelGGL((k_sweep_yzf<M, ORDER, MODE, true>), dim3(tiles), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep_yzf<M, ORDER, MODE, false>), dim3(tiles), dim3(kThreads), 0, s, a);
+}
+
+template <class M, int ORDER>
+void launch_sweep_yzf_m(const SweepArgs &a, int mode, int quirk, hipStream_t s) {
+  switch (mode) {
+    case kSweepAdd: launch_sweep_yzf_o<M, ORDER, kSweepAdd>(a, quirk, s); break;
+    case kSweepStage1: launch_sweep_yzf_o<M, ORDER, kSweepStage1>(a, quirk, s); break;
+    case kSweepStage2: launch_sweep_yzf_o<M, ORDER, kSweepStage2>(a, quirk, s); break;
+    default:
+      if constexpr (M::NC == kSrrmhdCons) {
+        if (mode == kSweepImex2) launch_sweep_yzf_o<M, ORDER, kSweepImex2>(a, quirk, s);
+        else if (mode == kSweepImexFinal) launch_sweep_yzf_o<M, ORDER, kSweepImexFinal>(a, quirk, s);
+      }
+      break;
+  }
+}
+
+void launch_sweep_yzf(const SweepArgs &a, int model, int order, int mode, int quirk, hipStream_t s) {
+  if (model == 0) {
+    if (order == 3) launch_sweep_yzf_m<SrmhdM, 3>(a, mode, quirk, s);
+    else launch_sweep_yzf_m<SrmhdM, 5>(a, mode, quirk, s);
+  } else {
+    if (order == 3) launch_sweep_yzf_m<SrrmhdM, 3>(a, mode, quirk, s);
+    else launch_sweep_yzf_m<SrrmhdM, 5>(a, mode, quirk, s);
   }
 }
 

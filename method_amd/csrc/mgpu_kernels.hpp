@@ -67,6 +67,9 @@ void launch_flux_stored(const Geo &g, int model, const double *U, const double *
                         int quirk, hipStream_t s);
 // Fused z and y sweeps of a 3-D grid: writes their summed net flux to a.fnet.
 void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStream_t s);
+// The y and z sweeps of a 3-D grid in one launch after the x sweep wrote
+// a.fnet; mode is the z sweep's (kSweepAdd leaves the net flux in a.fnet).
+void launch_sweep_yzf(const SweepArgs &a, int model, int order, int mode, int quirk, hipStream_t s);
 void launch_imex1_srrmhd(const SweepArgs &a, hipStream_t s);
 void launch_prims_to_all_srrmhd(const Geo &g, const double *V, double *U, double *A, double gam, double sigma,
                                 hipStream_t s);

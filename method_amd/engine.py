@@ -298,8 +298,10 @@ class Engine:
         self._check(self.L.mgpu_sync(self._h), "mgpu_sync")
 
     def set_sweep_plan(self, plan):
-        """3-D grids: 2 (default): sweeps x, y, z (the contiguous axis finishes);
-        0: sweeps z, y, x (x finishes); 1 (or True): fused z+y launch, then x."""
+        """3-D grids: 3 (default): an x sweep, then y and z in one launch that
+        finishes the stage; 2: sweeps x, y, z (the contiguous z finishes), the
+        same bits as 3; 0: sweeps z, y, x (x finishes); 1 (or True): fused z+y
+        launch, then x, the same bits as 0."""
         self._check(self.L.mgpu_set_sweep_plan(self._h, int(plan)), "mgpu_set_sweep_plan")
 
     def set_regime_plan(self, fused: bool):

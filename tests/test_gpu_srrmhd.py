@@ -167,6 +167,24 @@ def test_fused_yz_sweep_equals_separate_sweeps_bitwise():
         assert np.array_equal(a, b)
 
 
+def test_fused_yz_finishing_sweep_equals_separate_sweeps_bitwise():
+    """Sweep plan 3 with the resistive model under SSP2 (its implicit stage-two
+    and final modes run in the fused y+z launch) gives plan 2's bits."""
+    g, p0 = ot3d_prims(20)
+    grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
+    say("gpu: SRRMHD fused y+z finishing sweep vs plan 2")
+    out = []
+    for plan in (3, 2):
+        with engine(weno=5, **grid) as e:
+            e.set_sweep_plan(plan)
+            e.set_prims(p0)
+            for _ in range(2):
+                assert e.update_time() == 0
+            out.append(e.download())
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 def test_rksplit_matches_oracle():
     """RKSplit with the resistive model (explicit split source, sigma 10)."""
     g = idt.ot2d_grid(24)
