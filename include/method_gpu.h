@@ -322,8 +322,10 @@ int mgpu_batch_sync(mgpu_batch *b);
  * contiguous z launch finishing; 0 runs z, y, x with the x launch finishing;
  * 1 is plan 0 with the z and y launches fused. Plans 2 and 3 give identical
  * bits, and so do plans 0 and 1; the two pairs sum the axes in opposite
- * orders. Only launch count, HBM traffic and register pressure differ. 1-D
- * and 2-D grids always sweep as plan 0. */
+ * orders. Only launch count, HBM traffic and register pressure differ. On
+ * 2-D grids plan 3 (default) runs the y and x terms as one launch that
+ * finishes the stage, and plans 0-2 as a y launch and a finishing x launch,
+ * with identical bits; 1-D grids always run their one x launch. */
 int mgpu_set_sweep_plan(mgpu_engine *e, int plan);
 
 /* Plan of the REGIME source (no reference counterpart; the reference forms

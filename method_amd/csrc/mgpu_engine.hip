@@ -87,7 +87,7 @@ struct mgpu_engine {
   unsigned long long *dspeed = nullptr;  // max-wavespeed result, 3 entries
   int fused_yz = 0;                      // 3-D: fused z+y sweep launch (off: slower here)
   int xyz_order = 1;                     // 3-D: sweeps x, y, z with the last (contiguous) one finishing
-  int fused_yzf = 1;                     // 3-D: x sweep, then the y and z sweeps as one launch
+  int fused_yzf = 1;                     // plan 3: 3-D x sweep, then y + z as one launch; 2-D y + x as one
   // SRMHD RK2 and RKSplit steps keep the cons2prim root (v^2, Z) of each cell
   // instead of writing its 13 aux values: no kernel of those steps reads aux,
   // and what does (state download, wavespeed) forms it from the root first.
@@ -180,7 +180,8 @@ SweepArgs sweep_args(mgpu_engine *e, const double *U, const double *V, C2PCounte
 // (plan 3), or, under plan 2, three launches with the contiguous z sweep
 // applying it, with the same bits. 1-D and 2-D grids, and 3-D grids under
 // plan 0 or 1, sweep z, y, x with the x sweep applying it (plan 1 fuses the z
-// and y sweeps into one launch with the same bits as plan 0).
+// and y sweeps into one launch with the same bits as plan 0); on 2-D grids
+// plan 3 runs the y and x sweeps as one launch, with the same bits.
 int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
   const int dims = e->geo.dims;
   const int order = e->cfg.weno_order, quirk = e->cfg.quirks;
@@ -212,6 +213,15 @@ int run_sweeps(mgpu_engine *e, SweepArgs a, int last_mode) {
     static const char *yzf_names[6] = {"sweep_yz_add", "sweep_yz_add", "sweep_yz_stage1_c2p",
                                        "sweep_yz_stage2_c2p", "sweep_yz_imex2", "sweep_yz_final"};
     if (mark(e, yzf_names[mode])) return -1;
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  if (dims == 2 && e->fused_yzf) {
+    const int mode = (last_mode == kSweepWrite || last_mode == kSweepAdd) ? kSweepAdd : last_mode;
+    launch_sweep_xyf(a, e->cfg.model, order, mode, e->stream);
+    static const char *xyf_names[6] = {"sweep_yx_add", "sweep_yx_add", "sweep_yx_stage1_c2p",
+                                       "sweep_yx_stage2_c2p", "sweep_yx_imex2", "sweep_yx_final"};
+    if (mark(e, xyf_names[mode])) return -1;
     HIP_OK(hipGetLastError());
     return 0;
   }

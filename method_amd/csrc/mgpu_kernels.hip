@@ -165,7 +165,6 @@ struct SweepShape {
     return DFAST ? (v * TC + c) * (TL + 1) + lf : (v * (TL + 1) + lf) * TC + c;
   }
   static constexpr int kStrideL = DFAST ? 1 : TC;  // LDS step between neighbouring cells of a pencil
-  static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
   // fidx(v, lf, cc) of work item v * kFaces + idx, idx enumerating (lf, cc) in
   // gidx order: for the strided layout idx = lf * TC + cc already is the
   // offset within the variable; for the contiguous one each pencil skips the
@@ -486,6 +485,7 @@ template <class M, int ORDER, int DAX, int CAX, bool DFAST, int MODE, bool QUIRK
 __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
   using S = SweepShape<M, ORDER, DAX, CAX, DFAST>;
   constexpr int R = S::R, TL = S::TL, TC = S::TC, TLH = S::TLH, EAX = S::EAX;
+  static_assert(TL * TC == kThreads, "one cell per thread in the difference phase");
   constexpr int NV = M::NC;
   // f+ (then f-) of each variable [NV][TLH][TC]; reused for face fluxes.
   __shared__ double s_f[S::kLds];
@@ -718,20 +718,26 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
 
 // The y and z terms of a 3-D stage in one launch, after an x sweep has
 // written the x term (sweep plan 3): a 16 x 16 (y, z) tile of one x plane
-// reconstructs its y faces, then its z faces, through the same LDS, each
-// phase exactly as a separate sweep does it. Each thread keeps its cell's net
-// flux with the y term added in registers between the two phases, and the z
-// phase applies the sweep's mode. The sums are formed in the separate sweeps'
-// order, so plan 3 gives plan 2's bits, and the y-added net flux never makes
-// an HBM round trip.
+// reconstructs its y faces, then its z faces, each phase exactly as a
+// separate sweep does it. Each cell's net flux with the y term added waits in
+// LDS for the z term, and the z phase then applies the sweep's mode. The sums
+// are formed in the separate sweeps' order, so plan 3 gives plan 2's bits,
+// and the y-added net flux never makes an HBM round trip. SRRMHD's 14
+// variables take each phase over the tile's pencils in two groups of 8, so
+// that a thread keeps the split fluxes of one tile-or-halo cell and the face
+// values of about five (variable, face) items at a time (174-203 VGPRs where
+// the whole tile takes 184-256); SRMHD's 9 variables run faster as one group
+// of 16 (123-156 VGPRs, 3 waves per SIMD): in groups of 8 they fit 4 waves
+// per SIMD but took 1 % longer.
 template <class M, int ORDER, int MODE, bool QUIRK>
 __global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
-  using SY = SweepShape<M, ORDER, 1, 2, false, 16, 16>;
-  using SZ = SweepShape<M, ORDER, 2, 1, true, 16, 16>;
+  constexpr int kGroup = M::NC > 9 ? 8 : 16;  // pencils per group
+  using SY = SweepShape<M, ORDER, 1, 2, false, 16, kGroup>;
+  using SZ = SweepShape<M, ORDER, 2, 1, true, 16, kGroup>;
   constexpr int NV = M::NC;
   constexpr int kLds = SY::kLds > SZ::kLds ? SY::kLds : SZ::kLds;
   __shared__ double s_f[kLds];
-  __shared__ double s_acc[NV][kThreads];  // each cell's net flux with the y term added
+  __shared__ double s_acc[NV][kThreads];  // each cell's net flux, y term added, then z
   const Geo &g = a.geo;
   const int ny = g.n[1], nz = g.n[2];
   const int tiles_y = (ny + 15) / 16, tiles_z = (nz + 15) / 16;
@@ -744,30 +750,146 @@ __global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
   const long long base_e = (long long)i * g.stride[0];
   const long long vs = g.vstride;
   const unsigned tid = threadIdx.x;
-
-  // y phase: faces along y, pencils along z; the thread's cell is (y0 + r, z0 + c)
-  sweep_tile_faces<M, ORDER, 1, 2, false, 16, 16, false>(a, s_f, y0, z0, base_e, ny, nz);
+  // the thread's cell is (y0 + r, z0 + c)
   const int r = tid / 16, c = tid % 16;
   const bool live = y0 + r < ny && z0 + c < nz;
   const long long q = base_e + (long long)(y0 + r) * g.stride[1] + (z0 + c);
-  if (live) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-      s_acc[v][tid] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1], a.fnet[v * vs + q]);
-  }
-  __syncthreads();  // every y face is read before the z phase refills LDS
-  // the z phase's loads stay after the y phase, so its registers are free
-  __builtin_amdgcn_sched_barrier(0);
 
-  // z phase: faces along z, pencils along y; the same cell is (tile cell c, pencil r)
-  sweep_tile_faces<M, ORDER, 2, 1, true, 16, 16, QUIRK>(a, s_f, z0, y0, base_e, nz, ny);
   CellC2P cs;
-  if (live) {
-    double fn[NV];
+  if constexpr (kGroup == 16) {
+    // y phase over the whole tile, pencils along z
+    sweep_tile_faces<M, ORDER, 1, 2, false, 16, 16, false>(a, s_f, y0, z0, base_e, ny, nz);
+    if (live) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
-      fn[v] = fma(s_f[SZ::gidx(v, c + 1, r)] - s_f[SZ::gidx(v, c, r)], g.inv_d[2], s_acc[v][tid]);
-    sweep_apply<M, MODE>(a, q, fn, cs);
+      for (int v = 0; v < NV; ++v)
+        s_acc[v][tid] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1], a.fnet[v * vs + q]);
+    }
+    __syncthreads();  // every y face is read before the z phase refills LDS
+    // the z phase's loads stay after the y phase, so its registers are free
+    __builtin_amdgcn_sched_barrier(0);
+    // z phase, pencils along y: the thread's cell is tile cell c of pencil r,
+    // and it finishes straight from the faces in LDS
+    sweep_tile_faces<M, ORDER, 2, 1, true, 16, 16, QUIRK>(a, s_f, z0, y0, base_e, nz, ny);
+    if (live) {
+      double fn[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        fn[v] = fma(s_f[SZ::gidx(v, c + 1, r)] - s_f[SZ::gidx(v, c, r)], g.inv_d[2], s_acc[v][tid]);
+      sweep_apply<M, MODE>(a, q, fn, cs);
+    }
+  } else {
+    // y phase: faces along y; group h holds the kGroup pencils from z0 + kGroup h
+#pragma unroll 1
+    for (int h = 0; h < 16 / kGroup; ++h) {
+      sweep_tile_faces<M, ORDER, 1, 2, false, 16, kGroup, false>(a, s_f, y0, z0 + h * kGroup, base_e, ny, nz);
+      const int cg = c - h * kGroup;
+      if (live && cg >= 0 && cg < kGroup) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          s_acc[v][tid] = fma(s_f[SY::gidx(v, r + 1, cg)] - s_f[SY::gidx(v, r, cg)], g.inv_d[1], a.fnet[v * vs + q]);
+      }
+      __syncthreads();  // every face is read before LDS is refilled
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // z phase: faces along z; group h holds the kGroup pencils from y0 + kGroup h,
+    // whose cells belong to the 16 kGroup threads from 16 kGroup h
+#pragma unroll 1
+    for (int h = 0; h < 16 / kGroup; ++h) {
+      sweep_tile_faces<M, ORDER, 2, 1, true, 16, kGroup, QUIRK>(a, s_f, z0, y0 + h * kGroup, base_e, nz, ny);
+      const int rg = r - h * kGroup;
+      if (live && rg >= 0 && rg < kGroup) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          s_acc[v][tid] = fma(s_f[SZ::gidx(v, c + 1, rg)] - s_f[SZ::gidx(v, c, rg)], g.inv_d[2], s_acc[v][tid]);
+      }
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (live) {
+      double fn[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fn[v] = s_acc[v][tid];
+      sweep_apply<M, MODE>(a, q, fn, cs);
+    }
+  }
+  if (MODE >= kSweepStage1) reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
+}
+
+// The y and x terms of a 2-D stage in one launch (sweep plan 3 on 2-D
+// grids): a 16 x 16 (x, y) tile reconstructs its y faces, then its x faces,
+// each phase exactly as the separate y and x sweeps do it. Each cell's y net
+// flux waits in LDS for the x term, which is added as the separate x sweep
+// adds it (one fused multiply-add), and the x phase then applies the sweep's
+// mode; so the launch gives the separate sweeps' bits, and the y net flux
+// never makes an HBM round trip. The y phase runs along the contiguous axis.
+// As in k_sweep_yzf, SRRMHD takes each phase in two groups of 8 pencils.
+template <class M, int ORDER, int MODE>
+__global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
+  constexpr int kGroup = M::NC > 9 ? 8 : 16;  // pencils per group
+  using SY = SweepShape<M, ORDER, 1, 0, true, 16, kGroup>;
+  using SX = SweepShape<M, ORDER, 0, 1, false, 16, kGroup>;
+  constexpr int NV = M::NC;
+  constexpr int kLds = SY::kLds > SX::kLds ? SY::kLds : SX::kLds;
+  __shared__ double s_f[kLds];
+  __shared__ double s_acc[NV][kThreads];  // each cell's y net flux, then with the x term
+  const Geo &g = a.geo;
+  const int nx = g.n[0], ny = g.n[1];
+  const int tiles_y = (ny + 15) / 16;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int ty = lb % tiles_y, tx = lb / tiles_y;
+  const int x0 = tx * 16, y0 = ty * 16;
+  const unsigned tid = threadIdx.x;
+  // the thread's cell is (x0 + r, y0 + c): neighbouring threads along y, the
+  // contiguous axis
+  const int r = tid / 16, c = tid % 16;
+  const bool live = x0 + r < nx && y0 + c < ny;
+  const long long q = (long long)(x0 + r) * g.stride[0] + (long long)(y0 + c) * g.stride[1];
+
+  // y phase: faces along y; group h holds the kGroup x pencils from x0 + kGroup h,
+  // whose cells belong to the 16 kGroup threads from 16 kGroup h
+#pragma unroll 1
+  for (int h = 0; h < 16 / kGroup; ++h) {
+    sweep_tile_faces<M, ORDER, 1, 0, true, 16, kGroup, false>(a, s_f, y0, x0 + h * kGroup, 0, ny, nx);
+    const int rg = r - h * kGroup;
+    if (live && rg >= 0 && rg < kGroup) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_acc[v][tid] = (s_f[SY::gidx(v, c + 1, rg)] - s_f[SY::gidx(v, c, rg)]) * g.inv_d[1];
+    }
+    __syncthreads();  // every face is read before LDS is refilled
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  CellC2P cs;
+  if constexpr (kGroup == 16) {
+    // x phase over the whole tile, pencils along y: the thread's cell is tile
+    // cell r of pencil c, and it finishes straight from the faces in LDS
+    sweep_tile_faces<M, ORDER, 0, 1, false, 16, 16, false>(a, s_f, x0, y0, 0, nx, ny);
+    if (live) {
+      double fn[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        fn[v] = fma(s_f[SX::gidx(v, r + 1, c)] - s_f[SX::gidx(v, r, c)], g.inv_d[0], s_acc[v][tid]);
+      sweep_apply<M, MODE>(a, q, fn, cs);
+    }
+  } else {
+    // x phase: faces along x; group h holds the kGroup y pencils from y0 + kGroup h
+#pragma unroll 1
+    for (int h = 0; h < 16 / kGroup; ++h) {
+      sweep_tile_faces<M, ORDER, 0, 1, false, 16, kGroup, false>(a, s_f, x0, y0 + h * kGroup, 0, nx, ny);
+      const int cg = c - h * kGroup;
+      if (live && cg >= 0 && cg < kGroup) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          s_acc[v][tid] = fma(s_f[SX::gidx(v, r + 1, cg)] - s_f[SX::gidx(v, r, cg)], g.inv_d[0], s_acc[v][tid]);
+      }
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (live) {
+      double fn[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fn[v] = s_acc[v][tid];
+      sweep_apply<M, MODE>(a, q, fn, cs);
+    }
   }
   if (MODE >= kSweepStage1) reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
 }
@@ -1603,6 +1725,37 @@ void launch_sweep_yzf(const SweepArgs &a, int model, int order, int mode, int qu
   } else {
     if (order == 3) launch_sweep_yzf_m<SrrmhdM, 3>(a, mode, quirk, s);
     else launch_sweep_yzf_m<SrrmhdM, 5>(a, mode, quirk, s);
+  }
+}
+
+template <class M, int ORDER, int MODE>
+void launch_sweep_xyf_o(const SweepArgs &a, hipStream_t s) {
+  const int tiles = ((a.geo.n[0] + 15) / 16) * ((a.geo.n[1] + 15) / 16);
+  hipLaunchKernelGGL((k_sweep_xyf<M, ORDER, MODE>), dim3(tiles), dim3(kThreads), 0, s, a);
+}
+
+template <class M, int ORDER>
+void launch_sweep_xyf_m(const SweepArgs &a, int mode, hipStream_t s) {
+  switch (mode) {
+    case kSweepAdd: launch_sweep_xyf_o<M, ORDER, kSweepAdd>(a, s); break;
+    case kSweepStage1: launch_sweep_xyf_o<M, ORDER, kSweepStage1>(a, s); break;
+    case kSweepStage2: launch_sweep_xyf_o<M, ORDER, kSweepStage2>(a, s); break;
+    default:
+      if constexpr (M::NC == kSrrmhdCons) {
+        if (mode == kSweepImex2) launch_sweep_xyf_o<M, ORDER, kSweepImex2>(a, s);
+        else if (mode == kSweepImexFinal) launch_sweep_xyf_o<M, ORDER, kSweepImexFinal>(a, s);
+      }
+      break;
+  }
+}
+
+void launch_sweep_xyf(const SweepArgs &a, int model, int order, int mode, hipStream_t s) {
+  if (model == 0) {
+    if (order == 3) launch_sweep_xyf_m<SrmhdM, 3>(a, mode, s);
+    else launch_sweep_xyf_m<SrmhdM, 5>(a, mode, s);
+  } else {
+    if (order == 3) launch_sweep_xyf_m<SrrmhdM, 3>(a, mode, s);
+    else launch_sweep_xyf_m<SrrmhdM, 5>(a, mode, s);
   }
 }
 

@@ -70,6 +70,9 @@ void launch_sweep_yz(const SweepArgs &a, int model, int order, int quirk, hipStr
 // The y and z sweeps of a 3-D grid in one launch after the x sweep wrote
 // a.fnet; mode is the z sweep's (kSweepAdd leaves the net flux in a.fnet).
 void launch_sweep_yzf(const SweepArgs &a, int model, int order, int mode, int quirk, hipStream_t s);
+// The y and x sweeps of a 2-D grid in one launch; mode is the x sweep's
+// (kSweepAdd writes the net flux to a.fnet).
+void launch_sweep_xyf(const SweepArgs &a, int model, int order, int mode, hipStream_t s);
 void launch_imex1_srrmhd(const SweepArgs &a, hipStream_t s);
 void launch_prims_to_all_srrmhd(const Geo &g, const double *V, double *U, double *A, double gam, double sigma,
                                 hipStream_t s);

@@ -301,7 +301,9 @@ class Engine:
         """3-D grids: 3 (default): an x sweep, then y and z in one launch that
         finishes the stage; 2: sweeps x, y, z (the contiguous z finishes), the
         same bits as 3; 0: sweeps z, y, x (x finishes); 1 (or True): fused z+y
-        launch, then x, the same bits as 0."""
+        launch, then x, the same bits as 0. 2-D grids: 3 runs y and x as one
+        launch that finishes the stage, 0-2 a y sweep and an x sweep, all with
+        the same bits."""
         self._check(self.L.mgpu_set_sweep_plan(self._h, int(plan)), "mgpu_set_sweep_plan")
 
     def set_regime_plan(self, fused: bool):

@@ -167,15 +167,17 @@ def test_fused_yz_sweep_equals_separate_sweeps_bitwise():
         assert np.array_equal(a, b)
 
 
-def test_fused_yz_finishing_sweep_equals_separate_sweeps_bitwise():
-    """Sweep plan 3 with the resistive model under SSP2 (its implicit stage-two
-    and final modes run in the fused y+z launch) gives plan 2's bits."""
+@pytest.mark.parametrize("weno, integrator", [(5, eng.SSP2), (3, eng.SSP2), (3, eng.RK2), (5, eng.RK2)])
+def test_fused_yz_finishing_sweep_equals_separate_sweeps_bitwise(weno, integrator):
+    """Sweep plan 3 with the resistive model (its fused y+z launch runs each
+    phase in two groups of 8 pencils) under SSP2 (the implicit stage-two and
+    final modes) and RK2 gives plan 2's bits."""
     g, p0 = ot3d_prims(20)
     grid = dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0)
-    say("gpu: SRRMHD fused y+z finishing sweep vs plan 2")
+    say(f"gpu: SRRMHD fused y+z finishing sweep vs plan 2, WENO{weno} integrator {integrator}")
     out = []
     for plan in (3, 2):
-        with engine(weno=5, **grid) as e:
+        with engine(integrator=integrator, weno=weno, **grid) as e:
             e.set_sweep_plan(plan)
             e.set_prims(p0)
             for _ in range(2):
@@ -183,6 +185,31 @@ def test_fused_yz_finishing_sweep_equals_separate_sweeps_bitwise():
             out.append(e.download())
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n, weno, integrator, bc", [((23, 20), 3, eng.SSP2, (eng.PERIODIC,) * 3),
+                                                    ((32, 32), 3, eng.SSP2, (eng.PERIODIC,) * 3),
+                                                    ((20, 37), 5, eng.SSP2, (eng.OUTFLOW,) * 3),
+                                                    ((19, 26), 5, eng.RK2, (eng.PERIODIC,) * 3)])
+def test_fused_yx_finishing_sweep_2d_equals_separate_sweeps_bitwise(n, weno, integrator, bc):
+    """On 2-D grids plan 3 runs the y and x sweeps as one launch (in two groups
+    of 8 pencils for the resistive model); it must give plan 0's bits under
+    SSP2's implicit modes and RK2, on ragged grids."""
+    nx, ny = n
+    g = idt.Grid(nx, ny, 0, 0.0, 1.0, 0.0, 1.0)
+    p0 = idt.ot_prims_resistive(g)
+    grid = dict(nx=nx, ny=ny, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0)
+    say(f"gpu: SRRMHD 2-D fused y+x finishing sweep vs plan 0, {n} WENO{weno} integrator {integrator} bc={bc}")
+    out = []
+    for plan in (3, 0):
+        with engine(integrator=integrator, weno=weno, bc=bc, **grid) as e:
+            e.set_sweep_plan(plan)
+            e.set_prims(p0)
+            for _ in range(3):
+                assert e.update_time() == 0
+            out.append(e.download())
+    for a, b, name in zip(out[0], out[1], ("cons", "prims", "aux")):
+        assert np.array_equal(a, b), f"{name}: max diff {np.max(np.abs(a - b)):.3e}"
 
 
 def test_rksplit_matches_oracle():
