@@ -70,7 +70,8 @@ def parse():
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched-ensemble leg and print its JSON (for profiling its kernels)")
     ap.add_argument("--sweep-plan", type=int, default=None,
-                    help="sweep plan of the headline engine (mgpu_set_sweep_plan); default: the engine's")
+                    help="sweep plan of the headline engine and the SRRMHD leg (mgpu_set_sweep_plan); "
+                         "default: the engine's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -274,8 +275,9 @@ def learned_guess_error(e, params, scales, prims):
             "p_rel_err_p90": float(np.quantile(ep, 0.9))}
 
 
-def srrmhd_leg(steps: int):
-    """Resistive SRMHD Orszag-Tang 1024^2, sigma 1e3, SSP2 IMEX, WENO3."""
+def srrmhd_leg(steps: int, plan=None):
+    """Resistive SRMHD Orszag-Tang 1024^2, sigma 1e3, SSP2 IMEX, WENO3 (sweep
+    plan `plan` if given, else the engine's default)."""
     from method_amd import engine as eng
     from method_amd import initdata as idt
 
@@ -283,6 +285,8 @@ def srrmhd_leg(steps: int):
     cfg = eng.Config(nx=1024, ny=1024, nz=0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, model=eng.SRRMHD,
                      integrator=eng.SSP2, weno_order=3, bc=(eng.PERIODIC,) * 3, sigma=1e3, cp=0.1, cfl=0.5)
     with eng.Engine(cfg) as e:
+        if plan is not None:
+            e.set_sweep_plan(plan)
         e.set_prims(idt.ot_prims_resistive(g))
         for _ in range(2):
             e.update_time()
@@ -422,7 +426,7 @@ def main():
         print(json.dumps({"batched_ensemble": batch_leg(args.batch_steps, args.batch_members)}))
         return
     if args.srrmhd_only:
-        print(json.dumps({"srrmhd_ssp2": srrmhd_leg(args.srrmhd_steps)}))
+        print(json.dumps({"srrmhd_ssp2": srrmhd_leg(args.srrmhd_steps, args.sweep_plan)}))
         return
     if args.regime_only:
         print(json.dumps({"regime_rksplit": regime_leg(args.regime_n, args.regime_steps)}))
@@ -528,7 +532,7 @@ def main():
     e.close()
     if not args.no_srrmhd:
         print("bench: SRRMHD SSP2 1024^2", file=sys.stderr, flush=True)
-        out["srrmhd_ssp2"] = srrmhd_leg(args.srrmhd_steps)
+        out["srrmhd_ssp2"] = srrmhd_leg(args.srrmhd_steps, args.sweep_plan)
     if not args.no_regime:
         print(f"bench: SRMHD + REGIME RKSplit {args.regime_n}^3", file=sys.stderr, flush=True)
         out["regime_rksplit"] = regime_leg(args.regime_n, args.regime_steps)
