@@ -8,12 +8,16 @@
 // reach the first and last interior cells through the centred differences,
 // so the kernels here work on the same padded index space: a padded cell
 // reads its primitives from the interior cell Bcs::apply copies there, and
-// every range test is the reference's. Five launches over padded cells:
+// every range test is the reference's. The quantities, over padded cells:
 //   E = -v x B                                    (REGIME.cc:150-154)
 //   q = 4th-order centred div E where it fits     (REGIME.cc:186-212)
 //   per axis, the WENO face flux of (E, B-flux)   (REGIME.cc:293-370)
 //   K, then per axis D = M K, M = -df/dw dw/ds_b  (REGIME.cc:226-290,551-774)
 //   source = centred difference of D, interior    (REGIME.cc:92-138)
+// The default plan is two launches: k_regime_fq (every axis' face fluxes
+// and q, E formed from the primitives where it is used), then k_regime_ds
+// (K and D on chip, differenced into the source). The field-keeping plan
+// stores each quantity from its own launch for mgpu_regime_fields.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -95,16 +99,26 @@ __device__ __forceinline__ void load_prims(const double *V, long long vs, long l
   for (int v = 0; v < 8; ++v) P[v] = V[v * vs + q];
 }
 
+// E = -v x B from one cell's primitives (REGIME.cc:150-154). Every kernel
+// forms E through this function, so a value formed on the fly equals the
+// stored one bit for bit.
+__device__ __forceinline__ void regime_e_of(const double P[8], double E[3]) {
+  E[0] = -(P[2] * P[7] - P[3] * P[6]);
+  E[1] = -(P[3] * P[5] - P[1] * P[7]);
+  E[2] = -(P[1] * P[6] - P[2] * P[5]);
+}
+
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_e(Geo g, PadGeo pg, PadDiv dv, const double *V, double *E) {
   const long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * kRegimeThreads + threadIdx.x;
   if (p >= pg.ntot) return;
   int pos[3];
   padded_pos(dv, p, pos);
-  double P[8];
+  double P[8], e[3];
   load_prims(V, g.vstride, image_of(g, pg, pos), P);
-  E[p] = -(P[2] * P[7] - P[3] * P[6]);
-  E[pg.ntot + p] = -(P[3] * P[5] - P[1] * P[7]);
-  E[2 * pg.ntot + p] = -(P[1] * P[6] - P[2] * P[5]);
+  regime_e_of(P, e);
+  E[p] = e[0];
+  E[pg.ntot + p] = e[1];
+  E[2 * pg.ntot + p] = e[2];
 }
 
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_q(Geo g, PadGeo pg, PadDiv dv, Spacing sp, const double *E,
@@ -182,6 +196,149 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_flux(Geo g, PadGeo pg
   }
   for (int v = 0; v < 3; ++v)
     F[v * pg.ntot + p] = Weno<ORDER>::face([&](int m) { return fp[v][m]; }, [&](int m) { return fm[v][m]; });
+}
+
+// Tile of padded cells per block of k_regime_fq (the same shapes as the
+// source's tiles).
+template <int DIMS>
+struct FqTile {
+  static constexpr int T[3] = {DIMS == 3 ? 4 : (DIMS == 2 ? 16 : 256), DIMS == 3 ? 8 : (DIMS == 2 ? 16 : 1),
+                               DIMS == 3 ? 8 : 1};
+  static_assert(T[0] * T[1] * T[2] == kRegimeThreads, "one padded cell per thread");
+};
+
+// Stencil offsets along an axis of k_regime_fq: the face's -R .. R - 1 and
+// q's fourth-order -2 .. 2.
+template <int ORDER>
+struct FqHalo {
+  static constexpr int R = Weno<ORDER>::R;
+  static constexpr int LO = R > 2 ? R : 2, HI = R - 1 > 2 ? R - 1 : 2;
+};
+
+// Cells of a tile extended by the halo along axis AX.
+template <int ORDER, int DIMS, int AX>
+struct FqExt {
+  static constexpr int D0 = FqTile<DIMS>::T[0] + (AX == 0 ? FqHalo<ORDER>::LO + FqHalo<ORDER>::HI : 0);
+  static constexpr int D1 = FqTile<DIMS>::T[1] + (AX == 1 ? FqHalo<ORDER>::LO + FqHalo<ORDER>::HI : 0);
+  static constexpr int D2 = FqTile<DIMS>::T[2] + (AX == 2 ? FqHalo<ORDER>::LO + FqHalo<ORDER>::HI : 0);
+  static constexpr int kCells = D0 * D1 * D2;
+};
+
+template <int ORDER, int DIMS>
+constexpr int fq_lds_cells() {
+  int m = FqExt<ORDER, DIMS, 0>::kCells;
+  if (DIMS > 1 && FqExt<ORDER, DIMS, 1>::kCells > m) m = FqExt<ORDER, DIMS, 1>::kCells;
+  if (DIMS > 2 && FqExt<ORDER, DIMS, 2>::kCells > m) m = FqExt<ORDER, DIMS, 2>::kCells;
+  return m;
+}
+
+// One axis of k_regime_fq: E and B of the tile extended along AX go to LDS
+// (E formed from the primitives by regime_e_of), then each thread forms the
+// face flux of its cell along AX into F's rows for AX (zero outside
+// [R, N - R) along AX, as k_regime_flux) and, when its cell is inside
+// [2, N - 2) along every axis (qin), adds the AX term of q to qacc as
+// k_regime_q sums it.
+template <int ORDER, int DIMS, int AX, int NL>
+__device__ __forceinline__ void regime_fq_phase(const Geo &g, const PadGeo &pg, const Spacing &sp, const double *V,
+                                                const int o[3], const int c[3], bool live, bool qin, double &qacc,
+                                                double *F, double (&s_eb)[6][NL]) {
+  using X = FqExt<ORDER, DIMS, AX>;
+  constexpr int R = FqHalo<ORDER>::R, LO = FqHalo<ORDER>::LO;
+  for (int idx = threadIdx.x; idx < X::kCells; idx += kRegimeThreads) {
+    const int l2 = idx % X::D2, l1 = (idx / X::D2) % X::D1, l0 = idx / (X::D2 * X::D1);
+    int cp[3] = {o[0] + l0, o[1] + l1, o[2] + l2};
+    cp[AX] -= LO;
+    bool in = true;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) in = in && cp[ax] >= 0 && cp[ax] < pg.N[ax];
+    double e[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+    if (in) {
+      double P[8];
+      load_prims(V, g.vstride, image_of(g, pg, cp), P);
+      regime_e_of(P, e);
+      B[0] = P[5];
+      B[1] = P[6];
+      B[2] = P[7];
+    }
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      s_eb[v][idx] = e[v];
+      s_eb[3 + v][idx] = B[v];
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const int p0[3] = {o[0] + c[0], o[1] + c[1], o[2] + c[2]};
+    const long long p = ((long long)p0[0] * pg.N[1] + p0[1]) * pg.N[2] + p0[2];
+    const bool fin = p0[AX] >= R && p0[AX] < pg.N[AX] - R;
+    // LDS slot of the cell at offset m - LO along AX from this thread's
+    constexpr int kStep = AX == 0 ? X::D1 * X::D2 : (AX == 1 ? X::D2 : 1);
+    const int base = (c[0] * X::D1 + c[1]) * X::D2 + c[2];  // offset -LO
+    const long long nt = pg.ntot;
+    if (fin) {
+      double fp[3][2 * R], fm[3][2 * R];
+#pragma unroll
+      for (int m = 0; m < 2 * R; ++m) {
+        const int sl = base + (m + LO - R) * kStep;
+        double P[8] = {0.0, 0.0, 0.0, 0.0, 0.0, s_eb[3][sl], s_eb[4][sl], s_eb[5][sl]}, fb[3];
+        stiff_flux(AX, P, fb);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          const double e = s_eb[v][sl];
+          fp[v][m] = 0.5 * (fb[v] + e);
+          fm[v][m] = 0.5 * (fb[v] - e);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 3; ++v)
+        F[((long long)AX * 3 + v) * nt + p] =
+            Weno<ORDER>::face([&](int m) { return fp[v][m]; }, [&](int m) { return fm[v][m]; });
+    } else {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) F[((long long)AX * 3 + v) * nt + p] = 0.0;
+    }
+    if (qin) {
+      const double *ea = &s_eb[AX][base + LO * kStep];  // E_AX at this cell
+      const double t = (-ea[2 * kStep] + 8 * ea[kStep] - 8 * ea[-kStep] + ea[-2 * kStep]) / (12 * sp.d[AX]);
+      qacc = AX == 0 ? t : qacc + t;
+    }
+  }
+  __syncthreads();  // the next axis refills LDS
+}
+
+// The E-face fluxes of every active axis (REGIME.cc:293-370) and q
+// (REGIME.cc:186-212) of each padded cell in one launch: the values
+// k_regime_e, k_regime_q and the per-axis k_regime_flux launches give, bit
+// for bit (E through regime_e_of, the same sums in the same order), without
+// E going through HBM. A block takes a tile of padded cells and, one axis at
+// a time, stages E and B of the tile extended by the stencil halo along that
+// axis in LDS, so each cell's primitives are read about twice per axis
+// instead of once per stencil point. The source's one-launch plan runs this,
+// then k_regime_ds.
+template <int ORDER, int DIMS>
+__global__ __launch_bounds__(kRegimeThreads) void k_regime_fq(Geo g, PadGeo pg, Spacing sp, const double *V,
+                                                              double *F, double *Q, int run) {
+  using T = FqTile<DIMS>;
+  constexpr int NL = fq_lds_cells<ORDER, DIMS>();
+  __shared__ double s_eb[6][NL];
+  const int nt0 = (pg.N[0] + T::T[0] - 1) / T::T[0], nt1 = (pg.N[1] + T::T[1] - 1) / T::T[1],
+            nt2 = (pg.N[2] + T::T[2] - 1) / T::T[2];
+  const int lb = chunk_block(blockIdx.x, gridDim.x, run);
+  const int t2 = lb % nt2, t1 = (lb / nt2) % nt1, t0 = lb / (nt2 * nt1);
+  if (t0 >= nt0) return;  // whole block: no barrier is skipped by part of it
+  const int o[3] = {t0 * T::T[0], t1 * T::T[1], t2 * T::T[2]};
+  const int tid = threadIdx.x;
+  const int c[3] = {tid / (T::T[2] * T::T[1]), (tid / T::T[2]) % T::T[1], tid % T::T[2]};
+  bool live = true, qin = true;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) live = live && o[ax] + c[ax] < pg.N[ax];
+#pragma unroll
+  for (int ax = 0; ax < DIMS; ++ax) qin = qin && o[ax] + c[ax] >= 2 && o[ax] + c[ax] < pg.N[ax] - 2;
+  double qacc = 0.0;
+  regime_fq_phase<ORDER, DIMS, 0, NL>(g, pg, sp, V, o, c, live, qin, qacc, F, s_eb);
+  if constexpr (DIMS > 1) regime_fq_phase<ORDER, DIMS, 1, NL>(g, pg, sp, V, o, c, live, qin, qacc, F, s_eb);
+  if constexpr (DIMS > 2) regime_fq_phase<ORDER, DIMS, 2, NL>(g, pg, sp, V, o, c, live, qin, qacc, F, s_eb);
+  if (live) Q[((long long)(o[0] + c[0]) * pg.N[1] + o[1] + c[1]) * pg.N[2] + o[2] + c[2]] = qin ? qacc : 0.0;
 }
 
 // dw/ds_b (12 x 3, rows 0, 4 and 11 zero) times alpha (REGIME.cc:226-290)
@@ -272,14 +429,10 @@ struct RegimeCell {
   double dw[12][3];
 };
 
-__device__ __forceinline__ void regime_cell(const Geo &g, const PadGeo &pg, const double *V, const double *E,
-                                            const double *Q, long long p, const int pos[3], double sig,
-                                            RegimeCell &c) {
-  const long long nt = pg.ntot;
+__device__ __forceinline__ void regime_cell(const Geo &g, const PadGeo &pg, const double *V, const double *Q,
+                                            long long p, const int pos[3], double sig, RegimeCell &c) {
   load_prims(V, g.vstride, image_of(g, pg, pos), c.P);
-  c.Ec[0] = E[p];
-  c.Ec[1] = E[nt + p];
-  c.Ec[2] = E[2 * nt + p];
+  regime_e_of(c.P, c.Ec);
   c.q = Q[p];
   const double qs = c.q * c.q + sig * sig;
   c.alpha = 1 / (qs * (qs + sig * sig * (c.P[5] * c.P[5] + c.P[6] * c.P[6] + c.P[7] * c.P[7])));
@@ -320,7 +473,7 @@ __device__ __forceinline__ void regime_d_axis(const RegimeCell &c, const double 
 // K (REGIME.cc:293-370), then D = M K per active axis, into the work arrays
 // (the fields the C ABI reads back, and the two-launch source).
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, PadDiv dv, Spacing sp, const double *V,
-                                                             const double *E, const double *Q, const double *F,
+                                                             const double *Q, const double *F,
                                                              double *D, double *Kout, double *Aout, double gam,
                                                              double sig, int run) {
   const long long p = (long long)chunk_block(blockIdx.x, gridDim.x, run) * kRegimeThreads + threadIdx.x;
@@ -331,7 +484,7 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, P
   double K[3];
   regime_k(g, pg, sp, F, p, pos, K);
   RegimeCell c;
-  regime_cell(g, pg, V, E, Q, p, pos, sig, c);
+  regime_cell(g, pg, V, Q, p, pos, sig, c);
   for (int v = 0; v < 3; ++v) Kout[v * nt + p] = K[v];
   Aout[p] = c.alpha;
   double Dl[8];
@@ -350,7 +503,7 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_d(Geo g, PadGeo pg, P
 // D at padded position ppos along axis AX, zero outside the padded grid.
 template <int AX>
 __device__ __forceinline__ void regime_d_at(const Geo &g, const PadGeo &pg, const Spacing &sp, const double *V,
-                                            const double *E, const double *Q, const double *F, const int ppos[3],
+                                            const double *Q, const double *F, const int ppos[3],
                                             double gam, double sig, double D[8]) {
   bool in = true;
 #pragma unroll
@@ -363,7 +516,7 @@ __device__ __forceinline__ void regime_d_at(const Geo &g, const PadGeo &pg, cons
   double K[3];
   regime_k(g, pg, sp, F, p, ppos, K);
   RegimeCell c;
-  regime_cell(g, pg, V, E, Q, p, ppos, sig, c);
+  regime_cell(g, pg, V, Q, p, ppos, sig, c);
   regime_d_axis<AX>(c, K, gam, D);
 }
 
@@ -391,7 +544,7 @@ struct DsTile {
 // per-cell computation as k_regime_d's.
 template <int DIMS>
 __global__ __launch_bounds__(kRegimeThreads) void k_regime_ds(Geo g, PadGeo pg, Spacing sp, const double *V,
-                                                              const double *E, const double *Q, const double *F,
+                                                              const double *Q, const double *F,
                                                               double *src, double gam, double sig, int run) {
   using T = DsTile<DIMS>;
   constexpr int T0 = T::T0, T1 = T::T1, T2 = T::T2;
@@ -420,7 +573,7 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_ds(Geo g, PadGeo pg, 
       double K[3];
       regime_k(g, pg, sp, F, p, pp, K);
       RegimeCell c;
-      regime_cell(g, pg, V, E, Q, p, pp, sig, c);
+      regime_cell(g, pg, V, Q, p, pp, sig, c);
       regime_d_axis<0>(c, K, gam, D0);
       if (DIMS > 1) regime_d_axis<(DIMS > 1 ? 1 : 0)>(c, K, gam, D1);
       if (DIMS > 2) regime_d_axis<(DIMS > 2 ? 2 : 0)>(c, K, gam, D2);
@@ -440,17 +593,17 @@ __global__ __launch_bounds__(kRegimeThreads) void k_regime_ds(Geo g, PadGeo pg, 
       if (h < H0) {
         const int side = h / (T1 * T2), r = h % (T1 * T2), i1 = r / T2, i2 = r % T2;
         const int pp[3] = {o[0] + (side ? T0 : -1), o[1] + i1, o[2] + i2};
-        regime_d_at<0>(g, pg, sp, V, E, Q, F, pp, gam, sig, D);
+        regime_d_at<0>(g, pg, sp, V, Q, F, pp, gam, sig, D);
         slot = slot0(side ? T0 + 1 : 0, i1, i2);
       } else if (DIMS > 1 && h < H0 + H1) {
         const int hh = h - H0, side = hh / (T0 * T2), r = hh % (T0 * T2), i0 = r / T2, i2 = r % T2;
         const int pp[3] = {o[0] + i0, o[1] + (side ? T1 : -1), o[2] + i2};
-        regime_d_at<(DIMS > 1 ? 1 : 0)>(g, pg, sp, V, E, Q, F, pp, gam, sig, D);
+        regime_d_at<(DIMS > 1 ? 1 : 0)>(g, pg, sp, V, Q, F, pp, gam, sig, D);
         slot = slot1(i0, side ? T1 + 1 : 0, i2);
       } else {
         const int hh = h - H0 - H1, side = hh / (T0 * T1), r = hh % (T0 * T1), i0 = r / T1, i1 = r % T1;
         const int pp[3] = {o[0] + i0, o[1] + i1, o[2] + (side ? T2 : -1)};
-        regime_d_at<(DIMS > 2 ? 2 : 0)>(g, pg, sp, V, E, Q, F, pp, gam, sig, D);
+        regime_d_at<(DIMS > 2 ? 2 : 0)>(g, pg, sp, V, Q, F, pp, gam, sig, D);
         slot = slot2(i0, i1, side ? T2 + 1 : 0);
       }
       for (int l = 0; l < 8; ++l) sD[l][slot] = D[l];
@@ -552,14 +705,55 @@ void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], co
   const RegimeFields fl = regime_fields_of(pg, work);
   double *E = fl.E, *Q = fl.q, *F = fl.F, *D = fl.D;
   const PadDiv dv = pad_div(pg);
-  hipLaunchKernelGGL(k_regime_e, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, V, E);
   // an eighth of the blocks of one padded (stencil kernels) or interior
   // plane: each XCD then keeps the same rows of every plane, so both its
   // x and its y neighbours stay in its L2
   const int run_pad = (int)std::max<long long>(1, (long long)pg.N[1] * pg.N[2] / kRegimeThreads / 8);
   const int run_int = (int)std::max<long long>(1, (long long)g.n[1] * g.n[2] / kRegimeThreads / 8);
-  hipLaunchKernelGGL(k_regime_q, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, sp, E, Q, run_pad);
   const dim3 grid(blocks(nt)), block(kRegimeThreads);
+  if (fused) {
+    // face fluxes and q in one launch, E formed where it is used
+    {
+      const int n1 = (pg.N[1] + 7) / 8, n2 = (pg.N[2] + 7) / 8;  // 3-D tiles 4 x 8 x 8
+      if (g.dims == 3) {
+        const unsigned nb = (unsigned)(((pg.N[0] + 3) / 4) * n1 * n2);
+        const int run = std::max(1, n1 * n2 / 8);
+        if (order == 3) hipLaunchKernelGGL((k_regime_fq<3, 3>), dim3(nb), block, 0, s, g, pg, sp, V, F, Q, run);
+        else hipLaunchKernelGGL((k_regime_fq<5, 3>), dim3(nb), block, 0, s, g, pg, sp, V, F, Q, run);
+      } else if (g.dims == 2) {
+        const int m1 = (pg.N[1] + 15) / 16;
+        const unsigned nb = (unsigned)(((pg.N[0] + 15) / 16) * m1);
+        const int run = std::max(1, m1 / 8);
+        if (order == 3) hipLaunchKernelGGL((k_regime_fq<3, 2>), dim3(nb), block, 0, s, g, pg, sp, V, F, Q, run);
+        else hipLaunchKernelGGL((k_regime_fq<5, 2>), dim3(nb), block, 0, s, g, pg, sp, V, F, Q, run);
+      } else {
+        const unsigned nb = (unsigned)((pg.N[0] + 255) / 256);
+        if (order == 3) hipLaunchKernelGGL((k_regime_fq<3, 1>), dim3(nb), block, 0, s, g, pg, sp, V, F, Q, 1);
+        else hipLaunchKernelGGL((k_regime_fq<5, 1>), dim3(nb), block, 0, s, g, pg, sp, V, F, Q, 1);
+      }
+    }
+    if (g.dims == 3) {
+      using T = DsTile<3>;
+      const unsigned nb = (unsigned)(((g.n[0] + T::T0 - 1) / T::T0) * ((g.n[1] + T::T1 - 1) / T::T1) *
+                                     ((g.n[2] + T::T2 - 1) / T::T2));
+      const int run = std::max(1, ((g.n[1] + T::T1 - 1) / T::T1) * ((g.n[2] + T::T2 - 1) / T::T2) / 8);
+      hipLaunchKernelGGL(k_regime_ds<3>, dim3(nb), block, 0, s, g, pg, sp, V, Q, F, src, gam, sigma, run);
+    } else if (g.dims == 2) {
+      using T = DsTile<2>;
+      const unsigned nb = (unsigned)(((g.n[0] + T::T0 - 1) / T::T0) * ((g.n[1] + T::T1 - 1) / T::T1));
+      const int run = std::max(1, ((g.n[1] + T::T1 - 1) / T::T1) / 8);
+      hipLaunchKernelGGL(k_regime_ds<2>, dim3(nb), block, 0, s, g, pg, sp, V, Q, F, src, gam, sigma, run);
+    } else {
+      using T = DsTile<1>;
+      const unsigned nb = (unsigned)((g.n[0] + T::T0 - 1) / T::T0);
+      hipLaunchKernelGGL(k_regime_ds<1>, dim3(nb), block, 0, s, g, pg, sp, V, Q, F, src, gam, sigma, 1);
+    }
+    return;
+  }
+  // the field-keeping plan: E, q and each axis' face fluxes as separate
+  // launches that store them, then D, K and alpha stored, then the source
+  hipLaunchKernelGGL(k_regime_e, grid, block, 0, s, g, pg, dv, V, E);
+  hipLaunchKernelGGL(k_regime_q, grid, block, 0, s, g, pg, dv, sp, E, Q, run_pad);
   if (order == 3) {
     hipLaunchKernelGGL((k_regime_flux<3, 0>), grid, block, 0, s, g, pg, dv, V, E, F, run_pad);
     if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<3, 1>), grid, block, 0, s, g, pg, dv, V, E, F + 3 * nt, run_pad);
@@ -569,26 +763,7 @@ void launch_regime_source(const Geo &g, const PadGeo &pg, const double dx[3], co
     if (g.dims > 1) hipLaunchKernelGGL((k_regime_flux<5, 1>), grid, block, 0, s, g, pg, dv, V, E, F + 3 * nt, run_pad);
     if (g.dims > 2) hipLaunchKernelGGL((k_regime_flux<5, 2>), grid, block, 0, s, g, pg, dv, V, E, F + 6 * nt, run_pad);
   }
-  if (fused) {
-    if (g.dims == 3) {
-      using T = DsTile<3>;
-      const unsigned nb = (unsigned)(((g.n[0] + T::T0 - 1) / T::T0) * ((g.n[1] + T::T1 - 1) / T::T1) *
-                                     ((g.n[2] + T::T2 - 1) / T::T2));
-      const int run = std::max(1, ((g.n[1] + T::T1 - 1) / T::T1) * ((g.n[2] + T::T2 - 1) / T::T2) / 8);
-      hipLaunchKernelGGL(k_regime_ds<3>, dim3(nb), block, 0, s, g, pg, sp, V, E, Q, F, src, gam, sigma, run);
-    } else if (g.dims == 2) {
-      using T = DsTile<2>;
-      const unsigned nb = (unsigned)(((g.n[0] + T::T0 - 1) / T::T0) * ((g.n[1] + T::T1 - 1) / T::T1));
-      const int run = std::max(1, ((g.n[1] + T::T1 - 1) / T::T1) / 8);
-      hipLaunchKernelGGL(k_regime_ds<2>, dim3(nb), block, 0, s, g, pg, sp, V, E, Q, F, src, gam, sigma, run);
-    } else {
-      using T = DsTile<1>;
-      const unsigned nb = (unsigned)((g.n[0] + T::T0 - 1) / T::T0);
-      hipLaunchKernelGGL(k_regime_ds<1>, dim3(nb), block, 0, s, g, pg, sp, V, E, Q, F, src, gam, sigma, 1);
-    }
-    return;
-  }
-  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, sp, V, E, Q, F, D, fl.K,
+  hipLaunchKernelGGL(k_regime_d, dim3(blocks(nt)), dim3(kRegimeThreads), 0, s, g, pg, dv, sp, V, Q, F, D, fl.K,
                      fl.alpha, gam, sigma, run_pad);
   hipLaunchKernelGGL(k_regime_source, dim3(blocks(g.vstride)), dim3(kRegimeThreads), 0, s, g, pg, sp, D, src,
                      run_int);
