@@ -64,7 +64,7 @@ struct mgpu_engine {
   int ncons, nprims, naux;
   hipStream_t stream = nullptr;
   double *U = nullptr, *V = nullptr, *A = nullptr, *U1 = nullptr, *V1 = nullptr, *fnet = nullptr;
-  double *U2 = nullptr, *V2 = nullptr, *F1 = nullptr, *S1 = nullptr, *S2 = nullptr;  // SSP2 only
+  double *U2 = nullptr, *V2 = nullptr, *Pf = nullptr, *S1 = nullptr;  // SSP2 only
   // RKPlus stage states (aux only for the resistive model, whose source reads
   // the current from aux) and RK4's kept stage-four rhs
   double *rkU[4] = {}, *rkV[4] = {}, *rkA[4] = {}, *rkR = nullptr;
@@ -306,7 +306,7 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   a.Uout = e->U1;
   a.Vout = e->V1;
   a.Aout = nullptr;
-  a.S2 = e->S1;  // stage one writes source1
+  a.Sout = e->S1;  // stage one writes source1
   launch_imex1_srrmhd(a, e->stream);
   HIP_OK(hipGetLastError());
   if (mark(e, "imex_stage1_c2p")) return -1;
@@ -318,9 +318,8 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   b.Uout = e->U2;
   b.Vout = e->V2;
   b.Aout = nullptr;
-  b.F1 = e->F1;
+  b.Pf = e->Pf;
   b.S1 = e->S1;
-  b.S2 = e->S2;
   if (run_sweeps(e, b, kSweepImex2)) return -1;
 
   SweepArgs c = sweep_args(e, e->U2, e->V2, st);
@@ -330,9 +329,7 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   c.Uout = e->U;
   c.Vout = e->V;
   c.Aout = e->A;
-  c.F1 = e->F1;
-  c.S1 = e->S1;
-  c.S2 = e->S2;
+  c.Pf = e->Pf;
   return run_sweeps(e, c, kSweepImexFinal);
 }
 
@@ -667,9 +664,9 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
     size_t elems;
   } bufs[] = {{&e->U, n * e->ncons},  {&e->V, n * e->nprims}, {&e->A, n * e->naux},
               {&e->U1, n * e->ncons}, {&e->V1, n * e->nprims}, {&e->fnet, n * e->ncons},
-              {&e->U2, n * e->ncons}, {&e->V2, n * e->nprims}, {&e->F1, n * e->ncons},
-              {&e->S1, n * 5},        {&e->S2, n * 5}};
-  const int nbufs = c.integrator == MGPU_SSP2 ? 11 : 6;
+              {&e->U2, n * e->ncons}, {&e->V2, n * e->nprims}, {&e->Pf, n * e->ncons},
+              {&e->S1, n * 5}};
+  const int nbufs = c.integrator == MGPU_SSP2 ? 10 : 6;
   for (int bi = 0; bi < nbufs; ++bi) {
     auto &b = bufs[bi];
     if (hipMalloc(b.ptr, b.elems * sizeof(double)) != hipSuccess)
@@ -715,7 +712,7 @@ int mgpu_destroy(mgpu_engine *e) {
   if (!e) return 0;
   if (e->stream) hipStreamSynchronize(e->stream);
   double *bufs[] = {e->U,  e->V,  e->A,  e->U1, e->V1, e->fnet,     e->staging,
-                    e->U2, e->V2, e->F1, e->S1, e->S2, e->root};
+                    e->U2, e->V2, e->Pf, e->S1, e->root};
   for (double *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < 4; ++k) {

@@ -251,6 +251,49 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
   c2p_store_srrmhd(a, q, un, A, cs);
 }
 
+// SSP2 stage two at one cell once flux1 is complete (SSP2.cc:112-148): solve
+// from c = cons - dt (flux1 - om2gam source1), cons2prim, and store the part
+// of the final combination known now, U^n - dt/2 flux1 + dt/2 (source1 +
+// source2), so the final sweep reads one array instead of U^n, flux1 and
+// both sources.
+__device__ __forceinline__ void imex2_apply(const SweepArgs &a, long long q, const double *fn, CellC2P &cs) {
+  const long long vs = a.geo.vstride;
+  double c[kSrrmhdCons], pf[kSrrmhdCons], x[kSrrmhdCons], A[kSrrmhdAux], s1[5];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdCons; ++v) {
+    const double un = a.Un[v * vs + q];
+    c[v] = un - a.dt * fn[v];
+    pf[v] = un - 0.5 * a.dt * fn[v];
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    s1[m] = a.S1[m * vs + q];
+    c[8 + m] += a.dt * kSsp2Om2Gam * s1[m];
+  }
+  const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
+                                   a.Vg[4 * vs + q], a.gamma);
+  const int its = imex_stage(c, a.dt, a.gamma, a.sigma, a.cp, Zg, x);
+  c2p_store_srrmhd(a, q, x, A, cs);
+  if (its < 0) cs.fails = 1;
+  double s5[5];
+  srrmhd_source5(x, A, a.cp, s5);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) pf[8 + m] += 0.5 * a.dt * (s1[m] + s5[m]);
+#pragma unroll
+  for (int v = 0; v < kSrrmhdCons; ++v) a.Pf[v * vs + q] = pf[v];
+}
+
+// SSP2's final combination at one cell, cons += -dt/2 (flux1 + flux2 -
+// source1 - source2) (SSP2.cc:156-166), from the part stage two stored, then
+// cons2prim.
+__device__ __forceinline__ void imex_final_apply(const SweepArgs &a, long long q, const double *fn, CellC2P &cs) {
+  const long long vs = a.geo.vstride;
+  double un[kSrrmhdCons], A[kSrrmhdAux];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdCons; ++v) un[v] = a.Pf[v * vs + q] - 0.5 * a.dt * fn[v];
+  c2p_store_srrmhd(a, q, un, A, cs);
+}
+
 // Stored-flux sweeps only accumulate the net flux (write and add modes).
 template <class M>
 __device__ __forceinline__ void finish_rk2(const SweepArgs &, long long, const double *, StoredM<M>, CellC2P &) {}
@@ -275,33 +318,9 @@ __device__ __forceinline__ void sweep_apply(const SweepArgs &a, long long q, con
     }
     finish_rk2(a, q, un, M(), cs);
   } else if (MODE == kSweepImex2) {
-    // flux1 complete: keep it, then solve stage two from
-    // c = cons - dt (flux1 - om2gam source1) (SSP2.cc:112-148)
-    double c[kSrrmhdCons], x[kSrrmhdCons], A[kSrrmhdAux];
-#pragma unroll
-    for (int v = 0; v < kSrrmhdCons; ++v) {
-      a.F1[v * vs + q] = fn[v];
-      c[v] = a.Un[v * vs + q] - a.dt * fn[v];
-    }
-#pragma unroll
-    for (int m = 0; m < 5; ++m) c[8 + m] += a.dt * kSsp2Om2Gam * a.S1[m * vs + q];
-    const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
-                                     a.Vg[4 * vs + q], a.gamma);
-    const int its = imex_stage(c, a.dt, a.gamma, a.sigma, a.cp, Zg, x);
-    c2p_store_srrmhd(a, q, x, A, cs);
-    if (its < 0) cs.fails = 1;
-    double s5[5];
-    srrmhd_source5(x, A, a.cp, s5);
-#pragma unroll
-    for (int m = 0; m < 5; ++m) a.S2[m * vs + q] = s5[m];
+    imex2_apply(a, q, fn, cs);
   } else if (MODE == kSweepImexFinal) {
-    // cons += -dt/2 (flux1 + flux2 - source1 - source2) (SSP2.cc:156-166)
-    double un[kSrrmhdCons], A[kSrrmhdAux];
-#pragma unroll
-    for (int v = 0; v < kSrrmhdCons; ++v) un[v] = a.Un[v * vs + q] - 0.5 * a.dt * (a.F1[v * vs + q] + fn[v]);
-#pragma unroll
-    for (int m = 0; m < 5; ++m) un[8 + m] += 0.5 * a.dt * (a.S1[m * vs + q] + a.S2[m * vs + q]);
-    c2p_store_srrmhd(a, q, un, A, cs);
+    imex_final_apply(a, q, fn, cs);
   }
 }
 
@@ -683,33 +702,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
         }
         finish_rk2(a, q, un, M(), cs);
       } else if (MODE == kSweepImex2) {
-        // flux1 complete: keep it, then solve stage two from
-        // c = cons - dt (flux1 - om2gam source1) (SSP2.cc:112-148)
-        double c[kSrrmhdCons], x[kSrrmhdCons], A[kSrrmhdAux];
-#pragma unroll
-        for (int v = 0; v < kSrrmhdCons; ++v) {
-          a.F1[v * vs + q] = fn[v];
-          c[v] = a.Un[v * vs + q] - a.dt * fn[v];
-        }
-#pragma unroll
-        for (int m = 0; m < 5; ++m) c[8 + m] += a.dt * kSsp2Om2Gam * a.S1[m * vs + q];
-        const double Zg = srrmhd_guess_Z(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q],
-                                         a.Vg[4 * vs + q], a.gamma);
-        const int its = imex_stage(c, a.dt, a.gamma, a.sigma, a.cp, Zg, x);
-        c2p_store_srrmhd(a, q, x, A, cs);
-        if (its < 0) cs.fails = 1;
-        double s5[5];
-        srrmhd_source5(x, A, a.cp, s5);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) a.S2[m * vs + q] = s5[m];
+        imex2_apply(a, q, fn, cs);
       } else if (MODE == kSweepImexFinal) {
-        // cons += -dt/2 (flux1 + flux2 - source1 - source2) (SSP2.cc:156-166)
-        double un[kSrrmhdCons], A[kSrrmhdAux];
-#pragma unroll
-        for (int v = 0; v < kSrrmhdCons; ++v) un[v] = a.Un[v * vs + q] - 0.5 * a.dt * (a.F1[v * vs + q] + fn[v]);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) un[8 + m] += 0.5 * a.dt * (a.S1[m * vs + q] + a.S2[m * vs + q]);
-        c2p_store_srrmhd(a, q, un, A, cs);
+        imex_final_apply(a, q, fn, cs);
       }
     }
   }
@@ -1371,7 +1366,7 @@ __global__ __launch_bounds__(kThreads) void k_imex1_srrmhd(SweepArgs a) {
     double s5[5];
     srrmhd_source5(x, A, a.cp, s5);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) a.S2[m * vs + q] = s5[m];
+    for (int m = 0; m < 5; ++m) a.Sout[m * vs + q] = s5[m];
   }
   reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
 }

@@ -45,9 +45,10 @@ struct SweepArgs {
   int vout_n;        // SRMHD: prims written to Vout (0: all 8; 5: rho .. p, for a stage whose
                      // prims are read only by the next stage's sweeps, guess and retry)
   double dt, gamma, sigma, cp;
-  double *F1;        // SSP2: flux1 store
+  double *Pf;        // SSP2: U^n - dt/2 flux1 + dt/2 (source1 + source2), the part of the final
+                     // combination (SSP2.cc:156-166) known after stage two
   const double *S1;  // SSP2: source1 (components 8..12)
-  double *S2;        // SSP2: source2 (components 8..12)
+  double *Sout;      // SSP2 stage one: its source, source1 (components 8..12)
   C2PCounters *stats;
   int *fail_count;
   long long *fail_list;
