@@ -89,8 +89,9 @@ struct mgpu_engine {
   int xyz_order = 1;                     // 3-D: sweeps x, y, z with the last (contiguous) one finishing
   int fused_yzf = 1;                     // plan 3: 3-D x sweep, then y + z as one launch; 2-D y + x as one
   // SRMHD RK2 and RKSplit steps keep the cons2prim root (v^2, Z) of each cell
-  // instead of writing its 13 aux values: no kernel of those steps reads aux,
-  // and what does (state download, wavespeed) forms it from the root first.
+  // instead of writing its 13 aux values, and SRRMHD SSP2 steps the root Z
+  // instead of 17: no kernel of those steps reads aux, and what does (state
+  // download, wavespeed) forms it from the root first.
   double *root = nullptr;
   bool aux_stale = false;
   // cell-list operators (single-cell model calls): device copies of the
@@ -290,7 +291,10 @@ int rk2_step(mgpu_engine *e, double dt, C2PCounters *st) {
 // The aux of the resident state, formed now if the last step left it.
 int fresh_aux(mgpu_engine *e) {
   if (!e->aux_stale) return 0;
-  launch_aux_srmhd(e->geo, e->U, e->root, e->A, e->cfg.gamma, e->stream);
+  if (e->cfg.model == MGPU_SRMHD)
+    launch_aux_srmhd(e->geo, e->U, e->root, e->A, e->cfg.gamma, e->stream);
+  else
+    launch_aux_srrmhd(e->geo, e->U, e->root, e->A, e->cfg.gamma, e->cfg.sigma, e->stream);
   HIP_OK(hipGetLastError());
   e->aux_stale = false;
   return 0;
@@ -328,9 +332,13 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   c.Vg = e->V2;
   c.Uout = e->U;
   c.Vout = e->V;
-  c.Aout = e->A;
+  // the step keeps each cell's root Z; the aux is formed from it when read
+  c.Aout = nullptr;
+  c.root = e->root;
   c.Pf = e->Pf;
-  return run_sweeps(e, c, kSweepImexFinal);
+  if (run_sweeps(e, c, kSweepImexFinal)) return -1;
+  e->aux_stale = true;
+  return 0;
 }
 
 // The model extension's source of the state with prims V into e->rg_src
@@ -675,6 +683,10 @@ int mgpu_create(const mgpu_config *cfg, mgpu_engine **out) {
   }
   if (c.model == MGPU_SRMHD && (c.integrator == MGPU_RK2 || c.integrator == MGPU_RKSPLIT)) {
     if (hipMalloc(&e->root, 2 * n * sizeof(double)) != hipSuccess)
+      return cleanup(fail("hipMalloc of the cons2prim root store failed"));
+  }
+  if (c.model == MGPU_SRRMHD && c.integrator == MGPU_SSP2) {
+    if (hipMalloc(&e->root, n * sizeof(double)) != hipSuccess)
       return cleanup(fail("hipMalloc of the cons2prim root store failed"));
   }
   if (c.integrator >= MGPU_RK2B) {

@@ -241,6 +241,8 @@ __device__ __forceinline__ void c2p_store_srrmhd(const SweepArgs &a, long long q
   if (a.Aout) {
 #pragma unroll
     for (int v = 0; v < kSrrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
+  } else if (a.root) {
+    a.root[q] = A[10];  // Z: the aux follows from it and the cons (srrmhd_recover)
   }
   cs.add(its);
 }
@@ -1077,6 +1079,21 @@ __global__ __launch_bounds__(kThreads) void k_aux_srmhd(Geo g, const double *U, 
   for (int v = 0; v < 13; ++v) A[v * vs + q] = aux[v];
 }
 
+// Aux of every cell from its cons and kept cons2prim root Z (the values
+// srrmhd_c2p_cell writes: srrmhd_recover).
+__global__ __launch_bounds__(kThreads) void k_aux_srrmhd(Geo g, const double *U, const double *root, double *A,
+                                                          double gam, double sigma) {
+  const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= g.vstride) return;
+  const long long vs = g.vstride;
+  double u[kSrrmhdCons], P[kSrrmhdPrims], aux[kSrrmhdAux];
+#pragma unroll
+  for (int v = 0; v < kSrrmhdCons; ++v) u[v] = U[v * vs + q];
+  srrmhd_recover(u, gam, sigma, root[q], P, aux);
+#pragma unroll
+  for (int v = 0; v < kSrrmhdAux; ++v) A[v * vs + q] = aux[v];
+}
+
 // SRMHD::getPrimitiveVars on resident cons with resident prims as the guess
 // (srmhd.cc:390-562).
 // cold: start from v^2 = 0, Z = tau + D instead of the previous primitives.
@@ -1784,6 +1801,11 @@ void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double 
 
 void launch_aux_srmhd(const Geo &g, const double *U, const double *root, double *A, double gam, hipStream_t s) {
   hipLaunchKernelGGL(k_aux_srmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, root, A, gam);
+}
+
+void launch_aux_srrmhd(const Geo &g, const double *U, const double *root, double *A, double gam, double sigma,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_aux_srrmhd, dim3(blocks_for(g.vstride)), dim3(kThreads), 0, s, g, U, root, A, gam, sigma);
 }
 
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,

@@ -41,7 +41,7 @@ struct SweepArgs {
   const double *Un;  // stage 2: cons at t^n
   const double *Vg;  // prims giving the cons2prim starting guess
   double *Uout, *Vout, *Aout;
-  double *root;      // SRMHD: cons2prim root (v^2, Z) [2][cell], kept when Aout is null
+  double *root;      // cons2prim root, kept when Aout is null: SRMHD (v^2, Z) [2][cell], SRRMHD Z [cell]
   int vout_n;        // SRMHD: prims written to Vout (0: all 8; 5: rho .. p, for a stage whose
                      // prims are read only by the next stage's sweeps, guess and retry)
   double dt, gamma, sigma, cp;
@@ -87,6 +87,9 @@ void launch_prims_to_all_srmhd(const Geo &g, const double *V, double *U, double 
 // SRMHD aux from the resident cons and the cons2prim root kept by a stage,
 // through the same recovery formulas the stage would have applied.
 void launch_aux_srmhd(const Geo &g, const double *U, const double *root, double *A, double gam, hipStream_t s);
+// SRRMHD aux from the cons and the kept root Z [cell].
+void launch_aux_srrmhd(const Geo &g, const double *U, const double *root, double *A, double gam, double sigma,
+                       hipStream_t s);
 // hist: null, or kHistBins Newton-step counters the cells are added to.
 void launch_c2p_srmhd(const Geo &g, const double *U, double *V, double *A, double gam, C2PCounters *st,
                       int *fail_count, long long *fail_list, unsigned char *fail_flag, int cold,

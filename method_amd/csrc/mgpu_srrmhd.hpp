@@ -141,16 +141,9 @@ __device__ __forceinline__ int srrmhd_secant(double &Z, double SbSq, double D, d
   return -1;
 }
 
-// SRRMHD::getPrimitiveVarsSingleCell (srrmhd.cc:256-312) with guess Zg.
-// Writes prims P[11] and aux A[17]; returns secant iterations or -1.
-__device__ __forceinline__ int srrmhd_c2p_cell(const double u[14], double gam, double sigma, double Zg, double P[11],
-                                               double A[17]) {
-  P[5] = u[5];
-  P[6] = u[6];
-  P[7] = u[7];
-  P[8] = u[8];
-  P[9] = u[9];
-  P[10] = u[10];
+// The aux entries that depend on the cons only (srrmhd.cc:512-538): B^2, E^2,
+// S - E x B and its square, tau - (B^2 + E^2)/2.
+__device__ __forceinline__ void srrmhd_aux_of_cons(const double u[14], double A[17]) {
   A[7] = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
   A[8] = u[8] * u[8] + u[9] * u[9] + u[10] * u[10];
   A[12] = u[1] - (u[9] * u[7] - u[10] * u[6]);
@@ -158,8 +151,20 @@ __device__ __forceinline__ int srrmhd_c2p_cell(const double u[14], double gam, d
   A[14] = u[3] - (u[8] * u[6] - u[9] * u[5]);
   A[15] = A[12] * A[12] + A[13] * A[13] + A[14] * A[14];
   A[16] = u[4] - 0.5 * (A[7] + A[8]);
-  double Z = Zg;
-  const int its = srrmhd_secant(Z, A[15], u[0], A[16], gam);
+}
+
+// Prims and aux from the cons and the root Z = rho h W^2 (srrmhd.cc:539-581,
+// after the secant): a pure function of (u, Z), so an aux formed later from a
+// kept root equals the one the cons2prim would have written, bit for bit.
+__device__ __forceinline__ void srrmhd_recover(const double u[14], double gam, double sigma, double Z, double P[11],
+                                               double A[17]) {
+  P[5] = u[5];
+  P[6] = u[6];
+  P[7] = u[7];
+  P[8] = u[8];
+  P[9] = u[9];
+  P[10] = u[10];
+  srrmhd_aux_of_cons(u, A);
   A[10] = Z;
   A[9] = A[15] / (Z * Z);
   A[1] = 1.0 / sqrt(1.0 - A[9]);
@@ -177,6 +182,17 @@ __device__ __forceinline__ int srrmhd_c2p_cell(const double u[14], double gam, d
   A[4] = u[13] * P[1] + ws * (u[8] + (P[2] * u[7] - P[3] * u[6]) - A[11] * P[1]);
   A[5] = u[13] * P[2] + ws * (u[9] + (P[3] * u[5] - P[1] * u[7]) - A[11] * P[2]);
   A[6] = u[13] * P[3] + ws * (u[10] + (P[1] * u[6] - P[2] * u[5]) - A[11] * P[3]);
+}
+
+// SRRMHD::getPrimitiveVarsSingleCell (srrmhd.cc:256-312) with guess Zg.
+// Writes prims P[11] and aux A[17]; returns secant iterations or -1.
+__device__ __forceinline__ int srrmhd_c2p_cell(const double u[14], double gam, double sigma, double Zg, double P[11],
+                                               double A[17]) {
+  double Ac[17];
+  srrmhd_aux_of_cons(u, Ac);
+  double Z = Zg;
+  const int its = srrmhd_secant(Z, Ac[15], u[0], Ac[16], gam);
+  srrmhd_recover(u, gam, sigma, Z, P, A);
   return its;
 }
 

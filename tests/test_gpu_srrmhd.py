@@ -212,6 +212,34 @@ def test_fused_yx_finishing_sweep_2d_equals_separate_sweeps_bitwise(n, weno, int
         assert np.array_equal(a, b), f"{name}: max diff {np.max(np.abs(a - b)):.3e}"
 
 
+def test_ssp2_aux_formed_from_kept_root_is_consistent(golden):
+    """SSP2 steps keep each cell's cons2prim root Z = rho h W^2 in place of its
+    17 aux values, and the aux is formed from it when the state is read. It
+    must be the same bits on every later read, hold Z itself (aux 10) and B^2,
+    E^2 of the downloaded cons, and agree with the aux a fresh cons2prim of the
+    same cons writes (srrmhd.cc:256-312) to the secant tolerance."""
+    z = golden("ot2d_32_weno3_ssp2")
+    say("gpu: SRRMHD aux formed from the kept root after SSP2 steps, OT 32^2")
+    with engine(**OT32) as e:
+        e.set_prims(z["prims0"])
+        for _ in range(3):
+            assert e.update_time() == 0
+        c1, p1, a1 = e.download()
+        c2, p2, a2 = e.download()
+        rc, st = e.c2p()
+        assert rc == 0 and st["failures"] == 0
+        c3, p3, a3 = e.download()
+    for x, y in ((c1, c2), (p1, p2), (a1, a2)):
+        assert np.array_equal(x, y)
+    a, c, a_fresh = (parity.interior(x, 4, 2) for x in (a1, c1, a3))
+    B2, E2 = (c[5:8] ** 2).sum(axis=0), (c[8:11] ** 2).sum(axis=0)
+    assert np.max(np.abs(a[7] - B2)) <= 1e-14 * B2.max()
+    assert np.max(np.abs(a[8] - E2)) <= 1e-14 * E2.max()
+    for k in range(a.shape[0]):
+        scale = max(np.abs(a_fresh[k]).max(), 1e-300)
+        assert np.max(np.abs(a[k] - a_fresh[k])) <= 1e-10 * scale, f"aux {k}"
+
+
 def test_rksplit_matches_oracle():
     """RKSplit with the resistive model (explicit split source, sigma 10)."""
     g = idt.ot2d_grid(24)
