@@ -268,8 +268,8 @@ int rk2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   a.Uout = e->U1;
   a.Vout = e->V1;
   a.Aout = nullptr;
-  // V1 is read only by stage two's sweeps (vx .. p), guess and retry (rho .. p)
-  if (e->cfg.model == MGPU_SRMHD) a.vout_n = 5;
+  // V1 is read only by stage two's sweeps (SRMHD vx .. p, SRRMHD rho .. p), guess and retry (rho .. p)
+  a.vout_n = 5;
   if (run_sweeps(e, a, kSweepStage1)) return -1;
   if (retry(e, e->U1, e->V1, nullptr, st)) return -1;
 
@@ -309,6 +309,7 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   a.Vg = e->V;
   a.Uout = e->U1;
   a.Vout = e->V1;
+  a.vout_n = 5;  // V1 and V2 are read only as the next stage's flux prims and guess (rho .. p)
   a.Aout = nullptr;
   a.Sout = e->S1;  // stage one writes source1
   launch_imex1_srrmhd(a, e->stream);
@@ -321,6 +322,7 @@ int ssp2_step(mgpu_engine *e, double dt, C2PCounters *st) {
   b.Vg = e->V1;
   b.Uout = e->U2;
   b.Vout = e->V2;
+  b.vout_n = 5;
   b.Aout = nullptr;
   b.Pf = e->Pf;
   b.S1 = e->S1;

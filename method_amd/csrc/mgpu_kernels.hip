@@ -236,8 +236,10 @@ __device__ __forceinline__ void c2p_store_srrmhd(const SweepArgs &a, long long q
   const int its = srrmhd_c2p_cell(un, a.gamma, a.sigma, Zg, P, A);
 #pragma unroll
   for (int v = 0; v < kSrrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+  const int nvout = a.vout_n ? a.vout_n : kSrrmhdPrims;
 #pragma unroll
-  for (int v = 0; v < kSrrmhdPrims; ++v) a.Vout[v * vs + q] = P[v];
+  for (int v = 0; v < kSrrmhdPrims; ++v)
+    if (v < nvout) a.Vout[v * vs + q] = P[v];
   if (a.Aout) {
 #pragma unroll
     for (int v = 0; v < kSrrmhdAux; ++v) a.Aout[v * vs + q] = A[v];

@@ -42,8 +42,8 @@ struct SweepArgs {
   const double *Vg;  // prims giving the cons2prim starting guess
   double *Uout, *Vout, *Aout;
   double *root;      // cons2prim root, kept when Aout is null: SRMHD (v^2, Z) [2][cell], SRRMHD Z [cell]
-  int vout_n;        // SRMHD: prims written to Vout (0: all 8; 5: rho .. p, for a stage whose
-                     // prims are read only by the next stage's sweeps, guess and retry)
+  int vout_n;        // prims written to Vout (0: all; 5: rho .. p, for a stage whose prims are
+                     // read only by the next stage's sweeps, guess and retry)
   double dt, gamma, sigma, cp;
   double *Pf;        // SSP2: U^n - dt/2 flux1 + dt/2 (source1 + source2), the part of the final
                      // combination (SSP2.cc:156-166) known after stage two
