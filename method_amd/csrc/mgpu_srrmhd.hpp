@@ -286,30 +286,72 @@ __device__ __forceinline__ bool imex_residual(const double c[14], double dtg, do
   return true;
 }
 
-// Newton with a forward-difference Jacobian and partial pivoting; steps that
-// leave the physical region are halved. Converged when the largest step is
-// <= 1e-14 max(1, |y|). Returns iterations or -1.
-__device__ __forceinline__ int imex_solve(const double c[14], double dtg, double gam, double sigma, double y[4]) {
-  double F[4], Fp[4], M[4][5];
-  for (int it = 0; it < 60; ++it) {
-    if (!imex_residual(c, dtg, gam, sigma, y, F)) return -1;
+// imex_residual and its Jacobian J[i][j] = dF_i/dy_j at y, in closed form:
+// with Sb = S - E x B, v = Sb / Z, W = (1 - v^2)^-1/2, tauBar = tau - (B^2 +
+// E^2)/2 and G = E + v x B - (v.E) v, the residual is F_E = E - c_E +
+// dtg (q v + sigma W G) and F_Z = (1 - g/(gam W^2)) Z + (g/(gam W) - 1) D -
+// tauBar (g = gam - 1); dv/dE_k = (dSb/dE_k)/Z, dv/dZ = -v/Z,
+// dW = W^3 (v.dv), dtauBar/dE_k = -E_k. F is formed exactly as
+// imex_residual forms it.
+__device__ __forceinline__ bool imex_residual_jac(const double c[14], double dtg, double gam, double sigma,
+                                                  const double y[4], double F[4], double J[4][4]) {
+  const double Ex = y[0], Ey = y[1], Ez = y[2], Z = y[3];
+  const double Bx = c[5], By = c[6], Bz = c[7];
+  const double Sbx = c[1] - (Ey * Bz - Ez * By);
+  const double Sby = c[2] - (Ez * Bx - Ex * Bz);
+  const double Sbz = c[3] - (Ex * By - Ey * Bx);
+  const double tauBar = c[4] - 0.5 * (Bx * Bx + By * By + Bz * Bz + Ex * Ex + Ey * Ey + Ez * Ez);
+  if (!(Z > 0.0)) return false;
+  const double iZ = 1.0 / Z;
+  const double vx = Sbx * iZ, vy = Sby * iZ, vz = Sbz * iZ;
+  const double vsq = vx * vx + vy * vy + vz * vz;
+  if (!(vsq < 1.0)) return false;
+  const double W = 1.0 / sqrt(1.0 - vsq);
+  F[3] = (1.0 - (gam - 1.0) / (W * W * gam)) * Z + ((gam - 1.0) / (W * gam) - 1.0) * c[0] - tauBar;
+  const double vE = vx * Ex + vy * Ey + vz * Ez;
+  const double q = c[13], ws = W * sigma;
+  const double G[3] = {Ex + (vy * Bz - vz * By) - vE * vx, Ey + (vz * Bx - vx * Bz) - vE * vy,
+                       Ez + (vx * By - vy * Bx) - vE * vz};
+  F[0] = Ex - c[8] + dtg * (q * vx + ws * (Ex + (vy * Bz - vz * By) - vE * vx));
+  F[1] = Ey - c[9] + dtg * (q * vy + ws * (Ey + (vz * Bx - vx * Bz) - vE * vy));
+  F[2] = Ez - c[10] + dtg * (q * vz + ws * (Ez + (vx * By - vy * Bx) - vE * vz));
+  const double v[3] = {vx, vy, vz}, E[3] = {Ex, Ey, Ez};
+  // dSb/dE_k: columns (0, Bz, -By), (-Bz, 0, Bx), (By, -Bx, 0)
+  const double dSb[3][3] = {{0.0, -Bz, By}, {Bz, 0.0, -Bx}, {-By, Bx, 0.0}};  // [component][k]
+  const double g = gam - 1.0, W2 = W * W, W3 = W2 * W;
+  const double cW = 2.0 * g / (W3 * gam) * Z - g / (W2 * gam) * c[0];  // dF_Z/dW
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double yj = y[j];
-      double h = 1e-7 * fmax(fabs(yj), 1e-3);
-      y[j] = yj + h;
-      if (!imex_residual(c, dtg, gam, sigma, y, Fp)) {
-        h = -h;
-        y[j] = yj + h;
-        if (!imex_residual(c, dtg, gam, sigma, y, Fp)) return -1;
-      }
-      y[j] = yj;
-      const double ih = 1.0 / h;
+  for (int a = 0; a < 4; ++a) {
+    double dv[3];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) M[i][j] = (Fp[i] - F[i]) * ih;
+    for (int i = 0; i < 3; ++i) dv[i] = a < 3 ? dSb[i][a] * iZ : -v[i] * iZ;
+    const double dW = W3 * (v[0] * dv[0] + v[1] * dv[1] + v[2] * dv[2]);
+    J[3][a] = cW * dW + (a == 3 ? 1.0 - g / (W2 * gam) : E[a]);
+    const double dvE = dv[0] * E[0] + dv[1] * E[1] + dv[2] * E[2] + (a < 3 ? v[a] : 0.0);
+    const double dvxB[3] = {dv[1] * Bz - dv[2] * By, dv[2] * Bx - dv[0] * Bz, dv[0] * By - dv[1] * Bx};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double dG = (a == k ? 1.0 : 0.0) + dvxB[k] - dvE * v[k] - vE * dv[k];
+      J[k][a] = (a == k ? 1.0 : 0.0) + dtg * (q * dv[k] + sigma * (dW * G[k] + W * dG));
     }
+  }
+  return true;
+}
+
+// Newton with the closed-form Jacobian and partial pivoting; steps that leave
+// the physical region are halved. Converged when the largest step is
+// <= 1e-14 max(1, |y|). Returns iterations or -1. The residual and Jacobian
+// the step search forms at the accepted point serve the next iteration.
+__device__ __forceinline__ int imex_solve(const double c[14], double dtg, double gam, double sigma, double y[4]) {
+  double F[4], J[4][4], M[4][5];
+  if (!imex_residual_jac(c, dtg, gam, sigma, y, F, J)) return -1;
+  for (int it = 0; it < 60; ++it) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) M[i][4] = -F[i];
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) M[i][j] = J[i][j];
+      M[i][4] = -F[i];
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       // partial pivoting by unrolled conditional row swaps (static indices)
@@ -347,15 +389,20 @@ __device__ __forceinline__ int imex_solve(const double c[14], double dtg, double
       dmax = fmax(dmax, fabs(d[j]));
     }
     double lam = 1.0, yn[4];
+    bool ok = false;
     for (int tries = 0; tries < 40; ++tries) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) yn[j] = y[j] + lam * d[j];
-      if (imex_residual(c, dtg, gam, sigma, yn, Fp)) break;
+      if (imex_residual_jac(c, dtg, gam, sigma, yn, F, J)) {
+        ok = true;
+        break;
+      }
       lam *= 0.5;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) y[j] = yn[j];
     if (dmax <= 1e-14 * ymax) return it + 1;
+    if (!ok) return -1;  // no physical point along the step
   }
   return -1;
 }
