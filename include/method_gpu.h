@@ -91,8 +91,9 @@ int mgpu_upload_state(mgpu_engine *e, const double *cons, const double *prims, c
 /* Device -> host state in the reference layout with ghost cells filled by the
  * engine's boundary conditions, as Bcs::apply leaves them
  * (boundaryConds.cc:5-145,449-730). Any pointer may be NULL. After SRMHD RK2
- * and RKSplit steps, which keep each cell's cons2prim root rather than its
- * aux, the aux is formed from that root first (srmhd.cc:509-557). */
+ * and RKSplit steps and SRRMHD SSP2 steps, which keep each cell's cons2prim
+ * root rather than its aux, the aux is formed from that root first
+ * (srmhd.cc:509-557, srrmhd.cc:539-581). */
 int mgpu_download_state(mgpu_engine *e, double *cons, double *prims, double *aux);
 
 /* Model::primsToAll over the resident primitives, then the boundary
