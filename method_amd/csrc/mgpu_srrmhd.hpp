@@ -339,11 +339,10 @@ __device__ __forceinline__ bool imex_residual_jac(const double c[14], double dtg
 }
 
 // Newton with the closed-form Jacobian and partial pivoting; steps that leave
-// the physical region are halved. Converged when the largest step is
-// <= 1e-14 max(1, |y|). Returns iterations or -1. The residual and Jacobian
+// the physical region are halved. Returns iterations or -1. The residual and Jacobian
 // the step search forms at the accepted point serve the next iteration.
 __device__ __forceinline__ int imex_solve(const double c[14], double dtg, double gam, double sigma, double y[4]) {
-  double F[4], J[4][4], M[4][5];
+  double F[4], J[4][4], M[4][5], dprev = HUGE_VAL;
   if (!imex_residual_jac(c, dtg, gam, sigma, y, F, J)) return -1;
   for (int it = 0; it < 60; ++it) {
 #pragma unroll
@@ -388,12 +387,16 @@ __device__ __forceinline__ int imex_solve(const double c[14], double dtg, double
       ymax = fmax(ymax, fabs(y[j]));
       dmax = fmax(dmax, fabs(d[j]));
     }
+    // Converged after this step: it is below 1e-14 |y|, or below 1e-8 |y| and a
+    // thousandth of the previous step, so that convergence is quadratic and
+    // the next correction would be below about 1e-14 |y| too.
+    const bool last = dmax <= 1e-14 * ymax || (dmax <= 1e-8 * ymax && dmax <= 1e-3 * dprev);
     double lam = 1.0, yn[4];
     bool ok = false;
     for (int tries = 0; tries < 40; ++tries) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) yn[j] = y[j] + lam * d[j];
-      if (imex_residual_jac(c, dtg, gam, sigma, yn, F, J)) {
+      if (last ? imex_residual(c, dtg, gam, sigma, yn, F) : imex_residual_jac(c, dtg, gam, sigma, yn, F, J)) {
         ok = true;
         break;
       }
@@ -401,8 +404,9 @@ __device__ __forceinline__ int imex_solve(const double c[14], double dtg, double
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) y[j] = yn[j];
-    if (dmax <= 1e-14 * ymax) return it + 1;
+    if (last) return it + 1;
     if (!ok) return -1;  // no physical point along the step
+    dprev = dmax;
   }
   return -1;
 }
