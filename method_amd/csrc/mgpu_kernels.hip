@@ -142,7 +142,8 @@ __global__ __launch_bounds__(kThreads) void k_flux_stored(Geo g, int model, cons
   }
 }
 
-template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_ = (DFAST ? 32 : 16), int TC_ = (DFAST ? 8 : 16)>
+template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_ = (DFAST ? 32 : 16), int TC_ = (DFAST ? 8 : 16),
+          bool SPLIT_ALL = false>
 struct SweepShape {
   static constexpr int R = Weno<ORDER>::R;
   static constexpr int TL = TL_;  // cells along the sweep axis
@@ -150,9 +151,12 @@ struct SweepShape {
   static constexpr int TLH = TL + 2 * R;      // with halo
   static constexpr int EAX = 3 - DAX - CAX;
   static constexpr int kFaces = (TL + 1) * TC;
-  // SRMHD stages f+ and then f- through LDS (kSplit); the resistive model's
-  // 14 variables would need too many registers for that and keeps both.
-  static constexpr bool kSplit = M::NC <= 9;
+  // SRMHD stages f+ and then f- through LDS (kSplit), which halves the LDS.
+  // The resistive model's 14 variables keep both unless SPLIT_ALL asks for
+  // the split: where the threads take the tile's cells in two passes the
+  // kept f- needs 28 more registers, and the fused y+z sweep with 8-pencil
+  // groups measured 1-9 % slower with it.
+  static constexpr bool kSplit = M::NC <= 9 || SPLIT_ALL;
   static constexpr int kLds = (kSplit ? 1 : 2) * M::NC * TLH * TC;
   // LDS position of (variable slot v, cell l along the sweep, pencil c): the
   // unit-stride index follows the threads' fast index, which is l when the
@@ -334,10 +338,11 @@ __device__ __forceinline__ void sweep_apply(const SweepArgs &a, long long q, con
 // values in S::gidx order: face lf of pencil cc is the left face of tile
 // cell lf. d0 and c0 are the tile's first cell along DAX and CAX, base_e the
 // offset of its plane along the third axis, nd and nc the grid's extents.
-template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_, int TC_, bool QUIRK, int NLDS>
+template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_, int TC_, bool QUIRK, bool SPLIT_ALL = false,
+          int NLDS>
 __device__ __forceinline__ void sweep_tile_faces(const SweepArgs &a, double (&s_f)[NLDS], int d0, int c0,
                                                  long long base_e, int nd, int nc) {
-  using S = SweepShape<M, ORDER, DAX, CAX, DFAST, TL_, TC_>;
+  using S = SweepShape<M, ORDER, DAX, CAX, DFAST, TL_, TC_, SPLIT_ALL>;
   constexpr int R = S::R, TC = S::TC, TLH = S::TLH;
   constexpr int NV = M::NC;
   const Geo &g = a.geo;
@@ -821,12 +826,16 @@ __global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
 // adds it (one fused multiply-add), and the x phase then applies the sweep's
 // mode; so the launch gives the separate sweeps' bits, and the y net flux
 // never makes an HBM round trip. The y phase runs along the contiguous axis.
-// As in k_sweep_yzf, SRRMHD takes each phase in two groups of 8 pencils.
+// As in k_sweep_yzf, SRRMHD takes each phase in two groups of 8 pencils; a
+// group's 160-176 tile-and-halo cells are one pass of the threads, so its
+// split fluxes go through LDS one half at a time as SRMHD's do (46.6-48.4 KB
+// instead of 64.5-68.2, 125-156 VGPRs instead of 130-182: 3 blocks per CU
+// instead of 2, and the OT 1024^2 steps 4-9 % faster).
 template <class M, int ORDER, int MODE>
 __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
   constexpr int kGroup = M::NC > 9 ? 8 : 16;  // pencils per group
-  using SY = SweepShape<M, ORDER, 1, 0, true, 16, kGroup>;
-  using SX = SweepShape<M, ORDER, 0, 1, false, 16, kGroup>;
+  using SY = SweepShape<M, ORDER, 1, 0, true, 16, kGroup, true>;
+  using SX = SweepShape<M, ORDER, 0, 1, false, 16, kGroup, true>;
   constexpr int NV = M::NC;
   constexpr int kLds = SY::kLds > SX::kLds ? SY::kLds : SX::kLds;
   __shared__ double s_f[kLds];
@@ -848,7 +857,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
   // whose cells belong to the 16 kGroup threads from 16 kGroup h
 #pragma unroll 1
   for (int h = 0; h < 16 / kGroup; ++h) {
-    sweep_tile_faces<M, ORDER, 1, 0, true, 16, kGroup, false>(a, s_f, y0, x0 + h * kGroup, 0, ny, nx);
+    sweep_tile_faces<M, ORDER, 1, 0, true, 16, kGroup, false, true>(a, s_f, y0, x0 + h * kGroup, 0, ny, nx);
     const int rg = r - h * kGroup;
     if (live && rg >= 0 && rg < kGroup) {
 #pragma unroll
@@ -873,7 +882,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
     // x phase: faces along x; group h holds the kGroup y pencils from y0 + kGroup h
 #pragma unroll 1
     for (int h = 0; h < 16 / kGroup; ++h) {
-      sweep_tile_faces<M, ORDER, 0, 1, false, 16, kGroup, false>(a, s_f, x0, y0 + h * kGroup, 0, nx, ny);
+      sweep_tile_faces<M, ORDER, 0, 1, false, 16, kGroup, false, true>(a, s_f, x0, y0 + h * kGroup, 0, nx, ny);
       const int cg = c - h * kGroup;
       if (live && cg >= 0 && cg < kGroup) {
 #pragma unroll
