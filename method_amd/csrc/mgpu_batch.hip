@@ -82,8 +82,9 @@ struct BatchShape {
   static constexpr int NV = kSrmhdCons;
   static constexpr int WY = TY + 2 * R;  // cells per row in the y phase
   static constexpr int HX = TX + 2 * R;  // rows in the x phase
-  static constexpr int kLdsY = 2 * NV * TX * WY;
-  static constexpr int kLdsX = 2 * NV * HX * TY;
+  // one half (f+, then f-) of the split fluxes at a time
+  static constexpr int kLdsY = NV * TX * WY;
+  static constexpr int kLdsX = NV * HX * TY;
   static constexpr int kLds = kLdsY > kLdsX ? kLdsY : kLdsX;
   static_assert(TX * TY == kThreads, "one cell per thread");
 };
@@ -106,13 +107,93 @@ __device__ __forceinline__ void split_flux(const BatchArgs &a, long long q, doub
   }
 }
 
+// The WENO face values of one tile phase along DIR, left in s_f[item] for
+// the NITEMS (variable, face) items. Phase 1 forms the split fluxes of the
+// NCELLS tile-and-halo cells (cell_q gives a cell's index, or -1 outside the
+// instance, where the fluxes are zero), puts f+ in LDS at s_f[v NCELLS +
+// idx] and keeps f- in registers; the upwind half of every item is formed
+// from f+ (item_at gives the LDS offset of its first stencil cell, STEP the
+// offset between stencil cells), f- takes f+'s place and the downwind halves
+// follow. Holding one half at a time halves the LDS of a workgroup, so more
+// of them share a CU; the items are dealt out one per thread per pass.
+template <int ORDER, int DIR, int NCELLS, int NITEMS, int STEP, class CellQ, class ItemAt>
+__device__ __forceinline__ void tile_faces(const BatchArgs &a, double *s_f, int tid, CellQ cell_q, ItemAt item_at) {
+  constexpr int NV = kSrmhdCons;
+  constexpr int kCellPasses = (NCELLS + kThreads - 1) / kThreads;
+  constexpr int kPasses = (NITEMS + kThreads - 1) / kThreads;
+  double fm_keep[kCellPasses][NV];
+#pragma unroll
+  for (int cp = 0; cp < kCellPasses; ++cp) {
+    const int idx = tid + cp * kThreads;
+    if (idx < NCELLS) {
+      double fp[NV];
+      const long long q = cell_q(idx);
+      if (q >= 0) {
+        split_flux<DIR>(a, q, fp, fm_keep[cp]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fp[v] = fm_keep[cp][v] = 0.0;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_f[v * NCELLS + idx] = fp[v];
+    }
+  }
+  // the kept f- values are formed before the barrier
+#pragma unroll
+  for (int cp = 0; cp < kCellPasses; ++cp)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) asm volatile("" : "+v"(fm_keep[cp][v]));
+  __syncthreads();
+  double nu[kPasses], du[kPasses];
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < NITEMS) {
+      const double *fpv = s_f + item_at(item);
+      Weno<ORDER>::up([&](int m) { return fpv[m * STEP]; }, nu[k], du[k]);
+    }
+  }
+  // the upwind halves are formed before the barrier after which f- overwrites f+
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) asm volatile("" : "+v"(nu[k]), "+v"(du[k]));
+  __syncthreads();
+#pragma unroll
+  for (int cp = 0; cp < kCellPasses; ++cp) {
+    const int idx = tid + cp * kThreads;
+    if (idx < NCELLS) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_f[v * NCELLS + idx] = fm_keep[cp][v];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < NITEMS) {
+      const double *fmv = s_f + item_at(item);
+      double nd, dd;
+      Weno<ORDER>::down([&](int m) { return fmv[m * STEP]; }, nd, dd);
+      nu[k] = weno_combine(nu[k], du[k], nd, dd);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) asm volatile("" : "+v"(nu[k]));
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) {
+    const int item = tid + k * kThreads;
+    if (item < NITEMS) s_f[item] = nu[k];
+  }
+  __syncthreads();
+}
+
 struct Acc {
   int cells = 0, iters = 0, fails = 0, maxit = 0;
 };
 
 // One tile of one instance: net flux, stage update, cons2prim, store.
 template <int ORDER, int TX, int TY, int STAGE>
-__device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, int t, Acc &acc) {
+__device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, double *s_acc, int t, Acc &acc) {
   using S = BatchShape<ORDER, TX, TY>;
   constexpr int R = S::R, NV = S::NV, WY = S::WY, HX = S::HX;
   const int b = a.tile_inst[t];
@@ -123,100 +204,56 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, int 
   const int ty = local % I.tiles_y, tx = local / I.tiles_y;
   const int x0 = tx * TX, y0 = ty * TY;
   const long long vs = a.vs;
-  const int tid = threadIdx.x;
+  // The thread index as a value the compiler cannot see through: otherwise
+  // it hoists every tile-independent LDS offset of the face phases out of the
+  // tile loop and keeps them in registers across the cons2prim.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   const int r = tid / TY, c = tid % TY;  // the thread's cell in the tile
   __syncthreads();                       // the previous tile is done with LDS
 
-  // ---- y faces: rows x0 .. x0+TX-1, cells y0-R .. y0+TY+R-1
-  for (int idx = tid; idx < TX * WY; idx += kThreads) {
-    const int rr = idx / WY, cy = idx - rr * WY;
-    const int gx = x0 + rr;
-    double fp[NV], fm[NV];
-    if (gx < I.nx) {
-      const int gy = remap(min(y0 + cy - R, I.ny - 1 + R), I.ny, a.py);
-      split_flux<1>(a, I.off + (long long)gx * I.ny + gy, fp, fm);
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      s_f[(v * TX + rr) * WY + cy] = fp[v];
-      s_f[((NV + v) * TX + rr) * WY + cy] = fm[v];
-    }
-  }
-  __syncthreads();
-  // (variable, face) pairs dealt out one per thread per pass; the face of
-  // item = v * faces + (rr, lf) goes to s_f[item]
-  constexpr int kFacesY = TX * (TY + 1), kItemsY = NV * kFacesY;
-  constexpr int kPassesY = (kItemsY + kThreads - 1) / kThreads;
-  double face_y[kPassesY];
-#pragma unroll
-  for (int k = 0; k < kPassesY; ++k) {
-    const int item = tid + k * kThreads;
-    if (item < kItemsY) {
-      const int v = item / kFacesY, idx = item - v * kFacesY;
-      const int rr = idx / (TY + 1), lf = idx - rr * (TY + 1);
-      const double *fpv = s_f + (v * TX + rr) * WY + lf;
-      const double *fmv = s_f + ((NV + v) * TX + rr) * WY + lf;
-      face_y[k] = Weno<ORDER>::face([&](int m) { return fpv[m]; }, [&](int m) { return fmv[m]; });
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kPassesY; ++k) {
-    const int item = tid + k * kThreads;
-    if (item < kItemsY) s_f[item] = face_y[k];
-  }
-  __syncthreads();
-  double fny[NV];
+  // ---- y faces: rows x0 .. x0+TX-1, cells y0-R .. y0+TY+R-1; cell idx = rr WY + cy.
+  // The stencil of face lf of row rr (the left face of tile cell lf) starts
+  // at cell lf of the row.
+  constexpr int kFacesY = TX * (TY + 1);
+  tile_faces<ORDER, 1, TX * WY, NV * kFacesY, 1>(
+      a, s_f, tid,
+      [&](int idx) -> long long {
+        const int rr = idx / WY, cy = idx - rr * WY;
+        const int gx = x0 + rr;
+        if (gx >= I.nx) return -1;
+        const int gy = remap(min(y0 + cy - R, I.ny - 1 + R), I.ny, a.py);
+        return I.off + (long long)gx * I.ny + gy;
+      },
+      [&](int item) {
+        const int v = item / kFacesY, idx = item - v * kFacesY;
+        const int rr = idx / (TY + 1), lf = idx - rr * (TY + 1);
+        return (v * TX + rr) * WY + lf;
+      });
+  // the y net flux waits in LDS for the x term (registers are the scarcer)
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const double *g = s_f + (v * TX + r) * (TY + 1) + c;
-    fny[v] = (g[1] - g[0]) * I.inv_dy;
+    s_acc[v * kThreads + tid] = (g[1] - g[0]) * I.inv_dy;
   }
   __syncthreads();
 
-  // ---- x faces: rows x0-R .. x0+TX+R-1, cells y0 .. y0+TY-1
-  for (int idx = tid; idx < HX * TY; idx += kThreads) {
-    const int rx = idx / TY, cc = idx - rx * TY;
-    const int gy = y0 + cc;
-    double fp[NV], fm[NV];
-    if (gy < I.ny) {
-      const int gx = remap(min(x0 + rx - R, I.nx - 1 + R), I.nx, a.px);
-      split_flux<0>(a, I.off + (long long)gx * I.ny + gy, fp, fm);
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) fp[v] = fm[v] = 0.0;
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      s_f[(v * HX + rx) * TY + cc] = fp[v];
-      s_f[((NV + v) * HX + rx) * TY + cc] = fm[v];
-    }
-  }
-  __syncthreads();
-  constexpr int kFacesX = (TX + 1) * TY, kItemsX = NV * kFacesX;
-  constexpr int kPassesX = (kItemsX + kThreads - 1) / kThreads;
-  double face_x[kPassesX];
-#pragma unroll
-  for (int k = 0; k < kPassesX; ++k) {
-    const int item = tid + k * kThreads;
-    if (item < kItemsX) {
-      const int v = item / kFacesX, idx = item - v * kFacesX;
-      const int lf = idx / TY, cc = idx - lf * TY;
-      const double *fpv = s_f + (v * HX + lf) * TY + cc;
-      const double *fmv = s_f + ((NV + v) * HX + lf) * TY + cc;
-      face_x[k] = Weno<ORDER>::face([&](int m) { return fpv[m * TY]; }, [&](int m) { return fmv[m * TY]; });
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kPassesX; ++k) {
-    const int item = tid + k * kThreads;
-    if (item < kItemsX) s_f[item] = face_x[k];
-  }
-  __syncthreads();
+  // ---- x faces: rows x0-R .. x0+TX+R-1, cells y0 .. y0+TY-1; cell idx = rx TY + cc
+  constexpr int kFacesX = (TX + 1) * TY;
+  tile_faces<ORDER, 0, HX * TY, NV * kFacesX, TY>(
+      a, s_f, tid,
+      [&](int idx) -> long long {
+        const int rx = idx / TY, cc = idx - rx * TY;
+        const int gy = y0 + cc;
+        if (gy >= I.ny) return -1;
+        const int gx = remap(min(x0 + rx - R, I.nx - 1 + R), I.nx, a.px);
+        return I.off + (long long)gx * I.ny + gy;
+      },
+      [&](int item) {
+        const int v = item / kFacesX, idx = item - v * kFacesX;
+        const int lf = idx / TY, cc = idx - lf * TY;
+        return (v * HX + lf) * TY + cc;
+      });
 
   const int gx = x0 + r, gy = y0 + c;
   if (gx >= I.nx || gy >= I.ny) return;
@@ -225,7 +262,7 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, int 
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const double *g = s_f + (v * (TX + 1) + r) * TY + c;
-    const double fn = fma(g[TY] - g[0], I.inv_dx, fny[v]);
+    const double fn = fma(g[TY] - g[0], I.inv_dx, s_acc[v * kThreads + tid]);
     un[v] = STAGE == 1 ? rk2_stage1(a.U[v * vs + q], fn, dt) : rk2_stage2(a.Un[v * vs + q], a.U[v * vs + q], fn, dt);
   }
   // TimeIntegrator::finalise's cons2prim (timeInt.h:117-132, srmhd.cc:390-562)
@@ -366,6 +403,7 @@ __device__ __forceinline__ void xcd_range(int ntiles, int x, int &lo, int &hi) {
 template <int ORDER, int TX, int TY, int STAGE>
 __global__ __launch_bounds__(kThreads, 3) void k_batch_stage(BatchArgs a) {
   __shared__ double s_f[BatchShape<ORDER, TX, TY>::kLds];
+  __shared__ double s_acc[kSrmhdCons * kThreads];
   __shared__ int s_tile, s_last;
   Acc acc;
   const int x = blockIdx.x % kXcds;
@@ -373,7 +411,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_batch_stage(BatchArgs a) {
     int lo, hi;
     xcd_range(a.ntiles, x, lo, hi);
     const int nbx = ((int)gridDim.x - x + kXcds - 1) / kXcds;
-    for (int t = lo + (int)blockIdx.x / kXcds; t < hi; t += nbx) batch_tile<ORDER, TX, TY, STAGE>(a, s_f, t, acc);
+    for (int t = lo + (int)blockIdx.x / kXcds; t < hi; t += nbx) batch_tile<ORDER, TX, TY, STAGE>(a, s_f, s_acc, t, acc);
   } else {
     for (int k = 0; k < kXcds; ++k) {
       const int xx = (x + k) % kXcds;
@@ -385,7 +423,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_batch_stage(BatchArgs a) {
         __syncthreads();
         const int t = s_tile;
         if (t >= hi) break;
-        batch_tile<ORDER, TX, TY, STAGE>(a, s_f, t, acc);
+        batch_tile<ORDER, TX, TY, STAGE>(a, s_f, s_acc, t, acc);
       }
     }
   }
