@@ -337,17 +337,17 @@ __device__ __forceinline__ void sweep_apply(const SweepArgs &a, long long q, con
 // every face of the tile. On return (after a barrier) s_f holds the face
 // values in S::gidx order: face lf of pencil cc is the left face of tile
 // cell lf. d0 and c0 are the tile's first cell along DAX and CAX, base_e the
-// offset of its plane along the third axis, nd and nc the grid's extents.
+// offset of its plane along the third axis, nd and nc the grid's extents,
+// tid the thread's index in the workgroup.
 template <class M, int ORDER, int DAX, int CAX, bool DFAST, int TL_, int TC_, bool QUIRK, bool SPLIT_ALL = false,
           int NLDS>
 __device__ __forceinline__ void sweep_tile_faces(const SweepArgs &a, double (&s_f)[NLDS], int d0, int c0,
-                                                 long long base_e, int nd, int nc) {
+                                                 long long base_e, int nd, int nc, unsigned tid) {
   using S = SweepShape<M, ORDER, DAX, CAX, DFAST, TL_, TC_, SPLIT_ALL>;
   constexpr int R = S::R, TC = S::TC, TLH = S::TLH;
   constexpr int NV = M::NC;
   const Geo &g = a.geo;
   const long long vs = g.vstride;
-  const unsigned tid = threadIdx.x;
 
   if constexpr (S::kSplit) {
     // Phase 1: split fluxes of every tile cell and its halo along DAX. f+ goes
@@ -723,25 +723,26 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
 // The y and z terms of a 3-D stage in one launch, after an x sweep has
 // written the x term (sweep plan 3): a 16 x 16 (y, z) tile of one x plane
 // reconstructs its y faces, then its z faces, each phase exactly as a
-// separate sweep does it. Each cell's net flux with the y term added waits in
-// LDS for the z term, and the z phase then applies the sweep's mode. The sums
+// separate sweep does it. Each cell's net flux with the y term added waits
+// (in registers or LDS) for the z term, and the z phase then applies the
+// sweep's mode. The sums
 // are formed in the separate sweeps' order, so plan 3 gives plan 2's bits,
 // and the y-added net flux never makes an HBM round trip. SRRMHD's 14
 // variables take each phase over the tile's pencils in two groups of 8, so
 // that a thread keeps the split fluxes of one tile-or-halo cell and the face
 // values of about five (variable, face) items at a time (174-203 VGPRs where
 // the whole tile takes 184-256); SRMHD's 9 variables run faster as one group
-// of 16 (123-156 VGPRs, 3 waves per SIMD): in groups of 8 they fit 4 waves
-// per SIMD but took 1 % longer.
+// of 16, with each cell's y-added net flux kept in registers instead of LDS:
+// 116-128 VGPRs and 23-25 KB of LDS, 4 waves per SIMD (in groups of 8 they
+// fit 4 waves per SIMD too but took 1 % longer).
 template <class M, int ORDER, int MODE, bool QUIRK>
-__global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
+__global__ __launch_bounds__(kThreads, M::NC > 9 ? 1 : 4) void k_sweep_yzf(SweepArgs a) {
   constexpr int kGroup = M::NC > 9 ? 8 : 16;  // pencils per group
   using SY = SweepShape<M, ORDER, 1, 2, false, 16, kGroup>;
   using SZ = SweepShape<M, ORDER, 2, 1, true, 16, kGroup>;
   constexpr int NV = M::NC;
   constexpr int kLds = SY::kLds > SZ::kLds ? SY::kLds : SZ::kLds;
   __shared__ double s_f[kLds];
-  __shared__ double s_acc[NV][kThreads];  // each cell's net flux, y term added, then z
   const Geo &g = a.geo;
   const int ny = g.n[1], nz = g.n[2];
   const int tiles_y = (ny + 15) / 16, tiles_z = (nz + 15) / 16;
@@ -762,30 +763,41 @@ __global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
   CellC2P cs;
   if constexpr (kGroup == 16) {
     // y phase over the whole tile, pencils along z
-    sweep_tile_faces<M, ORDER, 1, 2, false, 16, 16, false>(a, s_f, y0, z0, base_e, ny, nz);
+    // each cell's net flux with the y term added waits in registers
+    unsigned ty = tid;
+    asm volatile("" : "+v"(ty));
+    sweep_tile_faces<M, ORDER, 1, 2, false, 16, 16, false>(a, s_f, y0, z0, base_e, ny, nz, ty);
+    double acc[NV];
     if (live) {
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        s_acc[v][tid] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1], a.fnet[v * vs + q]);
+        acc[v] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1], a.fnet[v * vs + q]);
     }
     __syncthreads();  // every y face is read before the z phase refills LDS
     // the z phase's loads stay after the y phase, so its registers are free
     __builtin_amdgcn_sched_barrier(0);
+    // Each phase takes the thread index as a value the compiler cannot see
+    // through, so that it neither forms the z phase's LDS offsets early nor
+    // shares them with the y phase's, which kept them in registers across
+    // both phases (156 VGPRs instead of 128).
+    unsigned tz = tid;
+    asm volatile("" : "+v"(tz));
     // z phase, pencils along y: the thread's cell is tile cell c of pencil r,
     // and it finishes straight from the faces in LDS
-    sweep_tile_faces<M, ORDER, 2, 1, true, 16, 16, QUIRK>(a, s_f, z0, y0, base_e, nz, ny);
+    sweep_tile_faces<M, ORDER, 2, 1, true, 16, 16, QUIRK>(a, s_f, z0, y0, base_e, nz, ny, tz);
     if (live) {
       double fn[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        fn[v] = fma(s_f[SZ::gidx(v, c + 1, r)] - s_f[SZ::gidx(v, c, r)], g.inv_d[2], s_acc[v][tid]);
+        fn[v] = fma(s_f[SZ::gidx(v, c + 1, r)] - s_f[SZ::gidx(v, c, r)], g.inv_d[2], acc[v]);
       sweep_apply<M, MODE>(a, q, fn, cs);
     }
   } else {
+    __shared__ double s_acc[NV][kThreads];  // each cell's net flux, y term added, then z
     // y phase: faces along y; group h holds the kGroup pencils from z0 + kGroup h
 #pragma unroll 1
     for (int h = 0; h < 16 / kGroup; ++h) {
-      sweep_tile_faces<M, ORDER, 1, 2, false, 16, kGroup, false>(a, s_f, y0, z0 + h * kGroup, base_e, ny, nz);
+      sweep_tile_faces<M, ORDER, 1, 2, false, 16, kGroup, false>(a, s_f, y0, z0 + h * kGroup, base_e, ny, nz, tid);
       const int cg = c - h * kGroup;
       if (live && cg >= 0 && cg < kGroup) {
 #pragma unroll
@@ -799,7 +811,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_yzf(SweepArgs a) {
     // whose cells belong to the 16 kGroup threads from 16 kGroup h
 #pragma unroll 1
     for (int h = 0; h < 16 / kGroup; ++h) {
-      sweep_tile_faces<M, ORDER, 2, 1, true, 16, kGroup, QUIRK>(a, s_f, z0, y0 + h * kGroup, base_e, nz, ny);
+      sweep_tile_faces<M, ORDER, 2, 1, true, 16, kGroup, QUIRK>(a, s_f, z0, y0 + h * kGroup, base_e, nz, ny, tid);
       const int rg = r - h * kGroup;
       if (live && rg >= 0 && rg < kGroup) {
 #pragma unroll
@@ -857,7 +869,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
   // whose cells belong to the 16 kGroup threads from 16 kGroup h
 #pragma unroll 1
   for (int h = 0; h < 16 / kGroup; ++h) {
-    sweep_tile_faces<M, ORDER, 1, 0, true, 16, kGroup, false, true>(a, s_f, y0, x0 + h * kGroup, 0, ny, nx);
+    sweep_tile_faces<M, ORDER, 1, 0, true, 16, kGroup, false, true>(a, s_f, y0, x0 + h * kGroup, 0, ny, nx, tid);
     const int rg = r - h * kGroup;
     if (live && rg >= 0 && rg < kGroup) {
 #pragma unroll
@@ -870,7 +882,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
   if constexpr (kGroup == 16) {
     // x phase over the whole tile, pencils along y: the thread's cell is tile
     // cell r of pencil c, and it finishes straight from the faces in LDS
-    sweep_tile_faces<M, ORDER, 0, 1, false, 16, 16, false>(a, s_f, x0, y0, 0, nx, ny);
+    sweep_tile_faces<M, ORDER, 0, 1, false, 16, 16, false>(a, s_f, x0, y0, 0, nx, ny, tid);
     if (live) {
       double fn[NV];
 #pragma unroll
@@ -882,7 +894,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
     // x phase: faces along x; group h holds the kGroup y pencils from y0 + kGroup h
 #pragma unroll 1
     for (int h = 0; h < 16 / kGroup; ++h) {
-      sweep_tile_faces<M, ORDER, 0, 1, false, 16, kGroup, false, true>(a, s_f, x0, y0 + h * kGroup, 0, nx, ny);
+      sweep_tile_faces<M, ORDER, 0, 1, false, 16, kGroup, false, true>(a, s_f, x0, y0 + h * kGroup, 0, nx, ny, tid);
       const int cg = c - h * kGroup;
       if (live && cg >= 0 && cg < kGroup) {
 #pragma unroll
