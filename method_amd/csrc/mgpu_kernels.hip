@@ -842,7 +842,7 @@ __global__ __launch_bounds__(kThreads, M::NC > 9 ? 1 : 4) void k_sweep_yzf(Sweep
 // group's 160-176 tile-and-halo cells are one pass of the threads, so its
 // split fluxes go through LDS one half at a time as SRMHD's do (46.6-48.4 KB
 // instead of 64.5-68.2, 125-156 VGPRs instead of 130-182: 3 blocks per CU
-// instead of 2, and the OT 1024^2 steps 4-9 % faster).
+// instead of 2, and the OT 1024^2 steps 3-9 % faster).
 template <class M, int ORDER, int MODE>
 __global__ __launch_bounds__(kThreads) void k_sweep_xyf(SweepArgs a) {
   constexpr int kGroup = M::NC > 9 ? 8 : 16;  // pencils per group
