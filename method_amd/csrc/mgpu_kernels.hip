@@ -208,18 +208,21 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
   const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
   double P[kSrmhdPrims], A[kSrmhdAux];
   srmhd_recover(un, a.gamma, v2, Z, P, A);
+  // the stage's outputs stream out (non-temporal stores: the next launch
+  // reads them, but they do not fit in the caches and would only evict the
+  // tiles' halo cells from the L2)
 #pragma unroll
-  for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+  for (int v = 0; v < kSrmhdCons; ++v) __builtin_nontemporal_store(un[v], &a.Uout[v * vs + q]);
   const int nvout = a.vout_n ? a.vout_n : kSrmhdPrims;
 #pragma unroll
   for (int v = 0; v < kSrmhdPrims; ++v)
-    if (v < nvout) a.Vout[v * vs + q] = P[v];
+    if (v < nvout) __builtin_nontemporal_store(P[v], &a.Vout[v * vs + q]);
   if (a.Aout) {
 #pragma unroll
     for (int v = 0; v < kSrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
   } else if (a.root) {
-    a.root[q] = v2;
-    a.root[vs + q] = Z;
+    __builtin_nontemporal_store(v2, &a.root[q]);
+    __builtin_nontemporal_store(Z, &a.root[vs + q]);
   }
   cs.add(its);
   if (its < 0) {
@@ -690,9 +693,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs a) {
     // multiply-add so every kernel that accumulates it rounds alike
     double fn[NV];
     if (MODE == kSweepWrite) {
+      // streamed out, as the stage outputs are (finish_rk2)
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        a.fnet[v * vs + q] = (s_f[S::gidx(v, l + 1, cc)] - s_f[S::gidx(v, l, cc)]) * inv_d;
+        __builtin_nontemporal_store((s_f[S::gidx(v, l + 1, cc)] - s_f[S::gidx(v, l, cc)]) * inv_d, &a.fnet[v * vs + q]);
     } else {
 #pragma unroll
       for (int v = 0; v < NV; ++v)
