@@ -275,14 +275,16 @@ __device__ __forceinline__ void batch_tile(const BatchArgs &a, double *s_f, doub
   const int its = srmhd_newton(un[0], un[4], Bsq, Ssq, BS, a.gamma, v2, Z);
   double P[kSrmhdPrims], A[kSrmhdAux];  // A unused: the aux is formed from the root when read
   srmhd_recover(un, a.gamma, v2, Z, P, A);
+  // streamed out with non-temporal stores: they do not fit in the caches and
+  // would only evict the tiles' halo cells from the L2
 #pragma unroll
-  for (int v = 0; v < kSrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+  for (int v = 0; v < kSrmhdCons; ++v) __builtin_nontemporal_store(un[v], &a.Uout[v * vs + q]);
   // stage one: rho .. p of V1, all that stage two's sweeps, guess and retry read
 #pragma unroll
-  for (int v = 0; v < (STAGE == 1 ? 5 : kSrmhdPrims); ++v) a.Vout[v * vs + q] = P[v];
+  for (int v = 0; v < (STAGE == 1 ? 5 : kSrmhdPrims); ++v) __builtin_nontemporal_store(P[v], &a.Vout[v * vs + q]);
   if (STAGE == 2) {
-    a.root[q] = v2;
-    a.root[vs + q] = Z;
+    __builtin_nontemporal_store(v2, &a.root[q]);
+    __builtin_nontemporal_store(Z, &a.root[vs + q]);
   }
   acc.cells += 1;
   if (its < 0) {

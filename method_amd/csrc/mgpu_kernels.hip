@@ -241,17 +241,18 @@ __device__ __forceinline__ void c2p_store_srrmhd(const SweepArgs &a, long long q
                                    a.Vg[4 * vs + q], a.gamma);
   double P[kSrrmhdPrims];
   const int its = srrmhd_c2p_cell(un, a.gamma, a.sigma, Zg, P, A);
+  // streamed out, as finish_rk2 does for SRMHD
 #pragma unroll
-  for (int v = 0; v < kSrrmhdCons; ++v) a.Uout[v * vs + q] = un[v];
+  for (int v = 0; v < kSrrmhdCons; ++v) __builtin_nontemporal_store(un[v], &a.Uout[v * vs + q]);
   const int nvout = a.vout_n ? a.vout_n : kSrrmhdPrims;
 #pragma unroll
   for (int v = 0; v < kSrrmhdPrims; ++v)
-    if (v < nvout) a.Vout[v * vs + q] = P[v];
+    if (v < nvout) __builtin_nontemporal_store(P[v], &a.Vout[v * vs + q]);
   if (a.Aout) {
 #pragma unroll
     for (int v = 0; v < kSrrmhdAux; ++v) a.Aout[v * vs + q] = A[v];
   } else if (a.root) {
-    a.root[q] = A[10];  // Z: the aux follows from it and the cons (srrmhd_recover)
+    __builtin_nontemporal_store(A[10], &a.root[q]);  // Z: the aux follows from it and the cons (srrmhd_recover)
   }
   cs.add(its);
 }
@@ -291,7 +292,7 @@ __device__ __forceinline__ void imex2_apply(const SweepArgs &a, long long q, con
 #pragma unroll
   for (int m = 0; m < 5; ++m) pf[8 + m] += 0.5 * a.dt * (s1[m] + s5[m]);
 #pragma unroll
-  for (int v = 0; v < kSrrmhdCons; ++v) a.Pf[v * vs + q] = pf[v];
+  for (int v = 0; v < kSrrmhdCons; ++v) __builtin_nontemporal_store(pf[v], &a.Pf[v * vs + q]);
 }
 
 // SSP2's final combination at one cell, cons += -dt/2 (flux1 + flux2 -
@@ -1410,7 +1411,7 @@ __global__ __launch_bounds__(kThreads) void k_imex1_srrmhd(SweepArgs a) {
     double s5[5];
     srrmhd_source5(x, A, a.cp, s5);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) a.Sout[m * vs + q] = s5[m];
+    for (int m = 0; m < 5; ++m) __builtin_nontemporal_store(s5[m], &a.Sout[m * vs + q]);
   }
   reduce_stats(a.stats, cs.cells, cs.iters, cs.fails, cs.maxit);
 }
