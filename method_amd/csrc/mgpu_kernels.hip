@@ -200,8 +200,11 @@ __device__ __forceinline__ void finish_rk2(const SweepArgs &a, long long q, cons
                                            CellC2P &cs) {
   const long long vs = a.geo.vstride;
   double v2, Z;
-  srmhd_plain_guess(a.Vg[0 * vs + q], a.Vg[1 * vs + q], a.Vg[2 * vs + q], a.Vg[3 * vs + q], a.Vg[4 * vs + q],
-                    a.gamma, v2, Z);
+  // the guess is read once per cell: non-temporal loads keep it from evicting
+  // the tiles' halo cells from the L2
+  srmhd_plain_guess(__builtin_nontemporal_load(&a.Vg[0 * vs + q]), __builtin_nontemporal_load(&a.Vg[1 * vs + q]),
+                    __builtin_nontemporal_load(&a.Vg[2 * vs + q]), __builtin_nontemporal_load(&a.Vg[3 * vs + q]),
+                    __builtin_nontemporal_load(&a.Vg[4 * vs + q]), a.gamma, v2, Z);
   const double BS = un[5] * un[1] + un[6] * un[2] + un[7] * un[3];
   const double Bsq = un[5] * un[5] + un[6] * un[6] + un[7] * un[7];
   const double Ssq = un[1] * un[1] + un[2] * un[2] + un[3] * un[3];
@@ -326,7 +329,8 @@ __device__ __forceinline__ void sweep_apply(const SweepArgs &a, long long q, con
       for (int v = 0; v < NV; ++v) un[v] = rk2_stage1(a.U[v * vs + q], fn[v], a.dt);
     } else {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) un[v] = rk2_stage2(a.Un[v * vs + q], a.U[v * vs + q], fn[v], a.dt);
+      for (int v = 0; v < NV; ++v)
+        un[v] = rk2_stage2(__builtin_nontemporal_load(&a.Un[v * vs + q]), a.U[v * vs + q], fn[v], a.dt);
     }
     finish_rk2(a, q, un, M(), cs);
   } else if (MODE == kSweepImex2) {
@@ -776,7 +780,8 @@ __global__ __launch_bounds__(kThreads, M::NC > 9 ? 1 : 4) void k_sweep_yzf(Sweep
     if (live) {
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        acc[v] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1], a.fnet[v * vs + q]);
+        acc[v] = fma(s_f[SY::gidx(v, r + 1, c)] - s_f[SY::gidx(v, r, c)], g.inv_d[1],
+                     __builtin_nontemporal_load(&a.fnet[v * vs + q]));  // read once
     }
     __syncthreads();  // every y face is read before the z phase refills LDS
     // the z phase's loads stay after the y phase, so its registers are free
